@@ -1,0 +1,174 @@
+/*
+ * mf_abi.h -- C ABI of the MI355X multifit engine (libmf_amd.so).
+ *
+ * Plain C, plain pointers and sizes.  Device buffers are HIP device pointers,
+ * streams are hipStream_t passed as void* (NULL = the null stream).  Every
+ * entry point returns an mf_status; mf_last_error() gives the message of the
+ * last failure on the calling thread.  Nothing here allocates on a caller's
+ * behalf except mf_context_create / mf_set_types (type tables) and the
+ * scratch of mf_fit_batch (kept in the context, grown on demand).
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * reference repository root):
+ *   mf_fit_batch ............. doFitWrapper (multifit_gpu/interface/PulseChiSqSNNLSWrapper.h:11,
+ *                              multifit_gpu/src/PulseChiSqSNNLSWrapper.cu:8-52) and the
+ *                              per-channel PulseChiSqSNNLS::DoFit it runs
+ *                              (multifit_cpu/interface/PulseChiSqSNNLS.h:13-18)
+ *   mf_inplace_fnnls_batch ... inplace_fnnls / inplace_fnnls_kernel
+ *                              (nnls/interface/inplace_fnnls.h:9-14, nnls/interface/kernel_wrapper.h:7-12)
+ *   mf_fnnls_batch ........... fnnls / fnnls_kernel / fnnls_wrapper
+ *                              (nnls/interface/fnnls.h:9-14, nnls/interface/kernel_wrapper.h:14-24)
+ *   mf_linalg_batch .......... cholesky_decomp, solve_forward_substitution,
+ *                              solve_backward_substitution, swap_row_column,
+ *                              fused_cholesky_forward_substitution_solver_{rcaddition,inbetween_removal}
+ *                              (linalgebra/interface/{cholesky,substitution,common,fused_cholesky_substitution}.hpp)
+ */
+#ifndef MF_ABI_H
+#define MF_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MF_ABI_VERSION 1
+#define MF_MAX_N 10   /* samples and pulses per channel */
+#define MF_FULL 19    /* FullSampleVector length (EigenMatrixTypes.h:7) */
+
+typedef enum {
+    MF_OK = 0,
+    MF_ERR_INVALID = 1,     /* bad argument (sizes, NULL, unsupported shape) */
+    MF_ERR_HIP = 2,         /* a HIP runtime call failed */
+    MF_ERR_NOMEM = 3,       /* device allocation failed */
+    MF_ERR_UNSUPPORTED = 4  /* option combination not built */
+} mf_status;
+
+/* One channel type: the per-channel constants of DoFit
+ * (PulseChiSqSNNLS.h:13-18: samplecor, pederr, bxs, fullpulse, fullpulsecov).
+ * The pulse matrix column of pulse j holds fullpulse[f_j + o_j ...] in rows
+ * f_j .. nsamples-1 with f_j = max(0, bx_j + first_off), o_j = pulse_off - bx_j
+ * (PulseChiSqSNNLS.cpp:50-62: first_off = 3, pulse_off = 4 for ECAL).
+ * Matrices are row-major: noisecor[i*10+j], fullpulsecov[i*19+j].
+ * Layout is identical to the oracle's or_chtype. */
+typedef struct mf_chtype {
+    int32_t nsamples;          /* 8 or 10                                  */
+    int32_t npulses;           /* == nsamples (8x8 or 10x10)               */
+    int32_t bx[MF_MAX_N];      /* BX offset of each pulse                  */
+    int32_t first_off;
+    int32_t pulse_off;
+    double  pederr;            /* noise sigma                              */
+    double  noisecor[MF_MAX_N * MF_MAX_N];
+    double  fullpulse[MF_FULL];
+    double  fullpulsecov[MF_FULL * MF_FULL];
+} mf_chtype;
+
+/* precision / algorithm of mf_fit_batch */
+enum {
+    MF_PREC_FP64 = 0,  /* plain cold-start FP64 fit with the reference's
+                          semantics (multifit_gpu: FNNLS max_iter 1000)     */
+    MF_PREC_FP32 = 1   /* FP32 bulk, incremental Cholesky, FP64 pivot
+                          fallback and FP64 refinement residual            */
+};
+enum { MF_SOLVER_INPLACE_FNNLS = 0, MF_SOLVER_FNNLS = 1 };
+enum { MF_DTYPE_F64 = 0, MF_DTYPE_F32 = 1 };
+
+typedef struct mf_opts {
+    int32_t precision;      /* MF_PREC_*                                      */
+    int32_t solver;         /* MF_SOLVER_* (FP64 only; FP32 is inplace)       */
+    int32_t max_iter;       /* FNNLS outer cap (reference GPU: 1000, CPU: 10) */
+    int32_t max_minimize;   /* Minimize cap (reference: 50)                   */
+    double  eps;            /* FNNLS max-w threshold (reference: 1e-11)       */
+    double  chi2_tol;       /* Minimize |dchi2| threshold (reference: 1e-3)   */
+    int32_t use_pulse_cov;  /* 1: add x_j^2 * fullpulsecov blocks (updateCov) */
+    int32_t warm_start;     /* FP32 only: 1 = learned warm start (needs weights) */
+    int32_t persistent;     /* 1: persistent kernel + device work queue       */
+    int32_t block;          /* threads per block (0 = tuned default)          */
+} mf_opts;
+
+/* Per-channel flags (mf_result.flags). */
+enum {
+    MF_F_MINIMIZE_CAP = 1,   /* Minimize ran max_minimize iterations          */
+    MF_F_LLT_FAIL     = 2,   /* a Cholesky pivot was <= 0 or NaN              */
+    MF_F_FNNLS_CAP    = 4,   /* an FNNLS call hit max_iter                    */
+    MF_F_NONFINITE    = 8,   /* non-finite input sample: not fitted, x = 0    */
+    MF_F_FP64_PIVOT   = 16,  /* FP32 path: a pivot was recomputed in FP64     */
+    MF_F_WARM_REJECT  = 32,  /* warm start failed its KKT check: cold refit   */
+    MF_F_REFINED      = 64   /* FP32 path: FP64 refinement step applied       */
+};
+
+/* Device-resident batch, structure of arrays.  samples[i*nch + c] for sample
+ * i of channel c (dtype MF_DTYPE_*).  type_of[c] indexes the table passed to
+ * mf_set_types (NULL: every channel is type 0). */
+typedef struct mf_batch {
+    int64_t        nch;
+    int32_t        sample_dtype;
+    int32_t        pad;
+    const void    *samples;
+    const uint8_t *type_of;
+} mf_batch;
+
+/* Device-resident results, structure of arrays, in the compute precision
+ * (double for MF_PREC_FP64, float for MF_PREC_FP32): x[p*nch + c] amplitude
+ * of pulse p (pulse order of the channel's type, the BXs() order of the
+ * reference), chi2[c], flags[c] (MF_F_*), and optionally iters[c] =
+ * n_minimize | n_fnnls_outer << 8 | n_fnnls_inner << 16 (NULL: skipped). */
+typedef struct mf_result {
+    void     *x;
+    void     *chi2;
+    int32_t  *flags;
+    int32_t  *iters;
+} mf_result;
+
+typedef struct mf_context mf_context;
+
+const char *mf_last_error(void);
+int mf_abi_version(void);
+
+mf_status mf_context_create(mf_context **ctx, int device);
+mf_status mf_context_destroy(mf_context *ctx);
+/* Copies ntypes channel types (host memory) to the device.  Shapes other
+ * than 10x10 and 8x8 are rejected. */
+mf_status mf_set_types(mf_context *ctx, const mf_chtype *types, int32_t ntypes);
+mf_status mf_default_opts(mf_opts *opts);
+
+/* Fits every channel of the batch: DoFit with error computation disabled,
+ * as every reference driver runs it (multifit_cpu/src/multifit.cpp:200,
+ * multifit_gpu/src/PulseChiSqSNNLS.cu:234).  Asynchronous on `stream`. */
+mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_result *out,
+                       const mf_opts *opts, void *stream);
+
+/* Batched NNLS solvers, one problem per channel, FP64.
+ * A: n matrices of m x m, column-major as the reference's FixedMatrix
+ * (nnls/interface/data_types.h:25-26), A[c*100 + col*10 + row] (leading
+ * dimension 10 whatever m is); b[c*10 + i]; x[c*10 + i] in/out (warm start
+ * in, solution out).  m <= 10; rows/cols >= m are ignored. */
+mf_status mf_inplace_fnnls_batch(const double *A, const double *b, double *x,
+                                 int64_t n, int32_t m, double eps,
+                                 uint32_t max_iter, void *stream);
+mf_status mf_fnnls_batch(const double *A, const double *b, double *x,
+                         int64_t n, int32_t m, double eps, uint32_t max_iter,
+                         void *stream);
+
+/* Batched linalgebra primitives on n row-major matrices of full x full
+ * (full <= 10), operating on the leading view x view block, FP64 or FP32
+ * (dtype MF_DTYPE_*).  Arguments by op (per problem c; M, L: full*full
+ * elements at c*full*full; vectors: full elements at c*full):
+ *   MF_LA_CHOLESKY        M -> L                 (cholesky_decomp)
+ *   MF_LA_FORWARD         L, b -> y              (solve_forward_substitution)
+ *   MF_LA_BACKWARD        L, b(=y) -> y(=x)      (solve_backward_substitution)
+ *   MF_LA_SWAP            M in place, i, j       (swap_row_column, j > i)
+ *   MF_LA_RCADDITION      M, L in/out, b, y in/out
+ *   MF_LA_INBETWEEN       M, L in/out, b, y in/out, i = position */
+enum {
+    MF_LA_CHOLESKY = 0, MF_LA_FORWARD = 1, MF_LA_BACKWARD = 2, MF_LA_SWAP = 3,
+    MF_LA_RCADDITION = 4, MF_LA_INBETWEEN = 5
+};
+mf_status mf_linalg_batch(int32_t op, int32_t dtype, void *M, void *L, void *b,
+                          void *y, int64_t n, int32_t full, int32_t view,
+                          int32_t i, int32_t j, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MF_ABI_H */

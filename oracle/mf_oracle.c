@@ -1,0 +1,712 @@
+/*
+ * mf_oracle.c -- FP64 CPU oracle for the per-channel multifit.
+ *
+ * TEST INFRASTRUCTURE: the parity checker and the CPU baseline.  Nothing in
+ * the product library links or calls this file.
+ *
+ * What it restates (reference files are cited relative to the reference
+ * repository root):
+ *   or_llt / or_cholesky_decomp ... linalgebra/interface/cholesky.hpp:8-38 and
+ *                                   Eigen LLT (called at
+ *                                   multifit_cpu/src/PulseChiSqSNNLS.cpp:192,
+ *                                   nnls/src/inplace_fnnls.cu:176-177)
+ *   or_forward_subst / backward ... linalgebra/interface/substitution.hpp:10-71
+ *   or_swap_row_column ............ linalgebra/interface/common.hpp:11-42
+ *   or_fused_rcaddition ........... linalgebra/interface/fused_cholesky_substitution.hpp:18-57
+ *   or_fused_inbetween_removal .... linalgebra/interface/fused_cholesky_substitution.hpp:62-102
+ *   or_transpose_multiply ......... nnls/src/inplace_fnnls.cu:11-25
+ *   or_inplace_fnnls .............. nnls/src/inplace_fnnls.cu:72-233
+ *   or_inplace_fnnls_chol ......... nnls/src/inplace_fnnls_test_cholesky.cu:35-227
+ *   or_fnnls ...................... nnls/src/fnnls.cu:28-173
+ *   or_build_pulsemat ............. multifit_cpu/src/PulseChiSqSNNLS.cpp:50-62
+ *   or_dofit (+ minimize, update_cov, nnls_step, chisq, approx uncertainty)
+ *                                   multifit_cpu/src/PulseChiSqSNNLS.cpp:21-207,219-274
+ *
+ * Arithmetic conventions (the reference leaves them to Eigen, whose order is
+ * not fixed by any reference test): dot products run in ascending index order
+ * with fused multiply-adds, Cholesky elements are (M_ij - sum)/L_jj as in
+ * cholesky.hpp, substitutions subtract term by term as in substitution.hpp.
+ * Build with -ffp-contract=off so that only the explicit fma() calls fuse.
+ *
+ * Pinning (checks live in tests/test_oracle.py):
+ *   dense primitives ...... golden vectors written by the reference's own
+ *                           linalgebra headers (tests/golden), numpy
+ *                           cholesky/solve, the 3x3 closed form.
+ *   FNNLS variants ........ scipy.optimize.nnls, brute force over all 2^n
+ *                           passive sets, KKT conditions, FP32 golden vectors
+ *                           from the reference's OpenCL inplace_fnnls compiled
+ *                           as host C.
+ *   multifit .............. closed forms (single pulse, noise-free recovery,
+ *                           identity covariance) and recomputation of chi2.
+ *   Parity unpinned: Eigen's maxCoeff tie order (ties are resolved to the
+ *   first index of the permuted tail, as Eigen's visitor does) and
+ *   rounding-level decisions (|dchi2| ~ 1e-3, max w ~ eps); no reference test
+ *   fixes them.
+ */
+#include "mf_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdatomic.h>
+#include <string.h>
+
+#define LD OR_LD
+#define IDX(i, j) ((i) * LD + (j))
+
+/* ------------------------------------------------------------------------ */
+/* dense primitives                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* Cholesky element recurrence shared by Eigen's unblocked LLT and
+ * linalgebra's cholesky_decomp: both compute
+ *   L_kk = sqrt(M_kk - sum_{j<k} L_kj^2),
+ *   L_ik = (M_ik - sum_{j<k} L_ij L_kj) / L_kk.
+ * Returns -1 or the failing column (pivot <= 0, Eigen llt_inplace::unblocked). */
+static int chol_core(const double *M, double *L, int ld, int n, int stop_on_fail)
+{
+    for (int k = 0; k < n; ++k) {
+        double sq = 0.0;
+        for (int j = 0; j < k; ++j) sq = fma(L[k * ld + j], L[k * ld + j], sq);
+        double d = M[k * ld + k] - sq;
+        if (stop_on_fail && !(d > 0.0)) {
+            if (d <= 0.0) {
+                /* Eigen leaves columns >= k as the input's lower triangle */
+                for (int c = k; c < n; ++c)
+                    for (int i = c; i < n; ++i) L[i * ld + c] = M[i * ld + c];
+                for (int i = 0; i < n; ++i)
+                    for (int j = i + 1; j < n; ++j) L[i * ld + j] = 0.0;
+                return k;
+            }
+            /* NaN pivot: Eigen's "x <= 0" test is false, it continues */
+        }
+        double lkk = sqrt(d);
+        L[k * ld + k] = lkk;
+        for (int i = k + 1; i < n; ++i) {
+            double t = 0.0;
+            for (int j = 0; j < k; ++j) t = fma(L[i * ld + j], L[k * ld + j], t);
+            L[i * ld + k] = (M[i * ld + k] - t) / lkk;
+        }
+        for (int j = k + 1; j < n; ++j) L[k * ld + j] = 0.0;
+    }
+    return -1;
+}
+
+int or_llt(const double *M, double *L, int n) { return chol_core(M, L, LD, n, 1); }
+
+void or_cholesky_decomp(const double *M, double *L, int ld, int n)
+{
+    /* linalgebra cholesky_decomp writes only the lower triangle */
+    for (int i = 0; i < n; ++i) {
+        double sumsq = 0.0;
+        for (int j = 0; j < i; ++j) {
+            double s2 = 0.0;
+            for (int k = 0; k < j; ++k) s2 = fma(L[i * ld + k], L[j * ld + k], s2);
+            double v = (M[i * ld + j] - s2) / L[j * ld + j];
+            L[i * ld + j] = v;
+            sumsq = fma(v, v, sumsq);
+        }
+        L[i * ld + i] = sqrt(M[i * ld + i] - sumsq);
+    }
+}
+
+void or_forward_subst(const double *L, const double *b, double *y, int ld, int n)
+{
+    for (int i = 0; i < n; ++i) {
+        double t = b[i];
+        for (int j = 0; j < i; ++j) t = fma(-L[i * ld + j], y[j], t);
+        y[i] = t / L[i * ld + i];
+    }
+}
+
+/* L^T x = y with L lower (substitution.hpp:34-50 reads L transposed) */
+void or_backward_subst(const double *L, const double *y, double *x, int ld, int n)
+{
+    for (int i = n - 1; i >= 0; --i) {
+        double t = y[i];
+        for (int j = i + 1; j < n; ++j) t = fma(-L[j * ld + i], x[j], t);
+        x[i] = t / L[i * ld + i];
+    }
+}
+
+void or_swap_row_column(double *M, int i, int j, int ld, int n)
+{
+    /* symmetric swap of rows/cols i and j (precondition j > i), as
+     * common.hpp:11-42: diagonal, then the elements left of i and right of i
+     * (the j-th entries of those loops touch the (i,j) pair itself) */
+    double t = M[i * ld + i];
+    M[i * ld + i] = M[j * ld + j];
+    M[j * ld + j] = t;
+    for (int e = 0; e < i; ++e) {
+        double tmp = M[i * ld + e];
+        M[i * ld + e] = M[j * ld + e];
+        M[j * ld + e] = tmp;
+        M[e * ld + i] = M[e * ld + j];
+        M[e * ld + j] = tmp;
+    }
+    for (int e = i + 1; e < n; ++e) {
+        double tmp = M[i * ld + e];
+        M[i * ld + e] = M[j * ld + e];
+        M[j * ld + e] = tmp;
+        M[e * ld + i] = M[e * ld + j];
+        M[e * ld + j] = tmp;
+    }
+}
+
+void or_fused_rcaddition(const double *M, double *L, const double *b, double *y,
+                         int ld, int n)
+{
+    int row = n - 1;
+    double sum = 0.0, total = b[row];
+    for (int col = 0; col < row; ++col) {
+        double s2 = 0.0;
+        for (int j = 0; j < col; ++j) s2 = fma(L[row * ld + j], L[col * ld + j], s2);
+        double v = (M[row * ld + col] - s2) / L[col * ld + col];
+        total = fma(-v, y[col], total);
+        sum = fma(v, v, sum);
+        L[row * ld + col] = v;
+    }
+    double d = sqrt(M[row * ld + row] - sum);
+    L[row * ld + row] = d;
+    y[row] = total / d;
+}
+
+void or_fused_inbetween_removal(const double *M, double *L, const double *b,
+                                double *y, int position, int ld, int n)
+{
+    for (int i = position; i < n; ++i) {
+        double sumsq = 0.0, total = b[i];
+        for (int j = 0; j < i; ++j) {
+            double s2 = 0.0;
+            for (int k = 0; k < j; ++k) s2 = fma(L[i * ld + k], L[j * ld + k], s2);
+            double v = (M[i * ld + j] - s2) / L[j * ld + j];
+            L[i * ld + j] = v;
+            total = fma(-v, y[j], total);
+            sumsq = fma(v, v, sumsq);
+        }
+        double d = sqrt(M[i * ld + i] - sumsq);
+        L[i * ld + i] = d;
+        y[i] = total / d;
+    }
+}
+
+static void tmul_mn(const double *A, double *AtA, int m, int n)
+{
+    for (int i = 0; i < n; ++i)
+        for (int j = i; j < n; ++j) {
+            double t = 0.0;
+            for (int k = 0; k < m; ++k) t = fma(A[IDX(k, i)], A[IDX(k, j)], t);
+            AtA[IDX(i, j)] = t;
+            AtA[IDX(j, i)] = t;
+        }
+}
+
+void or_transpose_multiply(const double *A, double *AtA, int n) { tmul_mn(A, AtA, n, n); }
+
+static void atb_mn(const double *A, const double *b, double *Atb, int m, int n)
+{
+    for (int i = 0; i < n; ++i) {
+        double t = 0.0;
+        for (int k = 0; k < m; ++k) t = fma(A[IDX(k, i)], b[k], t);
+        Atb[i] = t;
+    }
+}
+
+/* s = LLT(M[0:k,0:k]).solve(v[0:k]) with full refactorisation */
+static int llt_solve(const double *M, const double *v, double *s, int k)
+{
+    double L[LD * LD], y[LD];
+    int f = chol_core(M, L, LD, k, 1);
+    or_forward_subst(L, v, y, LD, k);
+    or_backward_subst(L, y, s, LD, k);
+    return f;
+}
+
+/* ------------------------------------------------------------------------ */
+/* NNLS solvers                                                              */
+/* ------------------------------------------------------------------------ */
+
+static void swap_d(double *a, int i, int j) { double t = a[i]; a[i] = a[j]; a[j] = t; }
+static void swap_i(int *a, int i, int j) { int t = a[i]; a[i] = a[j]; a[j] = t; }
+
+/* full symmetric row+column swap of a 10x10 matrix, as Eigen's
+ * AtA.col(a).swap(AtA.col(b)); AtA.row(a).swap(AtA.row(b)); */
+static void sym_swap(double *M, int a, int b)
+{
+    if (a == b) return;
+    for (int r = 0; r < LD; ++r) swap_d(M + r * LD, a, b);
+    for (int c = 0; c < LD; ++c) {
+        double t = M[IDX(a, c)];
+        M[IDX(a, c)] = M[IDX(b, c)];
+        M[IDX(b, c)] = t;
+    }
+}
+
+static double chol_flops(int k)
+{
+    return k * ((double)k * k - 1.0) / 3.0 + k * (k - 1) / 2.0 + k;
+}
+
+static void stats_solve(or_nnls_stats *st, int k)
+{
+    if (!st) return;
+    st->aflops += chol_flops(k) + 2.0 * k * k;
+    st->inner++;
+    st->ksum += k;
+    st->k2sum += (long long)k * k;
+    st->k3sum += (long long)k * k * k;
+}
+
+/* Shared driver of inplace_fnnls and its incremental-Cholesky twin
+ * (inplace_fnnls.cu:72-233 and inplace_fnnls_test_cholesky.cu:35-227): the
+ * loops are identical, only the passive-set solve differs. */
+static void inplace_fnnls_mn(const double *A, const double *b, double *x, int m,
+                             int n, double eps, unsigned max_iter,
+                             or_nnls_stats *st, int incremental)
+{
+    double AtA[LD * LD], Atb[LD], s[LD], w[LD], xp[LD];
+    double pL[LD * LD], py[LD];
+    int perm[LD];
+    memset(AtA, 0, sizeof(AtA));
+    memset(pL, 0, sizeof(pL));
+    tmul_mn(A, AtA, m, n);               /* :115 */
+    atb_mn(A, b, Atb, m, n);             /* :117 */
+    for (int i = 0; i < n; ++i) { perm[i] = i; xp[i] = x[i]; s[i] = 0.0; }
+
+    int np = 0;
+    unsigned iter;
+    for (iter = 0; iter < max_iter; ++iter) {          /* :127 */
+        int nact = n - np;
+        if (!nact) break;                                /* :134 */
+        /* w.tail = Atb.tail - (AtA*x).tail                 :137 */
+        for (int i = np; i < n; ++i) {
+            double t = 0.0;
+            for (int j = 0; j < n; ++j) t = fma(AtA[IDX(i, j)], xp[j], t);
+            w[i] = Atb[i] - t;
+        }
+        if (st) st->aflops += 2.0 * n * n + (n - np);
+        /* maxCoeff: first index of the maximum            :144 */
+        int q = np;
+        double maxw = w[np];
+        for (int i = np + 1; i < n; ++i)
+            if (w[i] > maxw) { maxw = w[i]; q = i; }
+        if (maxw < eps) break;                           /* :147 */
+        sym_swap(AtA, np, q);                            /* :157-164 */
+        swap_d(Atb, np, q);
+        swap_d(xp, np, q);
+        swap_i(perm, np, q);
+        ++np;
+        if (st) st->outer++;
+
+        int inner_iteration = 0, position_removed = -1;
+        while (np > 0) {                                 /* :175 */
+            if (!incremental) {
+                int f = llt_solve(AtA, Atb, s, np);      /* :176-177 */
+                if (f >= 0 && st) st->llt_fail = 1;
+            } else {
+                /* inplace_fnnls_test_cholesky.cu:145-166 */
+                if (np == 1) {
+                    double l00 = sqrt(AtA[0]);
+                    pL[0] = l00;
+                    py[0] = Atb[0] / l00;
+                    position_removed = -1;
+                } else if (inner_iteration == 0) {
+                    or_fused_rcaddition(AtA, pL, Atb, py, LD, np);
+                } else {
+                    or_fused_inbetween_removal(AtA, pL, Atb, py, position_removed,
+                                               LD, np);
+                }
+                or_backward_subst(pL, py, s, LD, np);
+            }
+            stats_solve(st, np);
+
+            double mins = s[0];                          /* :179 */
+            for (int i = 1; i < np; ++i) if (s[i] < mins) mins = s[i];
+            if (mins > 0.0) {
+                for (int i = 0; i < np; ++i) xp[i] = s[i];
+                break;
+            }
+            double alpha = DBL_MAX;                      /* :188-200 */
+            int aidx = 0;
+            for (int i = 0; i < np; ++i) {
+                if (s[i] <= 0.0) {
+                    double ratio = xp[i] / (xp[i] - s[i]);
+                    if (ratio < alpha) { alpha = ratio; aidx = i; }
+                }
+            }
+            if (alpha == DBL_MAX) {                      /* :202-205 */
+                for (int i = 0; i < np; ++i) xp[i] = s[i];
+                break;
+            }
+            for (int i = 0; i < np; ++i)                 /* :215 */
+                xp[i] = fma(alpha, s[i] - xp[i], xp[i]);
+            xp[aidx] = 0.0;                              /* :216 */
+            --np;
+            sym_swap(AtA, np, aidx);                     /* :222-229 */
+            swap_d(Atb, np, aidx);
+            swap_d(xp, np, aidx);
+            swap_i(perm, np, aidx);
+            position_removed = aidx;
+            inner_iteration++;
+        }
+    }
+    if (st && iter == max_iter && max_iter > 0) st->hit_max = 1;
+    for (int i = 0; i < n; ++i) x[perm[i]] = xp[i];      /* :232 */
+}
+
+void or_inplace_fnnls(const double *A, const double *b, double *x, int n,
+                      double eps, unsigned max_iter, or_nnls_stats *st)
+{
+    inplace_fnnls_mn(A, b, x, n, n, eps, max_iter, st, 0);
+}
+
+void or_inplace_fnnls_chol(const double *A, const double *b, double *x, int n,
+                           double eps, unsigned max_iter, or_nnls_stats *st)
+{
+    inplace_fnnls_mn(A, b, x, n, n, eps, max_iter, st, 1);
+}
+
+/* fnnls.cu:28-173.  Bool-mask FNNLS working on the full-length x.
+ * Deviation: the reference reads an uninitialised `s` if its passive count
+ * guard (:119) fires and sizes the sub-matrix from a counter that is one short
+ * when w's running maximum starts on a passive index (:82-95); here `s`
+ * starts at zero and the passive block is sized by the actual mask. */
+static void fnnls_mn(const double *A, const double *b, double *x, int m, int n,
+                     double eps, unsigned max_iter, or_nnls_stats *st)
+{
+    double AtA[LD * LD], Atb[LD], s[LD], w[LD];
+    int active[LD];
+    memset(AtA, 0, sizeof(AtA));
+    tmul_mn(A, AtA, m, n);
+    atb_mn(A, b, Atb, m, n);
+    for (int i = 0; i < n; ++i) { active[i] = 1; s[i] = 0.0; }
+    unsigned iter;
+    for (iter = 0; iter < max_iter; ++iter) {
+        for (int i = 0; i < n; ++i) {                    /* :73 */
+            double t = 0.0;
+            for (int j = 0; j < n; ++j) t = fma(AtA[IDX(i, j)], x[j], t);
+            w[i] = Atb[i] - t;
+        }
+        if (st) st->aflops += 2.0 * n * n + n;
+        int maxi = 0, nact = -1;
+        for (int i = 0; i < n; ++i)                      /* :88-95 */
+            if (active[i]) {
+                ++nact;
+                if (w[i] > w[maxi]) maxi = i;
+            }
+        active[maxi] = 0;                                /* :104 */
+        if (!nact || w[maxi] < eps) break;               /* :113 */
+        if (st) st->outer++;
+        for (;;) {                                       /* :118 */
+            int idx[LD], k = 0;
+            for (int i = 0; i < n; ++i) if (!active[i]) idx[k++] = i;
+            if (k == 0) { for (int i = 0; i < n; ++i) x[i] = s[i]; break; }
+            double Ms[LD * LD], vs[LD], ss[LD];
+            for (int r = 0; r < k; ++r) {
+                vs[r] = Atb[idx[r]];
+                for (int c = 0; c < k; ++c) Ms[IDX(r, c)] = AtA[IDX(idx[r], idx[c])];
+            }
+            if (llt_solve(Ms, vs, ss, k) >= 0 && st) st->llt_fail = 1;
+            stats_solve(st, k);
+            for (int i = 0, r = 0; i < n; ++i) s[i] = active[i] ? 0.0 : ss[r++];
+            double alpha = DBL_MAX;
+            int aidx = -1;
+            for (int i = 0; i < n; ++i)                  /* :141-149 */
+                if (!active[i] && s[i] <= 0.0) {
+                    double tmp = -x[i] / (s[i] - x[i]);
+                    if (tmp < alpha) { alpha = tmp; aidx = i; }
+                }
+            if (alpha == DBL_MAX) { for (int i = 0; i < n; ++i) x[i] = s[i]; break; }
+            for (int i = 0; i < n; ++i) x[i] = fma(alpha, s[i] - x[i], x[i]);
+            x[aidx] = 0.0;
+            active[aidx] = 1;
+        }
+    }
+    if (st && iter == max_iter && max_iter > 0) st->hit_max = 1;
+}
+
+void or_fnnls(const double *A, const double *b, double *x, int n, double eps,
+              unsigned max_iter, or_nnls_stats *st)
+{
+    fnnls_mn(A, b, x, n, n, eps, max_iter, st);
+}
+
+/* ------------------------------------------------------------------------ */
+/* per-channel multifit                                                      */
+/* ------------------------------------------------------------------------ */
+
+static int imax(int a, int b) { return a > b ? a : b; }
+
+/* PulseChiSqSNNLS.cpp:50-62: column j holds fullpulse[f+o ...] in rows f.. */
+void or_build_pulsemat(const or_chtype *t, double *P)
+{
+    memset(P, 0, sizeof(double) * LD * LD);
+    for (int j = 0; j < t->npulses; ++j) {
+        int bx = t->bx[j];
+        int f = imax(0, bx + t->first_off);
+        int o = t->pulse_off - bx;
+        for (int i = f; i < t->nsamples; ++i) P[IDX(i, j)] = t->fullpulse[i + o];
+    }
+}
+
+typedef struct {
+    const or_chtype *t;
+    const or_opts *o;
+    double P[LD * LD];
+    double s[LD];        /* _sampvec */
+    double x[LD];        /* _ampvec  */
+    double C[LD * LD];   /* _invcov (lower) */
+    double L[LD * LD];   /* _covdecomp.matrixL() */
+    double chisq;
+    int flags;
+    int has_pi;          /* fullpulsecov has a nonzero entry (flop count only) */
+    or_nnls_stats st;
+} fitctx;
+
+/* updateCov, PulseChiSqSNNLS.cpp:169-194 */
+static void update_cov(fitctx *c)
+{
+    const or_chtype *t = c->t;
+    int ns = t->nsamples;
+    double p2 = t->pederr * t->pederr;
+    memset(c->C, 0, sizeof(c->C));
+    for (int i = 0; i < ns; ++i)
+        for (int j = 0; j <= i; ++j) c->C[IDX(i, j)] = p2 * t->noisecor[IDX(i, j)];
+    for (int p = 0; p < t->npulses; ++p) {
+        if (c->x[p] == 0.0) continue;
+        int bx = t->bx[p];
+        int f = imax(0, bx + t->first_off);
+        int o = t->pulse_off - bx;
+        double a2 = c->x[p] * c->x[p];
+        for (int i = f; i < ns; ++i)
+            for (int j = f; j <= i; ++j)
+                c->C[IDX(i, j)] = fma(a2, t->fullpulsecov[(i + o) * OR_FULL + (j + o)],
+                                      c->C[IDX(i, j)]);
+    }
+    if (chol_core(c->C, c->L, LD, ns, 1) >= 0) c->flags |= OR_F_LLT_FAIL;
+    if (c->o->count_flops) {
+        const int n = ns, p = t->npulses;
+        double f = n * (n + 1) / 2.0 + chol_flops(n) + (double)p * n * n + (double)n * n +
+                   (double)n * p * (p + 1) + 2.0 * n * p + (2.0 * n * p + n + (double)n * n + 2.0 * n);
+        if (c->has_pi)
+            for (int q = 0; q < p; ++q) {
+                if (c->x[q] == 0.0) continue;
+                const int m = ns - imax(0, t->bx[q] + t->first_off);
+                if (m > 0) f += (double)m * (m + 1);
+            }
+        c->st.aflops += f;
+    }
+}
+
+/* NNLS, PulseChiSqSNNLS.cpp:219-274 */
+static void nnls_step(fitctx *c)
+{
+    const or_chtype *t = c->t;
+    int ns = t->nsamples, np = t->npulses;
+    double A[LD * LD], b[LD], col[LD], y[LD];
+    memset(A, 0, sizeof(A));
+    for (int j = 0; j < np; ++j) {                        /* :235 */
+        for (int i = 0; i < ns; ++i) col[i] = c->P[IDX(i, j)];
+        or_forward_subst(c->L, col, y, LD, ns);
+        for (int i = 0; i < ns; ++i) A[IDX(i, j)] = y[i];
+    }
+    or_forward_subst(c->L, c->s, b, LD, ns);              /* :236 */
+    double xx[LD];
+    for (int i = 0; i < np; ++i) xx[i] = c->x[i];         /* :258 */
+    or_nnls_stats st;
+    memset(&st, 0, sizeof(st));
+    switch (c->o->solver) {
+    case OR_SOLVER_INPLACE_CHOL:
+        inplace_fnnls_mn(A, b, xx, ns, np, c->o->eps, c->o->max_iter, &st, 1);
+        break;
+    case OR_SOLVER_FNNLS:
+        fnnls_mn(A, b, xx, ns, np, c->o->eps, c->o->max_iter, &st);
+        break;
+    default:
+        inplace_fnnls_mn(A, b, xx, ns, np, c->o->eps, c->o->max_iter, &st, 0);
+        break;
+    }
+    for (int i = 0; i < np; ++i) c->x[i] = xx[i];         /* :273 */
+    c->st.outer += st.outer;
+    c->st.inner += st.inner;
+    c->st.ksum += st.ksum;
+    c->st.k2sum += st.k2sum;
+    c->st.k3sum += st.k3sum;
+    c->st.aflops += st.aflops;
+    if (st.hit_max) c->flags |= OR_F_FNNLS_CAP;
+    if (st.llt_fail) c->flags |= OR_F_LLT_FAIL;
+}
+
+/* ComputeChiSq, PulseChiSqSNNLS.cpp:196-200: |L^-1 (P x - s)|^2 */
+static double compute_chisq(const fitctx *c)
+{
+    int ns = c->t->nsamples, np = c->t->npulses;
+    double r[LD] = {0}, y[LD];
+    for (int i = 0; i < ns; ++i) {
+        double t = 0.0;
+        for (int j = 0; j < np; ++j) t = fma(c->P[IDX(i, j)], c->x[j], t);
+        r[i] = t - c->s[i];
+    }
+    or_forward_subst(c->L, r, y, LD, ns);
+    double q = 0.0;
+    for (int i = 0; i < ns; ++i) q = fma(y[i], y[i], q);
+    return q;
+}
+
+/* Minimize, PulseChiSqSNNLS.cpp:149-166 */
+static int minimize(fitctx *c, int *iters)
+{
+    int it;
+    for (it = 0; it < c->o->max_minimize; ++it) {
+        update_cov(c);
+        nnls_step(c);
+        double now = compute_chisq(c);
+        double d = now - c->chisq;
+        c->chisq = now;
+        if (fabs(d) < c->o->chi2_tol) {
+            if (iters) *iters = it + 1;
+            return 1;
+        }
+    }
+    if (iters) *iters = it;
+    c->flags |= OR_F_MINIMIZE_CAP;
+    return 1;
+}
+
+/* ComputeApproxUncertainty, PulseChiSqSNNLS.cpp:202-207 */
+static double approx_uncertainty(const fitctx *c, int ip)
+{
+    int ns = c->t->nsamples;
+    double col[LD], y[LD];
+    for (int i = 0; i < ns; ++i) col[i] = c->P[IDX(i, ip)];
+    or_forward_subst(c->L, col, y, LD, ns);
+    double q = 0.0;
+    for (int i = 0; i < ns; ++i) q = fma(y[i], y[i], q);
+    return 1.0 / sqrt(q);
+}
+
+/* DoFit, PulseChiSqSNNLS.cpp:21-147 */
+int or_dofit(const or_chtype *t, const double *samples, const or_opts *o,
+             or_result *r)
+{
+    fitctx c;
+    memset(&c, 0, sizeof(c));
+    c.t = t;
+    c.o = o;
+    if (o->count_flops)
+        for (int i = 0; i < OR_FULL * OR_FULL; ++i)
+            if (t->fullpulsecov[i] != 0.0) { c.has_pi = 1; break; }
+    int ns = t->nsamples, np = t->npulses;
+    memset(r, 0, sizeof(*r));
+    for (int i = 0; i < ns; ++i) {
+        c.s[i] = samples[i];
+        if (!isfinite(samples[i])) c.flags |= OR_F_NONFINITE;
+    }
+    or_build_pulsemat(t, c.P);
+    int nmin = 0;
+    int status = minimize(&c, &nmin);                    /* :65 */
+    r->n_minimize = nmin;
+    for (int i = 0; i < np; ++i) r->x[i] = c.x[i];       /* :66 _ampvecmin */
+    r->n_outer = c.st.outer;
+    r->n_inner = c.st.inner;
+    r->ksum = c.st.ksum;
+    r->k2sum = c.st.k2sum;
+    r->k3sum = c.st.k3sum;
+    r->aflops = c.st.aflops;
+
+    if (status && o->compute_errors) {                   /* :70-146 */
+        int ipit = -1;
+        for (int i = 0; i < np; ++i)
+            if (t->bx[i] == 0) { ipit = i; break; }
+        if (ipit >= 0) {
+            const int ipitmin = ipit;
+            double approxerr = approx_uncertainty(&c, ipit);
+            double chisq0 = c.chisq;
+            double x0 = r->x[ipit];
+            /* :96-102 compares against _nP, which the new NNLS never moves
+             * off zero, so the column swap never happens */
+            double pulse[LD];
+            for (int i = 0; i < ns; ++i) {
+                pulse[i] = c.P[IDX(i, ipit)];
+                c.P[IDX(i, ipit)] = 0.0;
+            }
+            double xplus = x0 + approxerr;
+            c.x[ipit] = xplus;
+            for (int i = 0; i < ns; ++i) c.s[i] = samples[i] - c.x[ipit] * pulse[i];
+            status &= minimize(&c, NULL);
+            double chiplus = compute_chisq(&c);
+            double sigplus = fabs(xplus - x0) / sqrt(chiplus - chisq0);
+            if ((x0 / sigplus) > 0.5) {
+                double xminus = x0 - approxerr;
+                if (xminus < 0.0) xminus = 0.0;
+                c.x[ipit] = xminus;
+                for (int i = 0; i < ns; ++i) c.s[i] = samples[i] - c.x[ipit] * pulse[i];
+                status &= minimize(&c, NULL);
+                double chiminus = compute_chisq(&c);
+                double sigminus = fabs(xminus - x0) / sqrt(chiminus - chisq0);
+                r->err[ipitmin] = 0.5 * (sigplus + sigminus);
+            } else {
+                r->err[ipitmin] = sigplus;
+            }
+            c.chisq = chisq0;
+        }
+    }
+    r->chi2 = c.chisq;
+    r->status = status;
+    r->flags = c.flags;
+    return status;
+}
+
+/* ------------------------------------------------------------------------ */
+/* threaded batch (CPU baseline)                                             */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    const or_chtype *types;
+    const uint8_t *type_of;
+    const double *samples;
+    long long nch;
+    const or_opts *o;
+    or_result *res;
+    atomic_llong next;
+} batchctx;
+
+static void *batch_worker(void *arg)
+{
+    batchctx *b = (batchctx *)arg;
+    const long long chunk = 256;
+    for (;;) {
+        long long c0 = atomic_fetch_add(&b->next, chunk);
+        if (c0 >= b->nch) break;
+        long long c1 = c0 + chunk < b->nch ? c0 + chunk : b->nch;
+        for (long long c = c0; c < c1; ++c) {
+            const or_chtype *t = b->types + (b->type_of ? b->type_of[c] : 0);
+            double s[LD];
+            for (int i = 0; i < t->nsamples; ++i) s[i] = b->samples[i * b->nch + c];
+            or_dofit(t, s, b->o, b->res + c);
+        }
+    }
+    return NULL;
+}
+
+void or_dofit_batch(const or_chtype *types, const uint8_t *type_of,
+                    const double *samples, long long nch, const or_opts *o,
+                    or_result *res, int nthreads)
+{
+    batchctx b;
+    b.types = types;
+    b.type_of = type_of;
+    b.samples = samples;
+    b.nch = nch;
+    b.o = o;
+    b.res = res;
+    atomic_init(&b.next, 0);
+    if (nthreads <= 1) {
+        batch_worker(&b);
+        return;
+    }
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    for (int i = 0; i < nthreads; ++i) pthread_create(&th[i], NULL, batch_worker, &b);
+    for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
+}

@@ -1,0 +1,571 @@
+// fit_core.hpp -- per-channel multifit, one channel per lane.
+//
+// Everything here is written for registers: matrix sizes are template
+// constants, every loop is fully unrolled, every array index is a
+// compile-time constant.  Data-dependent choices (which pulses are in the
+// passive set, which index enters or leaves) become predicates and selects,
+// never dynamic register indexing, so a wavefront runs the linear algebra of
+// 64 channels in lock step with no scratch traffic.
+//
+// The same header compiles for gfx950 (hipcc) and for the host (g++), so that
+// the kernel's arithmetic can be checked against the oracle on a machine
+// without a GPU.  The product library only ever runs the device build.
+//
+// Reference semantics restated (paths relative to the reference root):
+//   fit_channel ........... PulseChiSqSNNLS::DoFit/Minimize/updateCov/NNLS/
+//                           ComputeChiSq, multifit_cpu/src/PulseChiSqSNNLS.cpp:21-71,149-200,219-274
+//   fnnls_inplace ......... inplace_fnnls, nnls/src/inplace_fnnls.cu:72-233
+//   fnnls_mask ............ fnnls, nnls/src/fnnls.cu:28-173
+//
+// How the reference's physical permutation is handled.  inplace_fnnls swaps
+// rows/columns of AtA so that the passive set occupies the leading block, and
+// solves that block with a fresh LLT.  Here AtA stays in original index order
+// and the passive block is solved by a *masked* Cholesky: rows/columns of
+// free indices are replaced by the identity on the fly, which leaves exactly
+// the Cholesky factor of AtA[P,P] (in ascending index order) embedded in a
+// full-size factor.  The reference's permutation is still tracked, as 4-bit
+// fields packed in a 64-bit word, because it fixes the tie-breaking of its
+// argmax over w (first maximum in permuted order, :144) and of its argmin
+// over the step ratios (first minimum in passive order, :192-199).  The
+// solution of the passive system is the same; only the rounding of the solve
+// differs from an LLT in permuted order.
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define MF_HD __host__ __device__ __forceinline__
+#else
+#include <cmath>
+#define MF_HD inline
+#endif
+
+namespace mf {
+
+// ---------------------------------------------------------------------------
+// scalar helpers
+// ---------------------------------------------------------------------------
+MF_HD double mfma(double a, double b, double c) { return fma(a, b, c); }
+MF_HD float mfma(float a, float b, float c) { return fmaf(a, b, c); }
+MF_HD double mabs(double a) { return fabs(a); }
+MF_HD float mabs(float a) { return fabsf(a); }
+
+// 1/sqrt(d).  FP64: two correctly rounded operations.  FP32 on the device:
+// the hardware reciprocal square root (1 ulp).
+MF_HD double mrsqrt(double d) { return 1.0 / sqrt(d); }
+MF_HD float mrsqrt(float d)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rsqf(d);
+#else
+    return 1.0f / sqrtf(d);
+#endif
+}
+// x / y: FP64 correctly rounded; FP32 on the device via the hardware
+// reciprocal (1 ulp), which is all the step-length ratio needs.
+MF_HD double mdiv(double x, double y) { return x / y; }
+MF_HD float mdiv(float x, float y)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return x * __builtin_amdgcn_rcpf(y);
+#else
+    return x / y;
+#endif
+}
+
+template <typename T> struct Lim;
+template <> struct Lim<double> {
+    MF_HD static double max() { return 1.7976931348623157e308; }
+    MF_HD static double eps() { return 2.220446049250313e-16; }
+};
+template <> struct Lim<float> {
+    MF_HD static float max() { return 3.40282347e38f; }
+    MF_HD static float eps() { return 1.1920929e-7f; }
+};
+
+// Opaque copy of a pointer: stops the compiler from hoisting loads of the
+// wave-uniform type tables (hundreds of values) out of the Minimize loop and
+// keeping them all live in scalar registers.
+template <typename P> MF_HD const P *opaque(const P *p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+s"(p));
+#endif
+    return p;
+}
+
+// 4-bit fields packed in a 64-bit word (permutation and its inverse)
+MF_HD int nib(uint64_t w, int k) { return (int)((w >> (4 * k)) & 15u); }
+MF_HD uint64_t setnib(uint64_t w, int k, int v)
+{
+    const int sh = 4 * k;
+    return (w & ~((uint64_t)15u << sh)) | ((uint64_t)(unsigned)v << sh);
+}
+MF_HD uint64_t ident_nibbles()
+{
+    uint64_t w = 0;
+    for (int k = 0; k < 16; ++k) w |= (uint64_t)k << (4 * k);
+    return w;
+}
+
+// ---------------------------------------------------------------------------
+// per-type constants, already expanded to the form the kernels read
+// ---------------------------------------------------------------------------
+template <typename T> struct TypeMats {
+    T P[10][10];       // pulse matrix, P[sample][pulse]
+    T C0[10][10];      // pederr^2 * noisecor (lower triangle used)
+    T Pi[10][10][10];  // Pi[p][i][j] = fullpulsecov[i+o_p][j+o_p] if i,j >= f_p else 0
+};
+struct TypeHdr {
+    int32_t ns, np, has_pi, intime;
+    int32_t first[10];
+    int32_t pad[2];
+};
+struct DevType {
+    TypeHdr h;
+    TypeMats<double> d;
+    TypeMats<float> f;
+};
+template <typename T> MF_HD const TypeMats<T> &mats(const DevType &t);
+template <> MF_HD const TypeMats<double> &mats<double>(const DevType &t) { return t.d; }
+template <> MF_HD const TypeMats<float> &mats<float>(const DevType &t) { return t.f; }
+
+struct FitParams {
+    double eps;          // FNNLS max-w threshold
+    double chi2_tol;     // Minimize |dchi2| threshold
+    int32_t max_iter;    // FNNLS outer cap
+    int32_t max_minimize;
+    int32_t use_pi;      // amplitude-weighted pulse covariance live
+    int32_t solver;      // 0 inplace_fnnls, 1 fnnls
+};
+
+enum {
+    F_MINIMIZE_CAP = 1, F_LLT_FAIL = 2, F_FNNLS_CAP = 4, F_NONFINITE = 8,
+    F_FP64_PIVOT = 16, F_WARM_REJECT = 32, F_REFINED = 64
+};
+
+struct FnnlsStats {
+    int outer = 0, inner = 0;
+    bool hit_max = false;
+    bool llt_fail = false;
+};
+
+// ---------------------------------------------------------------------------
+// dense kernels on register arrays (lower triangles of symmetric matrices)
+// ---------------------------------------------------------------------------
+
+// In-place Cholesky of the lower triangle of C (row-oriented, as
+// linalgebra/interface/cholesky.hpp:8-38); ri[k] = 1/L_kk.  Returns false if a
+// pivot was <= 0 or NaN.
+template <typename T, int N>
+MF_HD bool cholesky_inplace(T (&C)[10][10], T (&ri)[10])
+{
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        T sumsq = T(0);
+#pragma unroll
+        for (int j = 0; j < i; ++j) {
+            T s2 = T(0);
+#pragma unroll
+            for (int k = 0; k < j; ++k) s2 = mfma(C[i][k], C[j][k], s2);
+            const T v = (C[i][j] - s2) * ri[j];
+            C[i][j] = v;
+            sumsq = mfma(v, v, sumsq);
+        }
+        const T d = C[i][i] - sumsq;
+        ok = ok && (d > T(0));
+        const T r = mrsqrt(d);
+        ri[i] = r;
+        C[i][i] = d * r;
+    }
+    return ok;
+}
+
+// Lower-triangular L (diagonal reciprocals ri) -> L^{-1}, in place.  Row i of
+// the inverse needs row i of the factor and rows < i of the inverse, so row i
+// of the factor is copied out before it is overwritten:
+//   Z_ii = 1/L_ii,  Z_ij = -(sum_{k=j}^{i-1} L_ik Z_kj) / L_ii  (j < i).
+template <typename T, int N>
+MF_HD void invert_lower(T (&L)[10][10], const T (&ri)[10])
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        T row[10];
+#pragma unroll
+        for (int k = 0; k < i; ++k) row[k] = L[i][k];
+#pragma unroll
+        for (int j = 0; j < i; ++j) {
+            T t = T(0);
+#pragma unroll
+            for (int k = j; k < i; ++k) t = mfma(row[k], L[k][j], t);
+            L[i][j] = -t * ri[i];
+        }
+        L[i][i] = ri[i];
+    }
+}
+
+// updateCov (PulseChiSqSNNLS.cpp:169-194): lower(C) = pederr^2 R + sum_p x_p^2 Pi_p.
+// A pulse with x_p == 0 adds fma(0, Pi, C) == C exactly, so the reference's
+// `continue` (:178-179) needs no branch.
+template <typename T, int NS, int NP>
+MF_HD void build_cov(const TypeMats<T> &tm, bool use_pi, const T (&x)[10], T (&C)[10][10])
+{
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) C[i][j] = tm.C0[i][j];
+    if (use_pi) {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const T a2 = x[p] * x[p];
+#pragma unroll
+            for (int i = 0; i < NS; ++i)
+#pragma unroll
+                for (int j = 0; j <= i; ++j) C[i][j] = mfma(a2, tm.Pi[p][i][j], C[i][j]);
+        }
+    }
+}
+
+// Normal equations of the whitened problem A = L^{-1} P, b = L^{-1} s
+// (PulseChiSqSNNLS.cpp:235-236, inplace_fnnls.cu:115-117), streamed by rows
+// of A so that A is never held whole: row i of A needs only row i of L^{-1}.
+template <typename T, int NS, int NP>
+MF_HD void normal_equations(const TypeMats<T> &tm, const T (&Li)[10][10], const T (&s)[10],
+                            T (&AtA)[10][10], T (&Atb)[10])
+{
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        Atb[i] = T(0);
+#pragma unroll
+        for (int j = 0; j <= i; ++j) AtA[i][j] = T(0);
+    }
+#pragma unroll
+    for (int r = 0; r < NS; ++r) {
+        T b = T(0);
+#pragma unroll
+        for (int k = 0; k <= r; ++k) b = mfma(Li[r][k], s[k], b);
+        T a[10];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            T t = T(0);
+#pragma unroll
+            for (int k = 0; k <= r; ++k) t = mfma(Li[r][k], tm.P[k][p], t);
+            a[p] = t;
+        }
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            Atb[i] = mfma(a[i], b, Atb[i]);
+#pragma unroll
+            for (int j = 0; j <= i; ++j) AtA[i][j] = mfma(a[i], a[j], AtA[i][j]);
+        }
+    }
+}
+
+// chi2 = |L^{-1} (P x - s)|^2 (ComputeChiSq, PulseChiSqSNNLS.cpp:196-200)
+template <typename T, int NS, int NP>
+MF_HD T chisq(const TypeMats<T> &tm, const T (&Li)[10][10], const T (&s)[10], const T (&x)[10])
+{
+    T r[10];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        T t = T(0);
+#pragma unroll
+        for (int p = 0; p < NP; ++p) t = mfma(tm.P[i][p], x[p], t);
+        r[i] = t - s[i];
+    }
+    T q = T(0);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        T y = T(0);
+#pragma unroll
+        for (int k = 0; k <= i; ++k) y = mfma(Li[i][k], r[k], y);
+        q = mfma(y, y, q);
+    }
+    return q;
+}
+
+// Solve AtA[P,P] s_P = Atb_P for the passive set P (bit mask pm) with a
+// masked full-size Cholesky; s is 0 outside P.  Returns false on a
+// non-positive pivot inside P.
+template <typename T, int N>
+MF_HD bool solve_passive(const T (&M)[10][10], const T (&b)[10], uint32_t pm, T (&s)[10])
+{
+    T L[10][10], ri[10], y[10];
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const bool ink = (pm >> k) & 1u;
+        T sq = T(0);
+#pragma unroll
+        for (int j = 0; j < k; ++j) sq = mfma(L[k][j], L[k][j], sq);
+        const T d = ink ? M[k][k] - sq : T(1);
+        ok = ok && (d > T(0));
+        const T r = mrsqrt(d);
+        ri[k] = r;
+        T t = ink ? b[k] : T(0);
+#pragma unroll
+        for (int j = 0; j < k; ++j) t = mfma(-L[k][j], y[j], t);
+        y[k] = t * r;
+#pragma unroll
+        for (int i = k + 1; i < N; ++i) {
+            const bool ini = (pm >> i) & 1u;
+            T u = T(0);
+#pragma unroll
+            for (int j = 0; j < k; ++j) u = mfma(L[i][j], L[k][j], u);
+            L[i][k] = (ink && ini) ? (M[i][k] - u) * r : T(0);
+        }
+    }
+#pragma unroll
+    for (int i = N - 1; i >= 0; --i) {
+        T t = y[i];
+#pragma unroll
+        for (int j = i + 1; j < N; ++j) t = mfma(-L[j][i], s[j], t);
+        s[i] = t * ri[i];
+    }
+    return ok;
+}
+
+template <typename T>
+MF_HD T sym(const T (&M)[10][10], int i, int j) { return i >= j ? M[i][j] : M[j][i]; }
+
+// inplace_fnnls (nnls/src/inplace_fnnls.cu:72-233) on the normal equations.
+// x: warm start in, solution out (original index order).  eps_rel > 0 adds
+// eps_rel * (|Atb_i| + sum_j |AtA_ij x_j|), a bound on the rounding error of
+// w_i, to the convergence threshold (used by the FP32 path only; the FP64
+// path passes 0 and keeps the reference's test max_w < eps).
+template <typename T, int N>
+MF_HD void fnnls_inplace(const T (&AtA)[10][10], const T (&Atb)[10], T (&x)[10], T eps,
+                         T eps_rel, int max_iter, FnnlsStats &st)
+{
+    uint32_t pm = 0;
+    uint64_t perm = ident_nibbles();  // perm[pos] = original index
+    uint64_t ipos = perm;             // ipos[idx] = position
+    int np = 0;
+    int iter = 0;
+    for (; iter < max_iter; ++iter) {
+        if (np == N) break;                                     // :134
+        // w = Atb - AtA x on the free set; argmax, first in permuted order
+        T wmax = T(0), wtol = T(0);
+        int q = -1, qpos = 16;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            T t = T(0), ta = T(0);
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                const T a = sym(AtA, i, j);
+                t = mfma(a, x[j], t);
+                if (eps_rel > T(0)) ta = mfma(mabs(a), mabs(x[j]), ta);
+            }
+            const T w = Atb[i] - t;
+            const int pos = nib(ipos, i);
+            const bool isfree = !((pm >> i) & 1u);
+            const bool better = (q < 0) || (w > wmax) || (w == wmax && pos < qpos);
+            if (isfree && better) {
+                wmax = w;
+                q = i;
+                qpos = pos;
+                wtol = eps_rel > T(0) ? eps_rel * (mabs(Atb[i]) + ta) : T(0);
+            }
+        }
+        if (wmax < eps + wtol) break;                           // :147
+        {   // swap positions np <-> qpos                          :157-164
+            const int inp = nib(perm, np);
+            perm = setnib(setnib(perm, np, q), qpos, inp);
+            ipos = setnib(setnib(ipos, q, np), inp, qpos);
+        }
+        pm |= 1u << q;
+        ++np;
+        st.outer++;
+
+        for (;;) {                                              // :175
+            T s[10];
+            if (!solve_passive<T, N>(AtA, Atb, pm, s)) st.llt_fail = true;
+            st.inner++;
+            bool allpos = true;
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                if ((pm >> i) & 1u) allpos = allpos && (s[i] > T(0));
+            if (allpos) {                                       // :179-182
+#pragma unroll
+                for (int i = 0; i < N; ++i)
+                    if ((pm >> i) & 1u) x[i] = s[i];
+                break;
+            }
+            T alpha = Lim<T>::max();                            // :188-200
+            int a = -1, apos = 16;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                if (((pm >> i) & 1u) && s[i] <= T(0)) {
+                    const T ratio = mdiv(x[i], x[i] - s[i]);
+                    const int pos = nib(ipos, i);
+                    if (ratio < alpha || (a >= 0 && ratio == alpha && pos < apos)) {
+                        alpha = ratio;
+                        a = i;
+                        apos = pos;
+                    }
+                }
+            }
+            if (a < 0) {                                        // :202-205
+#pragma unroll
+                for (int i = 0; i < N; ++i)
+                    if ((pm >> i) & 1u) x[i] = s[i];
+                break;
+            }
+#pragma unroll
+            for (int i = 0; i < N; ++i)                         // :215-216
+                if ((pm >> i) & 1u) x[i] = (i == a) ? T(0) : mfma(alpha, s[i] - x[i], x[i]);
+            --np;
+            {   // swap positions apos <-> np                      :222-229
+                const int il = nib(perm, np);
+                perm = setnib(setnib(perm, apos, il), np, a);
+                ipos = setnib(setnib(ipos, il, apos), a, np);
+            }
+            pm &= ~(1u << a);
+            if (np == 0) break;
+        }
+    }
+    if (iter == max_iter && max_iter > 0) st.hit_max = true;
+}
+
+// fnnls (nnls/src/fnnls.cu:28-173): boolean-mask FNNLS on the full x.
+// The reference starts its running maximum at index 0 whether or not index 0
+// is still active (:82-95) and then sizes the passive block from a counter
+// that is one short in that case, reading an unset row; here the passive
+// block is the mask itself, as in the oracle.
+template <typename T, int N>
+MF_HD void fnnls_mask(const T (&AtA)[10][10], const T (&Atb)[10], T (&x)[10], T eps,
+                      int max_iter, FnnlsStats &st)
+{
+    uint32_t act = (1u << N) - 1u;  // active (= not passive) set
+    T s[10];
+#pragma unroll
+    for (int i = 0; i < N; ++i) s[i] = T(0);
+    int iter = 0;
+    for (; iter < max_iter; ++iter) {
+        T w[10];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            T t = T(0);
+#pragma unroll
+            for (int j = 0; j < N; ++j) t = mfma(sym(AtA, i, j), x[j], t);
+            w[i] = Atb[i] - t;
+        }
+        int maxi = 0, nact = -1;
+        T wm = w[0];
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if ((act >> i) & 1u) {
+                ++nact;
+                if (w[i] > wm) { wm = w[i]; maxi = i; }
+            }
+        act &= ~(1u << maxi);                                   // :104
+        if (!nact || wm < eps) break;                           // :113
+        st.outer++;
+        for (;;) {                                              // :118
+            const uint32_t pm = ((1u << N) - 1u) & ~act;
+            if (!pm) {
+#pragma unroll
+                for (int i = 0; i < N; ++i) x[i] = s[i];
+                break;
+            }
+            if (!solve_passive<T, N>(AtA, Atb, pm, s)) st.llt_fail = true;
+            st.inner++;
+            T alpha = Lim<T>::max();
+            int a = -1;
+#pragma unroll
+            for (int i = 0; i < N; ++i)                         // :141-149
+                if (((pm >> i) & 1u) && s[i] <= T(0)) {
+                    const T tmp = mdiv(-x[i], s[i] - x[i]);
+                    if (tmp < alpha) { alpha = tmp; a = i; }
+                }
+            if (a < 0) {
+#pragma unroll
+                for (int i = 0; i < N; ++i) x[i] = s[i];
+                break;
+            }
+#pragma unroll
+            for (int i = 0; i < N; ++i)                         // :163-165
+                x[i] = (i == a) ? T(0) : mfma(alpha, s[i] - x[i], x[i]);
+            act |= 1u << a;
+        }
+    }
+    if (iter == max_iter && max_iter > 0) st.hit_max = true;
+}
+
+// ---------------------------------------------------------------------------
+// the per-channel fit (DoFit with errors disabled)
+// ---------------------------------------------------------------------------
+template <typename T> struct FitOut {
+    T x[10];
+    T chi2;
+    int flags;
+    int n_min, n_outer, n_inner;
+};
+
+// Plain fit: the reference's algorithm in precision T.  Li (L^{-1} of the
+// covariance) is parked in `park` between the normal equations and chi2;
+// Park is any type with store(k, v) / load(k) for k < 55.
+template <typename T, int NS, int NP, typename Park>
+MF_HD void fit_plain(const TypeMats<T> &tm0, const T (&s)[10], const FitParams &prm,
+                     Park &park, FitOut<T> &out)
+{
+    T x[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) x[i] = T(0);
+    T chi2 = T(0);
+    int flags = 0, nmin = 0;
+    FnnlsStats st;
+    bool finite = true;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) finite = finite && (s[i] - s[i] == T(0));
+    if (!finite) flags |= F_NONFINITE;
+    const int maxmin = finite ? prm.max_minimize : 0;
+    int it = 0;
+    for (; it < maxmin; ++it) {                                 // :154
+        const TypeMats<T> &tm = *opaque(&tm0);
+        T Li[10][10], ri[10];
+        build_cov<T, NS, NP>(tm, prm.use_pi != 0, x, Li);
+        if (!cholesky_inplace<T, NS>(Li, ri)) flags |= F_LLT_FAIL;
+        invert_lower<T, NS>(Li, ri);
+        T AtA[10][10], Atb[10];
+        normal_equations<T, NS, NP>(tm, Li, s, AtA, Atb);
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+#pragma unroll
+            for (int j = 0; j <= i; ++j) park.store(i * (i + 1) / 2 + j, Li[i][j]);
+        if (prm.solver == 1)
+            fnnls_mask<T, NP>(AtA, Atb, x, T(prm.eps), prm.max_iter, st);
+        else
+            fnnls_inplace<T, NP>(AtA, Atb, x, T(prm.eps), T(0), prm.max_iter, st);
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+#pragma unroll
+            for (int j = 0; j <= i; ++j) Li[i][j] = park.load(i * (i + 1) / 2 + j);
+        const T now = chisq<T, NS, NP>(tm, Li, s, x);
+        const T d = now - chi2;
+        chi2 = now;
+        if (mabs(d) < T(prm.chi2_tol)) break;                   // :162
+    }
+    nmin = (it < maxmin) ? it + 1 : it;
+    if (finite && it == maxmin) flags |= F_MINIMIZE_CAP;
+    if (st.hit_max) flags |= F_FNNLS_CAP;
+    if (st.llt_fail) flags |= F_LLT_FAIL;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) out.x[i] = x[i];
+    out.chi2 = chi2;
+    out.flags = flags;
+    out.n_min = nmin;
+    out.n_outer = st.outer;
+    out.n_inner = st.inner;
+}
+
+// Register-array park for host builds and for kernels that keep L^{-1} live.
+template <typename T> struct RegPark {
+    T v[55];
+    MF_HD void store(int k, T x) { v[k] = x; }
+    MF_HD T load(int k) const { return v[k]; }
+};
+
+}  // namespace mf

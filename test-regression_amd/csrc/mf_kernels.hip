@@ -1,0 +1,519 @@
+// mf_kernels.hip -- gfx950 kernels of the multifit engine and the C ABI
+// (include/mf_abi.h).
+//
+// Fit kernel: one channel per lane.  Samples are read structure-of-arrays
+// (samples[i*nch + c]: 64 lanes read 64 consecutive values per sample row),
+// the per-type constants (pulse matrix, noise covariance, pulse covariance
+// blocks) are wave-uniform and are read through the scalar cache, and the
+// whole fit -- covariance, its Cholesky and inverse, the whitened normal
+// equations, the FNNLS active-set loop and chi2 -- runs in registers.  The
+// one value parked in LDS is L^{-1} of the covariance, between the normal
+// equations and chi2, laid out [element][lane] so that 64 lanes touch 64
+// consecutive words.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/mf_abi.h"
+#include "fit_core.hpp"
+#include "types_host.hpp"
+
+using namespace mf;
+
+// ---------------------------------------------------------------------------
+// error reporting
+// ---------------------------------------------------------------------------
+static thread_local char g_err[512] = "";
+
+static mf_status fail(mf_status st, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return st;
+}
+
+#define HIP_TRY(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(MF_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));     \
+    } while (0)
+
+extern "C" const char *mf_last_error(void) { return g_err; }
+extern "C" int mf_abi_version(void) { return MF_ABI_VERSION; }
+
+// ---------------------------------------------------------------------------
+// fit kernels
+// ---------------------------------------------------------------------------
+template <typename T> struct LdsPark {
+    T *base;
+    int stride;
+    __device__ __forceinline__ void store(int k, T v) { base[k * stride] = v; }
+    __device__ __forceinline__ T load(int k) const { return base[k * stride]; }
+};
+
+template <typename T, typename TS, int N>
+__global__ __launch_bounds__(256)
+void k_fit_plain(const DevType *__restrict__ types, int type_id,
+                                                   const int32_t *__restrict__ idx,
+                                                   const int64_t *__restrict__ count,
+                                                   int64_t nitems, const TS *__restrict__ samples,
+                                                   int64_t nch, T *__restrict__ xo,
+                                                   T *__restrict__ chi2o, int32_t *__restrict__ flo,
+                                                   int32_t *__restrict__ ito, FitParams prm)
+{
+    extern __shared__ unsigned char lds_raw[];
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = count ? *count : nitems;
+    if (g >= n) return;
+    const int64_t c = idx ? (int64_t)idx[g] : g;
+    const TypeMats<T> &tm = mats<T>(types[type_id]);
+    T s[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) s[i] = i < N ? (T)samples[(int64_t)i * nch + c] : T(0);
+    LdsPark<T> park{reinterpret_cast<T *>(lds_raw) + threadIdx.x, (int)blockDim.x};
+    FitOut<T> out;
+    fit_plain<T, N, N>(tm, s, prm, park, out);
+#pragma unroll
+    for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = out.x[p];
+    chi2o[c] = out.chi2;
+    flo[c] = out.flags;
+    if (ito) ito[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
+}
+
+// Partition of a mixed batch by channel type: counts, offsets, index lists.
+// part layout: counts[256] (int64), then cursors[256] (int64).
+__global__ void k_type_count(const uint8_t *__restrict__ type_of, int64_t nch,
+                             unsigned long long *__restrict__ counts)
+{
+    __shared__ unsigned int loc[256];
+    for (int t = threadIdx.x; t < 256; t += blockDim.x) loc[t] = 0;
+    __syncthreads();
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nch;
+         c += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&loc[type_of[c]], 1u);
+    __syncthreads();
+    for (int t = threadIdx.x; t < 256; t += blockDim.x)
+        if (loc[t]) atomicAdd(&counts[t], (unsigned long long)loc[t]);
+}
+
+__global__ void k_type_scatter(const uint8_t *__restrict__ type_of, int64_t nch,
+                               const int64_t *__restrict__ offsets,
+                               unsigned long long *__restrict__ cursors, int32_t *__restrict__ idx)
+{
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nch;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const int t = type_of[c];
+        const unsigned long long k = atomicAdd(&cursors[t], 1ull);
+        idx[offsets[t] + (int64_t)k] = (int32_t)c;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// solver-only kernels (inplace_fnnls / fnnls on given A, b)
+// ---------------------------------------------------------------------------
+template <int M>
+__device__ void solve_one(const double *__restrict__ A, const double *__restrict__ b,
+                          double *__restrict__ x, int solver, double eps, unsigned max_iter)
+{
+    // A column-major, leading dimension 10: A[col*10 + row]
+    double AtA[10][10], Atb[10], xx[10];
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < M; ++k) t = fma(A[i * 10 + k], A[j * 10 + k], t);
+            AtA[i][j] = t;
+        }
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) t = fma(A[i * 10 + k], b[k], t);
+        Atb[i] = t;
+        xx[i] = x[i];
+    }
+    FnnlsStats st;
+    if (solver == 1)
+        fnnls_mask<double, M>(AtA, Atb, xx, eps, (int)max_iter, st);
+    else
+        fnnls_inplace<double, M>(AtA, Atb, xx, eps, 0.0, (int)max_iter, st);
+#pragma unroll
+    for (int i = 0; i < M; ++i) x[i] = xx[i];
+}
+
+__global__ __launch_bounds__(256) void k_nnls(const double *__restrict__ A,
+                                              const double *__restrict__ b, double *__restrict__ x,
+                                              int64_t n, int m, int solver, double eps,
+                                              unsigned max_iter)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const double *Ac = A + c * 100;
+    const double *bc = b + c * 10;
+    double *xc = x + c * 10;
+    switch (m) {
+    case 1: solve_one<1>(Ac, bc, xc, solver, eps, max_iter); break;
+    case 2: solve_one<2>(Ac, bc, xc, solver, eps, max_iter); break;
+    case 3: solve_one<3>(Ac, bc, xc, solver, eps, max_iter); break;
+    case 4: solve_one<4>(Ac, bc, xc, solver, eps, max_iter); break;
+    case 5: solve_one<5>(Ac, bc, xc, solver, eps, max_iter); break;
+    case 6: solve_one<6>(Ac, bc, xc, solver, eps, max_iter); break;
+    case 7: solve_one<7>(Ac, bc, xc, solver, eps, max_iter); break;
+    case 8: solve_one<8>(Ac, bc, xc, solver, eps, max_iter); break;
+    case 9: solve_one<9>(Ac, bc, xc, solver, eps, max_iter); break;
+    default: solve_one<10>(Ac, bc, xc, solver, eps, max_iter); break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// linalgebra primitives, one problem per lane, row-major full x full
+// (operation order as linalgebra/interface/*.hpp)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_linalg(int op, T *M, T *L, T *b, T *y, int64_t n, int full, int view, int pi,
+                         int pj)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int64_t mo = c * full * full, vo = c * full;
+#define AT(P, r, q) P[mo + (int64_t)(r) * full + (q)]
+    switch (op) {
+    case MF_LA_CHOLESKY:  // cholesky.hpp:8-38
+        for (int i = 0; i < view; ++i) {
+            T sumsq = 0;
+            for (int j = 0; j < i; ++j) {
+                T s2 = 0;
+                for (int k = 0; k < j; ++k) s2 += AT(L, i, k) * AT(L, j, k);
+                AT(L, i, j) = (AT(M, i, j) - s2) / AT(L, j, j);
+                sumsq += AT(L, i, j) * AT(L, i, j);
+            }
+            AT(L, i, i) = sqrt(AT(M, i, i) - sumsq);
+        }
+        break;
+    case MF_LA_FORWARD:  // substitution.hpp:10-26
+        y[vo] = b[vo] / AT(L, 0, 0);
+        for (int i = 1; i < view; ++i) {
+            T t = b[vo + i];
+            for (int j = 0; j < i; ++j) t -= AT(L, i, j) * y[vo + j];
+            y[vo + i] = t / AT(L, i, i);
+        }
+        break;
+    case MF_LA_BACKWARD:  // substitution.hpp:34-50
+        y[vo + view - 1] = b[vo + view - 1] / AT(L, view - 1, view - 1);
+        for (int i = view - 2; i >= 0; --i) {
+            T t = b[vo + i];
+            for (int j = i + 1; j < view; ++j) t -= AT(L, j, i) * y[vo + j];
+            y[vo + i] = t / AT(L, i, i);
+        }
+        break;
+    case MF_LA_SWAP: {  // common.hpp:11-42 (j > i)
+        const int i = pi, j = pj;
+        T t = AT(M, i, i);
+        AT(M, i, i) = AT(M, j, j);
+        AT(M, j, j) = t;
+        for (int e = 0; e < i; ++e) {
+            T tmp = AT(M, i, e);
+            AT(M, i, e) = AT(M, j, e);
+            AT(M, j, e) = tmp;
+            AT(M, e, i) = AT(M, e, j);
+            AT(M, e, j) = tmp;
+        }
+        for (int e = i + 1; e < view; ++e) {
+            T tmp = AT(M, i, e);
+            AT(M, i, e) = AT(M, j, e);
+            AT(M, j, e) = tmp;
+            AT(M, e, i) = AT(M, e, j);
+            AT(M, e, j) = tmp;
+        }
+        break;
+    }
+    case MF_LA_RCADDITION: {  // fused_cholesky_substitution.hpp:18-57
+        const int row = view - 1;
+        T sum = 0, total = b[vo + row];
+        for (int col = 0; col < row; ++col) {
+            T s2 = 0;
+            for (int j = 0; j < col; ++j) s2 += AT(L, row, j) * AT(L, col, j);
+            const T v = (AT(M, row, col) - s2) / AT(L, col, col);
+            total -= v * y[vo + col];
+            sum += v * v;
+            AT(L, row, col) = v;
+        }
+        const T d = sqrt(AT(M, row, row) - sum);
+        AT(L, row, row) = d;
+        y[vo + row] = total / d;
+        break;
+    }
+    case MF_LA_INBETWEEN:  // fused_cholesky_substitution.hpp:62-102
+        for (int i = pi; i < view; ++i) {
+            T sumsq = 0, total = b[vo + i];
+            for (int j = 0; j < i; ++j) {
+                T s2 = 0;
+                for (int k = 0; k < j; ++k) s2 += AT(L, i, k) * AT(L, j, k);
+                const T v = (AT(M, i, j) - s2) / AT(L, j, j);
+                AT(L, i, j) = v;
+                total -= v * y[vo + j];
+                sumsq += v * v;
+            }
+            const T d = sqrt(AT(M, i, i) - sumsq);
+            AT(L, i, i) = d;
+            y[vo + i] = total / d;
+        }
+        break;
+    }
+#undef AT
+}
+
+// ---------------------------------------------------------------------------
+// context and host entry points
+// ---------------------------------------------------------------------------
+struct mf_context {
+    int device = 0;
+    std::vector<mf_chtype> types;
+    DevType *d_types = nullptr;
+    // partition scratch
+    void *d_part = nullptr;  // counts[256], cursors[256], offsets[256] (int64)
+    int32_t *d_idx = nullptr;
+    int64_t idx_cap = 0;
+};
+
+extern "C" mf_status mf_default_opts(mf_opts *o)
+{
+    if (!o) return fail(MF_ERR_INVALID, "opts is NULL");
+    memset(o, 0, sizeof(*o));
+    o->precision = MF_PREC_FP64;
+    o->solver = MF_SOLVER_INPLACE_FNNLS;
+    o->max_iter = 1000;      // multifit_gpu/src/PulseChiSqSNNLS.cu:215
+    o->max_minimize = 50;    // multifit_cpu/src/PulseChiSqSNNLS.cpp:153
+    o->eps = 1e-11;          // PulseChiSqSNNLS.cpp:242
+    o->chi2_tol = 1e-3;      // PulseChiSqSNNLS.cpp:162
+    o->use_pulse_cov = 1;
+    return MF_OK;
+}
+
+extern "C" mf_status mf_context_create(mf_context **ctx, int device)
+{
+    if (!ctx) return fail(MF_ERR_INVALID, "ctx is NULL");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+        return fail(MF_ERR_INVALID, "device %d out of range (%d devices)", device, ndev);
+    mf_context *c = new mf_context;
+    c->device = device;
+    *ctx = c;
+    return MF_OK;
+}
+
+extern "C" mf_status mf_context_destroy(mf_context *ctx)
+{
+    if (!ctx) return MF_OK;
+    hipSetDevice(ctx->device);
+    if (ctx->d_types) hipFree(ctx->d_types);
+    if (ctx->d_part) hipFree(ctx->d_part);
+    if (ctx->d_idx) hipFree(ctx->d_idx);
+    delete ctx;
+    return MF_OK;
+}
+
+extern "C" mf_status mf_set_types(mf_context *ctx, const mf_chtype *types, int32_t ntypes)
+{
+    if (!ctx || !types) return fail(MF_ERR_INVALID, "ctx or types is NULL");
+    if (ntypes <= 0 || ntypes > 256) return fail(MF_ERR_INVALID, "ntypes must be in 1..256");
+    std::vector<DevType> h((size_t)ntypes);
+    for (int t = 0; t < ntypes; ++t) {
+        const char *e = expand_type(types[t], h[(size_t)t]);
+        if (e) return fail(MF_ERR_INVALID, "type %d: %s", t, e);
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (ctx->d_types) {
+        HIP_TRY(hipFree(ctx->d_types));
+        ctx->d_types = nullptr;
+    }
+    HIP_TRY(hipMalloc(&ctx->d_types, sizeof(DevType) * (size_t)ntypes));
+    HIP_TRY(hipMemcpy(ctx->d_types, h.data(), sizeof(DevType) * (size_t)ntypes,
+                      hipMemcpyHostToDevice));
+    ctx->types.assign(types, types + ntypes);
+    return MF_OK;
+}
+
+template <typename T, typename TS, int N>
+static mf_status launch_fit(mf_context *ctx, int type_id, const int32_t *idx,
+                            const int64_t *count, int64_t nitems, const void *samples,
+                            int64_t nch, const mf_result *out, const FitParams &prm, int block,
+                            hipStream_t st)
+{
+    if (nitems <= 0) return MF_OK;
+    const int64_t grid = (nitems + block - 1) / block;
+    const size_t lds = sizeof(T) * 55 * (size_t)block;
+    hipLaunchKernelGGL((k_fit_plain<T, TS, N>), dim3((unsigned)grid), dim3(block), lds, st,
+                       ctx->d_types, type_id, idx, count, nitems,
+                       static_cast<const TS *>(samples), nch, static_cast<T *>(out->x),
+                       static_cast<T *>(out->chi2), out->flags, out->iters, prm);
+    HIP_TRY(hipGetLastError());
+    return MF_OK;
+}
+
+template <typename T>
+static mf_status dispatch_fit(mf_context *ctx, int type_id, const int32_t *idx,
+                              const int64_t *count, int64_t nitems, const mf_batch *in,
+                              const mf_result *out, const FitParams &prm, int block,
+                              hipStream_t st)
+{
+    const int ns = ctx->types[(size_t)type_id].nsamples;
+    const bool f64in = in->sample_dtype == MF_DTYPE_F64;
+    if (ns == 10)
+        return f64in ? launch_fit<T, double, 10>(ctx, type_id, idx, count, nitems, in->samples,
+                                                 in->nch, out, prm, block, st)
+                     : launch_fit<T, float, 10>(ctx, type_id, idx, count, nitems, in->samples,
+                                                in->nch, out, prm, block, st);
+    return f64in ? launch_fit<T, double, 8>(ctx, type_id, idx, count, nitems, in->samples,
+                                            in->nch, out, prm, block, st)
+                 : launch_fit<T, float, 8>(ctx, type_id, idx, count, nitems, in->samples, in->nch,
+                                           out, prm, block, st);
+}
+
+extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_result *out,
+                                  const mf_opts *opts, void *stream)
+{
+    if (!ctx || !in || !out || !opts) return fail(MF_ERR_INVALID, "NULL argument");
+    if (ctx->types.empty()) return fail(MF_ERR_INVALID, "no channel types set (mf_set_types)");
+    if (in->nch < 0 || in->nch > (int64_t)INT32_MAX)
+        return fail(MF_ERR_INVALID, "nch must be in 0..2^31-1");
+    if (in->nch == 0) return MF_OK;
+    if (!in->samples || !out->x || !out->chi2 || !out->flags)
+        return fail(MF_ERR_INVALID, "samples, x, chi2 and flags are required");
+    if (in->sample_dtype != MF_DTYPE_F64 && in->sample_dtype != MF_DTYPE_F32)
+        return fail(MF_ERR_INVALID, "sample_dtype must be MF_DTYPE_F64 or MF_DTYPE_F32");
+    if (opts->precision != MF_PREC_FP64 && opts->precision != MF_PREC_FP32)
+        return fail(MF_ERR_INVALID, "precision must be MF_PREC_FP64 or MF_PREC_FP32");
+    if (opts->max_minimize < 0 || opts->max_iter < 0)
+        return fail(MF_ERR_INVALID, "max_minimize and max_iter must be >= 0");
+    if (opts->solver != MF_SOLVER_INPLACE_FNNLS && opts->solver != MF_SOLVER_FNNLS)
+        return fail(MF_ERR_INVALID, "unknown solver");
+    if (opts->warm_start) return fail(MF_ERR_UNSUPPORTED, "warm start is not available yet");
+    int block = opts->block > 0 ? opts->block : 64;
+    if (block % 64 || block > 256) return fail(MF_ERR_INVALID, "block must be 64, 128, 192 or 256");
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    FitParams prm;
+    prm.eps = opts->eps;
+    prm.chi2_tol = opts->chi2_tol;
+    prm.max_iter = opts->max_iter;
+    prm.max_minimize = opts->max_minimize;
+    prm.use_pi = opts->use_pulse_cov;
+    prm.solver = opts->solver;
+    const int ntypes = (int)ctx->types.size();
+    auto go = [&](int type_id, const int32_t *idx, const int64_t *count, int64_t nitems) {
+        return opts->precision == MF_PREC_FP64
+                   ? dispatch_fit<double>(ctx, type_id, idx, count, nitems, in, out, prm, block, st)
+                   : dispatch_fit<float>(ctx, type_id, idx, count, nitems, in, out, prm, block, st);
+    };
+    if (!in->type_of || ntypes == 1) return go(0, nullptr, nullptr, in->nch);
+    // mixed batch: partition channel indices by type on the device
+    if (!ctx->d_part) HIP_TRY(hipMalloc(&ctx->d_part, sizeof(int64_t) * 768));
+    if (ctx->idx_cap < in->nch) {
+        if (ctx->d_idx) HIP_TRY(hipFree(ctx->d_idx));
+        ctx->d_idx = nullptr;
+        HIP_TRY(hipMalloc(&ctx->d_idx, sizeof(int32_t) * (size_t)in->nch));
+        ctx->idx_cap = in->nch;
+    }
+    int64_t *counts = static_cast<int64_t *>(ctx->d_part);
+    int64_t *cursors = counts + 256, *offsets = counts + 512;
+    HIP_TRY(hipMemsetAsync(counts, 0, sizeof(int64_t) * 512, st));
+    const int pgrid = (int)std::min<int64_t>(2048, (in->nch + 255) / 256);
+    hipLaunchKernelGGL(k_type_count, dim3(pgrid), dim3(256), 0, st, in->type_of, in->nch,
+                       reinterpret_cast<unsigned long long *>(counts));
+    HIP_TRY(hipGetLastError());
+    // offsets from counts: ntypes is small; one host round trip keeps it simple
+    int64_t hc[256];
+    HIP_TRY(hipMemcpyAsync(hc, counts, sizeof(int64_t) * 256, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    int64_t ho[256], acc = 0;
+    for (int t = 0; t < 256; ++t) {
+        ho[t] = acc;
+        acc += hc[t];
+        if (hc[t] && t >= ntypes)
+            return fail(MF_ERR_INVALID, "type_of holds type %d but only %d types are set", t,
+                        ntypes);
+    }
+    HIP_TRY(hipMemcpyAsync(offsets, ho, sizeof(int64_t) * 256, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_type_scatter, dim3(pgrid), dim3(256), 0, st, in->type_of, in->nch,
+                       offsets, reinterpret_cast<unsigned long long *>(cursors), ctx->d_idx);
+    HIP_TRY(hipGetLastError());
+    for (int t = 0; t < ntypes; ++t) {
+        if (!hc[t]) continue;
+        mf_status s = go(t, ctx->d_idx + ho[t], nullptr, hc[t]);
+        if (s != MF_OK) return s;
+    }
+    // hc/ho are host stack arrays: make sure the H2D copy of ho finished
+    HIP_TRY(hipStreamSynchronize(st));
+    return MF_OK;
+}
+
+static mf_status nnls_batch(const double *A, const double *b, double *x, int64_t n, int32_t m,
+                            double eps, uint32_t max_iter, int solver, void *stream)
+{
+    if (n < 0) return fail(MF_ERR_INVALID, "n < 0");
+    if (n == 0) return MF_OK;
+    if (!A || !b || !x) return fail(MF_ERR_INVALID, "NULL argument");
+    if (m < 1 || m > 10) return fail(MF_ERR_INVALID, "m must be in 1..10");
+    const int block = 64;
+    hipLaunchKernelGGL(k_nnls, dim3((unsigned)((n + block - 1) / block)), dim3(block), 0,
+                       static_cast<hipStream_t>(stream), A, b, x, n, m, solver, eps, max_iter);
+    HIP_TRY(hipGetLastError());
+    return MF_OK;
+}
+
+extern "C" mf_status mf_inplace_fnnls_batch(const double *A, const double *b, double *x,
+                                            int64_t n, int32_t m, double eps, uint32_t max_iter,
+                                            void *stream)
+{
+    return nnls_batch(A, b, x, n, m, eps, max_iter, 0, stream);
+}
+
+extern "C" mf_status mf_fnnls_batch(const double *A, const double *b, double *x, int64_t n,
+                                    int32_t m, double eps, uint32_t max_iter, void *stream)
+{
+    return nnls_batch(A, b, x, n, m, eps, max_iter, 1, stream);
+}
+
+extern "C" mf_status mf_linalg_batch(int32_t op, int32_t dtype, void *M, void *L, void *b, void *y,
+                                     int64_t n, int32_t full, int32_t view, int32_t i, int32_t j,
+                                     void *stream)
+{
+    if (n < 0) return fail(MF_ERR_INVALID, "n < 0");
+    if (n == 0) return MF_OK;
+    if (full < 1 || full > 10 || view < 1 || view > full)
+        return fail(MF_ERR_INVALID, "need 1 <= view <= full <= 10");
+    if (op < MF_LA_CHOLESKY || op > MF_LA_INBETWEEN) return fail(MF_ERR_INVALID, "unknown op");
+    if (op == MF_LA_SWAP && !(0 <= i && i < j && j < view))
+        return fail(MF_ERR_INVALID, "swap needs 0 <= i < j < view");
+    if (op == MF_LA_INBETWEEN && !(0 <= i && i < view))
+        return fail(MF_ERR_INVALID, "position must be in 0..view-1");
+    const bool needM = op == MF_LA_CHOLESKY || op == MF_LA_SWAP || op == MF_LA_RCADDITION ||
+                       op == MF_LA_INBETWEEN;
+    const bool needL = op != MF_LA_SWAP;
+    const bool needv = op != MF_LA_CHOLESKY && op != MF_LA_SWAP;
+    if ((needM && !M) || (needL && !L) || (needv && (!b || !y)))
+        return fail(MF_ERR_INVALID, "missing operand for op %d", op);
+    const int block = 64;
+    const dim3 grid((unsigned)((n + block - 1) / block));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (dtype == MF_DTYPE_F64)
+        hipLaunchKernelGGL(k_linalg<double>, grid, dim3(block), 0, st, op,
+                           static_cast<double *>(M), static_cast<double *>(L),
+                           static_cast<double *>(b), static_cast<double *>(y), n, full, view, i, j);
+    else if (dtype == MF_DTYPE_F32)
+        hipLaunchKernelGGL(k_linalg<float>, grid, dim3(block), 0, st, op, static_cast<float *>(M),
+                           static_cast<float *>(L), static_cast<float *>(b),
+                           static_cast<float *>(y), n, full, view, i, j);
+    else
+        return fail(MF_ERR_INVALID, "dtype must be MF_DTYPE_F64 or MF_DTYPE_F32");
+    HIP_TRY(hipGetLastError());
+    return MF_OK;
+}

@@ -1,0 +1,54 @@
+// types_host.hpp -- expands a channel type (mf_chtype) into the per-type
+// tables the fit kernels read (mf::DevType).  Host code.
+#pragma once
+
+#include <string.h>
+
+#include "../../include/mf_abi.h"
+#include "fit_core.hpp"
+
+namespace mf {
+
+inline int imax_(int a, int b) { return a > b ? a : b; }
+
+// Returns 0 or an error string.
+inline const char *expand_type(const mf_chtype &t, DevType &d)
+{
+    if (!((t.nsamples == 10 && t.npulses == 10) || (t.nsamples == 8 && t.npulses == 8)))
+        return "channel type shape must be 10x10 or 8x8 (nsamples x npulses)";
+    memset(&d, 0, sizeof(d));
+    const int ns = t.nsamples, np = t.npulses;
+    d.h.ns = ns;
+    d.h.np = np;
+    d.h.intime = -1;
+    bool has_pi = false;
+    for (int i = 0; i < MF_FULL * MF_FULL; ++i)
+        if (t.fullpulsecov[i] != 0.0) has_pi = true;
+    d.h.has_pi = has_pi ? 1 : 0;
+    const double p2 = t.pederr * t.pederr;
+    for (int i = 0; i < ns; ++i)
+        for (int j = 0; j < ns; ++j) d.d.C0[i][j] = p2 * t.noisecor[i * 10 + j];
+    for (int p = 0; p < np; ++p) {
+        const int bx = t.bx[p];
+        if (bx == 0 && d.h.intime < 0) d.h.intime = p;
+        const int f = imax_(0, bx + t.first_off);   // PulseChiSqSNNLS.cpp:54
+        const int o = t.pulse_off - bx;             // PulseChiSqSNNLS.cpp:55
+        if (f < ns && (f + o < 0 || ns - 1 + o >= MF_FULL))
+            return "pulse window falls outside fullpulse (check bx, first_off, pulse_off)";
+        d.h.first[p] = f;
+        for (int i = f; i < ns; ++i) {
+            d.d.P[i][p] = t.fullpulse[i + o];
+            for (int j = f; j < ns; ++j)
+                d.d.Pi[p][i][j] = t.fullpulsecov[(i + o) * MF_FULL + (j + o)];
+        }
+    }
+    for (int i = 0; i < 10; ++i)
+        for (int j = 0; j < 10; ++j) {
+            d.f.P[i][j] = (float)d.d.P[i][j];
+            d.f.C0[i][j] = (float)d.d.C0[i][j];
+            for (int p = 0; p < 10; ++p) d.f.Pi[p][i][j] = (float)d.d.Pi[p][i][j];
+        }
+    return 0;
+}
+
+}  // namespace mf

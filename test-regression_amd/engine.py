@@ -1,0 +1,139 @@
+"""Batched multifit on the GPU through libmf_amd.so.
+
+torch is used only as device-memory and stream plumbing: tensors hold the
+structure-of-arrays batch and results, the fit itself runs in the library's
+gfx950 kernels (mf_fit_batch).  Nothing here computes a fit on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import torch
+
+from . import _abi
+
+
+@dataclass
+class FitResult:
+    x: torch.Tensor        # [npulses_max=10, nch] amplitudes (pulse order of the type)
+    chi2: torch.Tensor     # [nch]
+    flags: torch.Tensor    # [nch] int32 MF_F_* bits
+    iters: torch.Tensor    # [nch] int32: n_minimize | outer << 8 | inner << 16
+
+    def n_minimize(self):
+        return self.iters & 0xFF
+
+    def n_outer(self):
+        return (self.iters >> 8) & 0xFF
+
+    def n_inner(self):
+        return (self.iters >> 16) & 0xFFFF
+
+
+def _stream_ptr(stream) -> int | None:
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return stream.cuda_stream
+
+
+class Engine:
+    """One device context holding the channel-type table (mf_context)."""
+
+    def __init__(self, types, device: int = 0):
+        self._lib = _abi.lib()
+        if not torch.cuda.is_available():
+            raise _abi.MFError("no GPU visible: the multifit engine runs on the GPU only")
+        self.device = device
+        self._ctx = C.c_void_p()
+        _abi.check(self._lib.mf_context_create(C.byref(self._ctx), device), "mf_context_create")
+        self.set_types(types)
+
+    def set_types(self, types):
+        if isinstance(types, _abi.ChType):
+            types = [types]
+        arr = (_abi.ChType * len(types))(*types)
+        _abi.check(self._lib.mf_set_types(self._ctx, arr, len(types)), "mf_set_types")
+        self.types = list(arr)
+
+    def close(self):
+        if self._ctx:
+            self._lib.mf_context_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def alloc_result(self, nch: int, precision: int = _abi.MF_PREC_FP64) -> FitResult:
+        dt = torch.float64 if precision == _abi.MF_PREC_FP64 else torch.float32
+        dev = torch.device("cuda", self.device)
+        return FitResult(
+            x=torch.zeros((10, nch), dtype=dt, device=dev),
+            chi2=torch.zeros(nch, dtype=dt, device=dev),
+            flags=torch.zeros(nch, dtype=torch.int32, device=dev),
+            iters=torch.zeros(nch, dtype=torch.int32, device=dev),
+        )
+
+    def fit(self, samples: torch.Tensor, type_of: torch.Tensor | None = None, opts=None,
+            out: FitResult | None = None, stream=None) -> FitResult:
+        """samples: [>=nsamples, nch] float64/float32 on the device (row i = sample i)."""
+        if opts is None:
+            opts = _abi.default_opts()
+        if samples.dim() != 2 or not samples.is_cuda:
+            raise ValueError("samples must be a 2-D device tensor [samples, nch]")
+        samples = samples.contiguous()
+        nch = samples.shape[1]
+        need = max(t.nsamples for t in self.types)
+        if samples.shape[0] < need:
+            raise ValueError(f"samples has {samples.shape[0]} rows, the types need {need}")
+        if type_of is not None and (type_of.dtype != torch.uint8 or type_of.numel() != nch
+                                    or not type_of.is_cuda):
+            raise ValueError("type_of must be a uint8 device tensor with one entry per channel")
+        if samples.dtype == torch.float64:
+            sdt = _abi.MF_DTYPE_F64
+        elif samples.dtype == torch.float32:
+            sdt = _abi.MF_DTYPE_F32
+        else:
+            raise ValueError("samples must be float64 or float32")
+        if out is None:
+            out = self.alloc_result(nch, opts.precision)
+        b = _abi.Batch(nch=nch, sample_dtype=sdt, pad=0, samples=samples.data_ptr(),
+                       type_of=type_of.data_ptr() if type_of is not None else None)
+        r = _abi.Result(x=out.x.data_ptr(), chi2=out.chi2.data_ptr(),
+                        flags=out.flags.data_ptr(), iters=out.iters.data_ptr())
+        _abi.check(self._lib.mf_fit_batch(self._ctx, C.byref(b), C.byref(r), C.byref(opts),
+                                          C.c_void_p(_stream_ptr(stream))), "mf_fit_batch")
+        return out
+
+
+def nnls_batch(A: torch.Tensor, b: torch.Tensor, x: torch.Tensor, m: int = 10,
+               eps: float = 1e-11, max_iter: int = 1000, solver: str = "inplace_fnnls",
+               stream=None) -> torch.Tensor:
+    """Batched inplace_fnnls / fnnls.  A [n,10,10] column-major per problem
+    (A[c, col, row]), b [n,10], x [n,10] in/out (warm start)."""
+    L = _abi.lib()
+    for t in (A, b, x):
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+            raise ValueError("A, b, x must be contiguous float64 device tensors")
+    f = L.mf_inplace_fnnls_batch if solver == "inplace_fnnls" else L.mf_fnnls_batch
+    _abi.check(f(A.data_ptr(), b.data_ptr(), x.data_ptr(), A.shape[0], m, eps, max_iter,
+                 C.c_void_p(_stream_ptr(stream))), solver)
+    return x
+
+
+def linalg_batch(op: int, M=None, L=None, b=None, y=None, full: int = 10, view: int = 10,
+                 i: int = 0, j: int = 0, stream=None):
+    """Batched linalgebra primitive (see mf_linalg_batch)."""
+    ts = [t for t in (M, L, b, y) if t is not None]
+    dt = ts[0].dtype
+    if dt not in (torch.float64, torch.float32) or any(t.dtype != dt for t in ts):
+        raise ValueError("operands must share one dtype, float64 or float32")
+    n = (M if M is not None else L).shape[0]
+    ptr = lambda t: t.data_ptr() if t is not None else None  # noqa: E731
+    _abi.check(_abi.lib().mf_linalg_batch(
+        op, _abi.MF_DTYPE_F64 if dt == torch.float64 else _abi.MF_DTYPE_F32,
+        ptr(M), ptr(L), ptr(b), ptr(y), n, full, view, i, j,
+        C.c_void_p(_stream_ptr(stream))), "mf_linalg_batch")

@@ -1,0 +1,85 @@
+// hostcore.cpp -- TEST HARNESS: the product's per-channel fit code
+// (test-regression_amd/csrc/fit_core.hpp) compiled for the host CPU, so that
+// CPU-only test runs can compare the kernel's arithmetic with the oracle.  The
+// product library never uses this build; on a GPU the tests call the gfx950
+// kernels through the C ABI instead.
+#include <stdint.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/mf_abi.h"
+#include "../../test-regression_amd/csrc/fit_core.hpp"
+#include "../../test-regression_amd/csrc/types_host.hpp"
+
+using namespace mf;
+
+template <typename T, int N>
+static void run(const DevType &dt, const double *samples, int64_t nch, int64_t c,
+                const FitParams &prm, double *x, double *chi2, int32_t *flags, int32_t *iters)
+{
+    T s[10];
+    for (int i = 0; i < 10; ++i) s[i] = i < N ? (T)samples[i * nch + c] : T(0);
+    RegPark<T> park;
+    FitOut<T> out;
+    fit_plain<T, N, N>(mats<T>(dt), s, prm, park, out);
+    for (int p = 0; p < 10; ++p) x[p * nch + c] = p < N ? (double)out.x[p] : 0.0;
+    chi2[c] = (double)out.chi2;
+    flags[c] = out.flags;
+    iters[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
+}
+
+extern "C" int hc_fit_batch(const mf_chtype *types, int ntypes, const uint8_t *type_of,
+                            const double *samples, int64_t nch, const mf_opts *o, double *x,
+                            double *chi2, int32_t *flags, int32_t *iters)
+{
+    std::vector<DevType> dt((size_t)ntypes);
+    for (int t = 0; t < ntypes; ++t)
+        if (expand_type(types[t], dt[(size_t)t])) return 1;
+    FitParams prm;
+    prm.eps = o->eps;
+    prm.chi2_tol = o->chi2_tol;
+    prm.max_iter = o->max_iter;
+    prm.max_minimize = o->max_minimize;
+    prm.use_pi = o->use_pulse_cov;
+    prm.solver = o->solver;
+    for (int64_t c = 0; c < nch; ++c) {
+        const DevType &d = dt[type_of ? type_of[c] : 0];
+        const bool f64 = o->precision == MF_PREC_FP64;
+        if (d.h.ns == 10) {
+            if (f64) run<double, 10>(d, samples, nch, c, prm, x, chi2, flags, iters);
+            else run<float, 10>(d, samples, nch, c, prm, x, chi2, flags, iters);
+        } else {
+            if (f64) run<double, 8>(d, samples, nch, c, prm, x, chi2, flags, iters);
+            else run<float, 8>(d, samples, nch, c, prm, x, chi2, flags, iters);
+        }
+    }
+    return 0;
+}
+
+// single NNLS problems (A column-major ld 10, m x m) through fit_core's solvers
+extern "C" void hc_nnls(const double *A, const double *b, double *x, int m, int solver,
+                        double eps, unsigned max_iter)
+{
+    double AtA[10][10] = {{0}}, Atb[10] = {0}, xx[10] = {0};
+    for (int i = 0; i < m; ++i) {
+        for (int j = 0; j <= i; ++j) {
+            double t = 0;
+            for (int k = 0; k < m; ++k) t = fma(A[i * 10 + k], A[j * 10 + k], t);
+            AtA[i][j] = t;
+        }
+        double t = 0;
+        for (int k = 0; k < m; ++k) t = fma(A[i * 10 + k], b[k], t);
+        Atb[i] = t;
+        xx[i] = x[i];
+    }
+    FnnlsStats st;
+#define CASE(M)                                                                   \
+    case M:                                                                       \
+        if (solver == 1) fnnls_mask<double, M>(AtA, Atb, xx, eps, (int)max_iter, st); \
+        else fnnls_inplace<double, M>(AtA, Atb, xx, eps, 0.0, (int)max_iter, st); \
+        break;
+    switch (m) { CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) }
+#undef CASE
+    for (int i = 0; i < m; ++i) x[i] = xx[i];
+}

@@ -1,0 +1,145 @@
+"""GPU parity of mf_fit_batch (plain FP64 path) against the FP64 oracle.
+
+Bar (FP64): identical active sets, amplitudes within 1e-10 of the oracle
+relative to the channel's largest amplitude (floor 1 ADC), chi2 within 1e-10
+relative (floor 1), identical Minimize / FNNLS iteration counts on >= 99.9%
+of channels.  The oracle follows the reference's operation order; the kernel
+solves the passive system in ascending index order (masked Cholesky), so the
+two agree to rounding, not bit for bit.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle_binding import oracle_fit, oracle_opts
+
+pytestmark = pytest.mark.gpu
+
+TOL_X = 1e-10
+TOL_CHI2 = 1e-10
+
+
+def _run(pkg, cfg, pulse_cov, n, precision=0, seed_off=0, **kw):
+    from test_regression_amd import engine, workloads, _abi
+    w = workloads.config(cfg, pulse_cov)
+    w.seed += seed_off
+    ct = w.chtypes()
+    s, t, _ = w.generate(n)
+    eng = engine.Engine(list(ct))
+    opts = _abi.default_opts(precision=precision, **kw)
+    ts = torch.from_numpy(s).cuda()
+    tt = torch.from_numpy(t).cuda() if t is not None else None
+    print(f"[gpu-fit] cfg={cfg} pulse_cov={pulse_cov} n={n} prec={precision}", flush=True)
+    r = eng.fit(ts, tt, opts)
+    torch.cuda.synchronize()
+    return ct, s, t, r
+
+
+def _compare(ct, s, t, r, oo=None):
+    ref = oracle_fit(ct, s, t, oo or oracle_opts(), nthreads=8)
+    x = r.x.double().cpu().numpy()
+    xr = ref["x"].T
+    scale = np.maximum(np.abs(xr).max(axis=0), 1.0)
+    err = (np.abs(x - xr) / scale).max(axis=0)
+    aset = ((x > 0) == (xr > 0)).all(axis=0)
+    chi2 = r.chi2.double().cpu().numpy()
+    cerr = np.abs(chi2 - ref["chi2"]) / np.maximum(ref["chi2"], 1.0)
+    it = r.iters.cpu().numpy()
+    same_it = ((it & 255) == ref["n_minimize"]) & (((it >> 8) & 255) == ref["n_outer"])
+    return err, aset, cerr, same_it, ref, r
+
+
+@pytest.mark.parametrize("cfg,pulse_cov,n", [
+    ("c1", "zero", 10_000),
+    ("c2", "zero", 20_000),
+    ("c2", "live", 20_000),
+    ("c3", "live", 20_000),
+    ("c5", "zero", 10_000),
+])
+def test_fp64_fit_matches_oracle(pkg, gpu, cfg, pulse_cov, n):
+    ct, s, t, r = _run(pkg, cfg, pulse_cov, n)
+    err, aset, cerr, same_it, ref, r = _compare(ct, s, t, r)
+    print(f"max err {err.max():.3e} aset {aset.mean():.6f} chi2 {cerr.max():.3e} "
+          f"same iters {same_it.mean():.6f}")
+    assert aset.mean() >= 0.999
+    assert np.quantile(err, 0.999) <= TOL_X
+    assert np.quantile(cerr, 0.999) <= TOL_CHI2
+    assert same_it.mean() >= 0.999
+    fl = r.flags.cpu().numpy()
+    assert (fl == ref["flags"]).mean() >= 0.999
+
+
+def test_fp64_fit_fnnls_solver(pkg, gpu):
+    from test_regression_amd import _abi
+    from oracle_binding import OR_SOLVER_FNNLS
+    ct, s, t, r = _run(pkg, "c2", "zero", 5000, solver=_abi.MF_SOLVER_FNNLS)
+    err, aset, cerr, same_it, ref, r = _compare(ct, s, t, r, oracle_opts(solver=OR_SOLVER_FNNLS))
+    assert aset.mean() >= 0.999
+    assert np.quantile(err, 0.999) <= TOL_X
+
+
+def test_fp64_fit_cpu_max_iter(pkg, gpu):
+    """multifit_cpu's FNNLS cap of 10 (PulseChiSqSNNLS.cpp:243) truncates some fits."""
+    ct, s, t, r = _run(pkg, "c2", "live", 5000, max_iter=10)
+    err, aset, cerr, same_it, ref, r = _compare(ct, s, t, r, oracle_opts(max_iter=10))
+    assert aset.mean() >= 0.999
+    assert np.quantile(err, 0.999) <= TOL_X
+
+
+def test_fp64_fit_edge_cases(pkg, gpu):
+    """All-zero, NaN, huge and negative samples; empty batch."""
+    from test_regression_amd import engine, workloads, _abi
+    w = workloads.config("c1")
+    ct = w.chtypes()
+    eng = engine.Engine(list(ct))
+    s = np.zeros((10, 6))
+    s[:, 1] = np.nan
+    s[:, 2] = 1e6 * np.exp(-0.5 * (np.arange(10) - 5) ** 2)
+    s[:, 3] = -5.0
+    s[:, 4] = 1e-300
+    s[3, 5] = np.inf
+    r = eng.fit(torch.from_numpy(s).cuda(), None, _abi.default_opts())
+    torch.cuda.synchronize()
+    x = r.x.cpu().numpy()
+    fl = r.flags.cpu().numpy()
+    assert np.all(x[:, 0] == 0) and r.chi2[0].item() == 0.0
+    assert fl[1] & _abi.FLAGS["nonfinite"] and fl[5] & _abi.FLAGS["nonfinite"]
+    assert np.all(x[:, 1] == 0) and np.all(x[:, 5] == 0)
+    ref = oracle_fit(ct, s[:, [0, 2, 3, 4]].copy(), None, oracle_opts())
+    xr = ref["x"].T
+    got = x[:, [0, 2, 3, 4]]
+    scale = np.maximum(np.abs(xr).max(axis=0), 1.0)
+    assert np.all(np.abs(got - xr) / scale <= 1e-10)
+    # empty batch is a no-op
+    eng.fit(torch.zeros((10, 0), dtype=torch.float64, device="cuda"), None, _abi.default_opts())
+
+
+def test_fp64_fit_float_samples(pkg, gpu):
+    """FP32 sample storage feeds the FP64 fit: same as FP64 samples rounded to FP32."""
+    from test_regression_amd import engine, workloads, _abi
+    w = workloads.config("c2")
+    ct = w.chtypes()
+    s, t, _ = w.generate(3000)
+    s32 = s.astype(np.float32)
+    eng = engine.Engine(list(ct))
+    r = eng.fit(torch.from_numpy(s32).cuda(), None, _abi.default_opts())
+    torch.cuda.synchronize()
+    ref = oracle_fit(ct, s32.astype(np.float64), None, oracle_opts(), nthreads=8)
+    xr = ref["x"].T
+    scale = np.maximum(np.abs(xr).max(axis=0), 1.0)
+    err = (np.abs(r.x.cpu().numpy() - xr) / scale).max(axis=0)
+    assert np.quantile(err, 0.999) <= TOL_X
+
+
+def test_fit_rejects_bad_input(pkg, gpu):
+    from test_regression_amd import engine, workloads, _abi
+    w = workloads.config("c3")
+    eng = engine.Engine(list(w.chtypes()))
+    s = torch.zeros((10, 4), dtype=torch.float64, device="cuda")
+    bad_t = torch.tensor([0, 1, 2, 0], dtype=torch.uint8, device="cuda")
+    with pytest.raises(_abi.MFError, match="type_of holds type 2"):
+        eng.fit(s, bad_t, _abi.default_opts())
+    with pytest.raises(_abi.MFError):
+        eng.fit(s, None, _abi.default_opts(precision=7))
+    with pytest.raises(_abi.MFError):
+        eng.fit(s, None, _abi.default_opts(block=100))

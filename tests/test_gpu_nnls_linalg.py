@@ -1,0 +1,132 @@
+"""GPU parity of the batched solver and linalgebra entry points.
+
+mf_inplace_fnnls_batch / mf_fnnls_batch against the oracle's inplace_fnnls /
+fnnls and against scipy's Lawson-Hanson NNLS; mf_linalg_batch against the
+oracle's restatement of linalgebra/interface/*.hpp (same operation order:
+bit-identical in FP64 up to fma contraction, checked to 1e-13 relative).
+"""
+import numpy as np
+import pytest
+import torch
+from scipy.optimize import nnls as scipy_nnls
+
+from oracle_binding import OrStats, dptr, oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _problems(n, m, seed, kind="random"):
+    rng = np.random.default_rng(seed)
+    A = np.zeros((n, 10, 10))          # [c, col, row]
+    b = np.zeros((n, 10))
+    for c in range(n):
+        M = rng.standard_normal((m, m))
+        if kind == "pulse":
+            M = np.abs(M) + np.eye(m) * 2
+        A[c, :m, :m] = M.T             # column-major: A[c, col, row]
+        b[c, :m] = rng.standard_normal(m) * 10
+    return A, b
+
+
+@pytest.mark.parametrize("solver", ["inplace_fnnls", "fnnls"])
+@pytest.mark.parametrize("m", [10, 7, 3, 1])
+def test_nnls_batch_matches_oracle_and_scipy(pkg, gpu, solver, m):
+    from test_regression_amd import engine
+    n = 4000
+    A, b = _problems(n, m, 100 + m)
+    x0 = np.zeros((n, 10))
+    x = torch.from_numpy(x0).cuda()
+    print(f"[gpu-nnls] solver={solver} m={m} n={n}", flush=True)
+    engine.nnls_batch(torch.from_numpy(A).cuda(), torch.from_numpy(b).cuda(), x, m=m,
+                      solver=solver)
+    torch.cuda.synchronize()
+    xg = x.cpu().numpy()
+    L = oracle()
+    f = L.or_inplace_fnnls if solver == "inplace_fnnls" else L.or_fnnls
+    worst = 0.0
+    for c in range(n):
+        Ar = np.zeros((10, 10))
+        Ar[:m, :m] = A[c, :m, :m].T      # row-major for the oracle (ld 10)
+        xr = np.zeros(10)
+        st = OrStats()
+        f(dptr(np.ascontiguousarray(Ar)), dptr(np.ascontiguousarray(b[c])), dptr(xr), m, 1e-11,
+          1000, st)
+        scale = max(np.abs(xr).max(), 1.0)
+        worst = max(worst, np.abs(xg[c, :m] - xr[:m]).max() / scale)
+        if c < 300:
+            xs, _ = scipy_nnls(A[c, :m, :m].T, b[c, :m])
+            assert np.allclose(xg[c, :m], xs, rtol=1e-7, atol=1e-7 * max(1, np.abs(xs).max()))
+    assert worst <= 1e-10
+    assert np.all(xg[:, m:] == 0)
+
+
+def test_inplace_fnnls_warm_start_keeps_stale_values(pkg, gpu):
+    """The reference keeps the warm x but resets the passive set
+    (nnls/src/inplace_fnnls.cu:97,137): a warm start can end away from the
+    cold optimum.  The batch entry reproduces that, as the oracle does."""
+    from test_regression_amd import engine
+    n, m = 2000, 10
+    A, b = _problems(n, m, 7)
+    rng = np.random.default_rng(8)
+    x0 = np.abs(rng.standard_normal((n, 10))) * 3
+    x = torch.from_numpy(x0.copy()).cuda()
+    engine.nnls_batch(torch.from_numpy(A).cuda(), torch.from_numpy(b).cuda(), x, m=m)
+    torch.cuda.synchronize()
+    xg = x.cpu().numpy()
+    L = oracle()
+    for c in range(n):
+        Ar = np.ascontiguousarray(A[c].T)
+        xr = x0[c].copy()
+        L.or_inplace_fnnls(dptr(Ar), dptr(np.ascontiguousarray(b[c])), dptr(xr), m, 1e-11, 1000,
+                           OrStats())
+        assert np.abs(xg[c] - xr).max() <= 1e-10 * max(1, np.abs(xr).max())
+
+
+def _spd(rng, n):
+    M = rng.standard_normal((n, n))
+    return M @ M.T + n * np.eye(n)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_linalg_batch(pkg, gpu, dtype):
+    from test_regression_amd import engine, _abi
+    rng = np.random.default_rng(3)
+    n, full, view = 512, 10, 8
+    Ms = np.zeros((n, full, full))
+    bs = rng.standard_normal((n, full))
+    for c in range(n):
+        Ms[c, :view, :view] = _spd(rng, view)
+    tol = 1e-12 if dtype == torch.float64 else 2e-4
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dtype).cuda()  # noqa: E731
+    M, L = T(Ms), T(np.zeros_like(Ms))
+    print(f"[gpu-linalg] dtype={dtype}", flush=True)
+    engine.linalg_batch(_abi.MF_LA_CHOLESKY, M=M, L=L, full=full, view=view)
+    Lg = L.double().cpu().numpy()
+    for c in range(0, n, 17):
+        Lr = np.linalg.cholesky(Ms[c, :view, :view])
+        assert np.allclose(np.tril(Lg[c, :view, :view]), Lr, rtol=tol, atol=tol)
+    y = T(np.zeros((n, full)))
+    engine.linalg_batch(_abi.MF_LA_FORWARD, L=L, b=T(bs), y=y, full=full, view=view)
+    z = T(np.zeros((n, full)))
+    engine.linalg_batch(_abi.MF_LA_BACKWARD, L=L, b=y, y=z, full=full, view=view)
+    zg = z.double().cpu().numpy()
+    for c in range(0, n, 17):
+        xs = np.linalg.solve(Ms[c, :view, :view], bs[c, :view])
+        assert np.allclose(zg[c, :view], xs, rtol=10 * tol, atol=10 * tol)
+    # swap rows/cols 2 and 5 of M
+    M2 = T(Ms)
+    engine.linalg_batch(_abi.MF_LA_SWAP, M=M2, full=full, view=view, i=2, j=5)
+    M2g = M2.double().cpu().numpy()
+    p = list(range(view))
+    p[2], p[5] = 5, 2
+    assert np.allclose(M2g[0, :view, :view], Ms[0][np.ix_(p, p)])
+    # row/column addition: factor 7x7, then append row 7 -> 8x8
+    L7 = T(np.zeros_like(Ms))
+    engine.linalg_batch(_abi.MF_LA_CHOLESKY, M=M, L=L7, full=full, view=view - 1)
+    y7 = T(np.zeros((n, full)))
+    engine.linalg_batch(_abi.MF_LA_FORWARD, L=L7, b=T(bs), y=y7, full=full, view=view - 1)
+    engine.linalg_batch(_abi.MF_LA_RCADDITION, M=M, L=L7, b=T(bs), y=y7, full=full, view=view)
+    assert np.allclose(L7.double().cpu().numpy()[:, :view, :view], Lg[:, :view, :view],
+                       rtol=10 * tol, atol=10 * tol)
+    assert np.allclose(y7.double().cpu().numpy()[:, :view], y.double().cpu().numpy()[:, :view],
+                       rtol=10 * tol, atol=10 * tol)
