@@ -51,9 +51,23 @@ MF_HD float mfma(float a, float b, float c) { return fmaf(a, b, c); }
 MF_HD double mabs(double a) { return fabs(a); }
 MF_HD float mabs(float a) { return fabsf(a); }
 
-// 1/sqrt(d).  FP64: two correctly rounded operations.  FP32 on the device:
-// the hardware reciprocal square root (1 ulp).
-MF_HD double mrsqrt(double d) { return 1.0 / sqrt(d); }
+// 1/sqrt(d).  FP64 on the device: the hardware estimate (v_rsq_f64) and two
+// Newton-Raphson steps r <- r (3 - d r^2) / 2, each of which squares the
+// relative error of the estimate, to full FP64 accuracy (about 1 ulp) at a
+// fraction of the cost of a correctly rounded sqrt and divide.  FP32 on the
+// device: the hardware reciprocal square root (1 ulp).  Host builds: 1/sqrt.
+MF_HD double mrsqrt(double d)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rsq(d);
+    const double h = 0.5 * d;
+    r = r * fma(-h * r, r, 1.5);
+    r = r * fma(-h * r, r, 1.5);
+    return r;
+#else
+    return 1.0 / sqrt(d);
+#endif
+}
 MF_HD float mrsqrt(float d)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -116,6 +130,12 @@ template <typename T> struct TypeMats {
     T P[10][10];       // pulse matrix, P[sample][pulse]
     T C0[10][10];      // pederr^2 * noisecor (lower triangle used)
     T Pi[10][10][10];  // Pi[p][i][j] = fullpulsecov[i+o_p][j+o_p] if i,j >= f_p else 0
+    // The whitened problem at x = 0, where the covariance is C0 for every
+    // channel of the type (computed once on the host in FP64):
+    T Li0[10][10];     // L0^{-1}, L0 L0^T = C0 (lower)
+    T A0[10][10];      // L0^{-1} P
+    T AtA0[10][10];    // A0^T A0 (both triangles)
+    T G0[10][10];      // A0^T L0^{-1}: Atb = G0 s
 };
 struct TypeHdr {
     int32_t ns, np, has_pi, intime;
@@ -138,6 +158,9 @@ struct FitParams {
     int32_t max_minimize;
     int32_t use_pi;      // amplitude-weighted pulse covariance live
     int32_t solver;      // 0 inplace_fnnls, 1 fnnls
+    float eps_rel32;     // FP32 path: rounding allowance of w (see fnnls_inplace)
+    float tau32;         // FP32 path: pivot ratio below which a solve goes to FP64
+    double eps_rel64;    // FP32 path: rounding allowance of w in the FP64 phase
 };
 
 enum {
@@ -289,11 +312,15 @@ MF_HD T chisq(const TypeMats<T> &tm, const T (&Li)[10][10], const T (&s)[10], co
 // Solve AtA[P,P] s_P = Atb_P for the passive set P (bit mask pm) with a
 // masked full-size Cholesky; s is 0 outside P.  Returns false on a
 // non-positive pivot inside P.
+// tau > 0: *ill is set when a pivot inside P keeps less than a fraction tau
+// of its diagonal entry (M_kk - sum_j L_kj^2 < tau M_kk): the cancellation has
+// eaten the digits T carries and the solve should be redone in FP64.
 template <typename T, int N>
-MF_HD bool solve_passive(const T (&M)[10][10], const T (&b)[10], uint32_t pm, T (&s)[10])
+MF_HD bool solve_passive(const T (&M)[10][10], const T (&b)[10], uint32_t pm, T (&s)[10],
+                         T tau = T(0), bool *ill = nullptr)
 {
     T L[10][10], ri[10], y[10];
-    bool ok = true;
+    bool ok = true, bad = false;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         const bool ink = (pm >> k) & 1u;
@@ -302,6 +329,7 @@ MF_HD bool solve_passive(const T (&M)[10][10], const T (&b)[10], uint32_t pm, T 
         for (int j = 0; j < k; ++j) sq = mfma(L[k][j], L[k][j], sq);
         const T d = ink ? M[k][k] - sq : T(1);
         ok = ok && (d > T(0));
+        if (tau > T(0)) bad = bad || (ink && d < tau * M[k][k]);
         const T r = mrsqrt(d);
         ri[k] = r;
         T t = ink ? b[k] : T(0);
@@ -324,6 +352,7 @@ MF_HD bool solve_passive(const T (&M)[10][10], const T (&b)[10], uint32_t pm, T 
         for (int j = i + 1; j < N; ++j) t = mfma(-L[j][i], s[j], t);
         s[i] = t * ri[i];
     }
+    if (ill) *ill = bad || !ok;
     return ok;
 }
 
@@ -430,6 +459,9 @@ MF_HD void fnnls_inplace(const T (&AtA)[10][10], const T (&Atb)[10], T (&x)[10],
 }
 
 // fnnls (nnls/src/fnnls.cu:28-173): boolean-mask FNNLS on the full x.
+// Its outer test `!n_active || w < eps` (:113) counts the active indices
+// minus one, so it stops as soon as a single index is still free, without
+// adding it: a full-support optimum is never reached.  Kept as is.
 // The reference starts its running maximum at index 0 whether or not index 0
 // is still active (:82-95) and then sizes the passive block from a counter
 // that is one short in that case, reading an unset row; here the passive
@@ -504,11 +536,66 @@ template <typename T> struct FitOut {
     int n_min, n_outer, n_inner;
 };
 
-// Plain fit: the reference's algorithm in precision T.  Li (L^{-1} of the
-// covariance) is parked in `park` between the normal equations and chi2;
-// Park is any type with store(k, v) / load(k) for k < 55.
+// One Minimize iteration of the reference (PulseChiSqSNNLS.cpp:154-164):
+// updateCov + LLT, NNLS on the whitened problem (warm x), chi2.  Li (L^{-1} of
+// the covariance) is parked in `park` between the normal equations and chi2;
+// Park is any type with store(k, v) / load(k) for k < 55.  Returns chi2.
 template <typename T, int NS, int NP, typename Park>
-MF_HD void fit_plain(const TypeMats<T> &tm0, const T (&s)[10], const FitParams &prm,
+MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams &prm,
+                      Park &park, T (&x)[10], FnnlsStats &st, int &flags)
+{
+    const TypeMats<T> &tm = *opaque(&tm0);
+    T Li[10][10], ri[10];
+    build_cov<T, NS, NP>(tm, prm.use_pi != 0, x, Li);
+    if (!cholesky_inplace<T, NS>(Li, ri)) flags |= F_LLT_FAIL;
+    invert_lower<T, NS>(Li, ri);
+    T AtA[10][10], Atb[10];
+    normal_equations<T, NS, NP>(tm, Li, s, AtA, Atb);
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) park.store(i * (i + 1) / 2 + j, Li[i][j]);
+    if (prm.solver == 1)
+        fnnls_mask<T, NP>(AtA, Atb, x, T(prm.eps), prm.max_iter, st);
+    else
+        fnnls_inplace<T, NP>(AtA, Atb, x, T(prm.eps), T(0), prm.max_iter, st);
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) Li[i][j] = park.load(i * (i + 1) / 2 + j);
+    return chisq<T, NS, NP>(tm, Li, s, x);
+}
+
+// Minimize (PulseChiSqSNNLS.cpp:149-166) from iteration `it0` on, with the
+// previous chi2 `chi2` and amplitudes x.  Returns the iteration count.
+template <typename T, int NS, int NP, typename Park>
+MF_HD int minimize_from(const TypeMats<T> &tm, const T (&s)[10], const FitParams &prm,
+                        Park &park, T (&x)[10], T &chi2, int it0, FnnlsStats &st, int &flags)
+{
+    int it = it0;
+    for (; it < prm.max_minimize; ++it) {                       // :154
+        const T now = minimize_step<T, NS, NP>(tm, s, prm, park, x, st, flags);
+        const T d = now - chi2;
+        chi2 = now;
+        if (mabs(d) < T(prm.chi2_tol)) return it + 1;           // :162
+    }
+    flags |= F_MINIMIZE_CAP;
+    return it;
+}
+
+template <typename T, int NS>
+MF_HD bool all_finite(const T (&s)[10])
+{
+    bool finite = true;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) finite = finite && (s[i] - s[i] == T(0));
+    return finite;
+}
+
+// Plain fit: the reference's algorithm in precision T, cold start.
+// Non-finite samples are not fitted (x = 0, F_NONFINITE).
+template <typename T, int NS, int NP, typename Park>
+MF_HD void fit_plain(const TypeMats<T> &tm, const T (&s)[10], const FitParams &prm,
                      Park &park, FitOut<T> &out)
 {
     T x[10];
@@ -517,44 +604,242 @@ MF_HD void fit_plain(const TypeMats<T> &tm0, const T (&s)[10], const FitParams &
     T chi2 = T(0);
     int flags = 0, nmin = 0;
     FnnlsStats st;
-    bool finite = true;
-#pragma unroll
-    for (int i = 0; i < NS; ++i) finite = finite && (s[i] - s[i] == T(0));
-    if (!finite) flags |= F_NONFINITE;
-    const int maxmin = finite ? prm.max_minimize : 0;
-    int it = 0;
-    for (; it < maxmin; ++it) {                                 // :154
-        const TypeMats<T> &tm = *opaque(&tm0);
-        T Li[10][10], ri[10];
-        build_cov<T, NS, NP>(tm, prm.use_pi != 0, x, Li);
-        if (!cholesky_inplace<T, NS>(Li, ri)) flags |= F_LLT_FAIL;
-        invert_lower<T, NS>(Li, ri);
-        T AtA[10][10], Atb[10];
-        normal_equations<T, NS, NP>(tm, Li, s, AtA, Atb);
-#pragma unroll
-        for (int i = 0; i < NS; ++i)
-#pragma unroll
-            for (int j = 0; j <= i; ++j) park.store(i * (i + 1) / 2 + j, Li[i][j]);
-        if (prm.solver == 1)
-            fnnls_mask<T, NP>(AtA, Atb, x, T(prm.eps), prm.max_iter, st);
-        else
-            fnnls_inplace<T, NP>(AtA, Atb, x, T(prm.eps), T(0), prm.max_iter, st);
-#pragma unroll
-        for (int i = 0; i < NS; ++i)
-#pragma unroll
-            for (int j = 0; j <= i; ++j) Li[i][j] = park.load(i * (i + 1) / 2 + j);
-        const T now = chisq<T, NS, NP>(tm, Li, s, x);
-        const T d = now - chi2;
-        chi2 = now;
-        if (mabs(d) < T(prm.chi2_tol)) break;                   // :162
-    }
-    nmin = (it < maxmin) ? it + 1 : it;
-    if (finite && it == maxmin) flags |= F_MINIMIZE_CAP;
+    if (all_finite<T, NS>(s))
+        nmin = minimize_from<T, NS, NP>(tm, s, prm, park, x, chi2, 0, st, flags);
+    else
+        flags |= F_NONFINITE;
     if (st.hit_max) flags |= F_FNNLS_CAP;
     if (st.llt_fail) flags |= F_LLT_FAIL;
 #pragma unroll
     for (int i = 0; i < 10; ++i) out.x[i] = x[i];
     out.chi2 = chi2;
+    out.flags = flags;
+    out.n_min = nmin;
+    out.n_outer = st.outer;
+    out.n_inner = st.inner;
+}
+
+// ---------------------------------------------------------------------------
+// the fast path: FP32 bulk, exact NNLS optimum in the first Minimize
+// iteration (FP64 refinement + FP64 KKT verification), the reference's FNNLS
+// semantics in later iterations
+// ---------------------------------------------------------------------------
+
+// Active-set NNLS (Lawson-Hanson / FNNLS steps) from a feasible start: x > 0
+// on the passive mask pm and x = 0 off it.  x = 0 with any pm is accepted
+// too: the first solve then drops infeasible indices one at a time (alpha =
+// 0).  Free entries stay exactly 0, so the fixed point is the NNLS optimum of
+// (M, Atb).  Ties go to the lowest index.
+//
+// tau > 0 turns on the pivot monitor of solve_passive: the first solve that
+// reports an ill-conditioned passive block returns with *ill set and (x, pm)
+// left at the last feasible state, for the caller to continue in FP64.
+//
+// UNI: M lives in memory and is the same for every lane (a type's table): it
+// is re-read from the scalar cache on every pass instead of being held in
+// scalar registers across the loops.
+template <typename T, int N, bool UNI = false>
+MF_HD void fnnls_active(const T (&M0)[10][10], const T (&Atb)[10], T (&x)[10], uint32_t &pm,
+                        T eps, T eps_rel, int max_iter, FnnlsStats &st, T tau = T(0),
+                        bool *ill = nullptr)
+{
+    bool solve = pm != 0;
+    int adds = 0;
+    for (;;) {
+        while (solve && pm) {
+            T s[10];
+            bool bad = false;
+            const T(&M)[10][10] = UNI ? *opaque(&M0) : M0;
+            const bool ok = solve_passive<T, N>(M, Atb, pm, s, tau, &bad);
+            st.inner++;
+            if (tau > T(0) && bad) {
+                *ill = true;
+                return;
+            }
+            if (!ok) st.llt_fail = true;
+            bool allpos = true;
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                if ((pm >> i) & 1u) allpos = allpos && (s[i] > T(0));
+            if (allpos) {
+#pragma unroll
+                for (int i = 0; i < N; ++i)
+                    if ((pm >> i) & 1u) x[i] = s[i];
+                break;
+            }
+            T alpha = Lim<T>::max();
+            int a = -1, afirst = -1;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                if (((pm >> i) & 1u) && !(s[i] > T(0))) {
+                    if (afirst < 0) afirst = i;
+                    const T ratio = mdiv(x[i], x[i] - s[i]);
+                    if (ratio < alpha) {
+                        alpha = ratio;
+                        a = i;
+                    }
+                }
+            }
+            if (a < 0) {  // only 0/0 ratios (x_i = s_i = 0) or NaN: drop without a step
+                a = afirst;
+                alpha = T(0);
+            }
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                if ((pm >> i) & 1u) x[i] = (i == a) ? T(0) : mfma(alpha, s[i] - x[i], x[i]);
+            pm &= ~(1u << a);
+        }
+        solve = false;
+        if (adds >= max_iter) {
+            st.hit_max = true;
+            break;
+        }
+        int q = -1;
+        T best = T(0);
+        const T(&M)[10][10] = UNI ? *opaque(&M0) : M0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            T t = T(0), ta = T(0);
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                const T a = sym(M, i, j);
+                t = mfma(a, x[j], t);
+                ta = mfma(mabs(a), mabs(x[j]), ta);
+            }
+            const T w = Atb[i] - t;
+            const T thr = eps + eps_rel * (mabs(Atb[i]) + ta);
+            if (!((pm >> i) & 1u) && w > thr && (q < 0 || w > best)) {
+                best = w;
+                q = i;
+            }
+        }
+        if (q < 0) break;
+        pm |= 1u << q;
+        ++adds;
+        st.outer++;
+        solve = true;
+    }
+}
+
+// chi2 with an FP64 residual: r = s - P x in FP64, y = Li r with the given
+// (FP32 or FP64) inverse factor, chi2 = |y|^2.
+template <typename TL, typename TX, int NS, int NP>
+MF_HD double chisq64(const TypeMats<double> &td, const TL (&Li)[10][10], const double (&s)[10],
+                     const TX (&x)[10])
+{
+    double r[10];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        double t = s[i];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) t = fma(-td.P[i][p], (double)x[p], t);
+        r[i] = t;
+    }
+    double q = 0.0;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        double y = 0.0;
+#pragma unroll
+        for (int k = 0; k <= i; ++k) y = fma((double)Li[i][k], r[k], y);
+        q = fma(y, y, q);
+    }
+    return q;
+}
+
+// First Minimize iteration of the fast fit.  The covariance is C0 for every
+// channel of the type at x = 0, so the normal equations are the type's AtA0
+// and Atb = G0 s (no per-channel covariance, Cholesky or whitening).
+//   phase 1 (FP32): active-set iterations from (x, pm) -- x = 0, pm = 0 for a
+//     cold start, a predicted passive set for a warm start -- until the FP32
+//     KKT test passes or a pivot of the passive block loses more than a
+//     fraction tau32 of its diagonal (F_FP64_PIVOT);
+//   phase 2 (FP64): the same iterations continue from that state on the FP64
+//     normal equations with the reference's threshold eps: the first solve
+//     refines the amplitudes to FP64, the KKT test verifies the active set
+//     (a warm start that needed more steps is flagged F_WARM_REJECT).
+// The result is the NNLS optimum of the x = 0 problem, which is what the
+// reference's first (cold) FNNLS computes.  Returns chi2 (FP64); x64 holds
+// the amplitudes.
+template <int NS, int NP>
+MF_HD double fast_first(const DevType &ty, const float (&s32)[10], const double (&s64)[10],
+                        const FitParams &prm, float (&x)[10], uint32_t &pm, bool warm,
+                        double (&x64)[10], FnnlsStats &st, int &flags)
+{
+    const TypeMats<float> &tf = ty.f;
+    const TypeMats<double> &td = ty.d;
+    float Atb[10];
+    double Atb64[10];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        float t = 0.0f;
+        double t64 = 0.0;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            t = mfma(tf.G0[p][k], s32[k], t);
+            t64 = fma(td.G0[p][k], s64[k], t64);
+        }
+        Atb[p] = t;
+        Atb64[p] = t64;
+    }
+    bool ill = false;
+    fnnls_active<float, NP, true>(tf.AtA0, Atb, x, pm, (float)prm.eps, prm.eps_rel32,
+                                  prm.max_iter, st, prm.tau32, &ill);
+    if (ill) flags |= F_FP64_PIVOT;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) x64[i] = ((pm >> i) & 1u) ? (double)x[i] : 0.0;
+    const int outer0 = st.outer;
+    fnnls_active<double, NP, true>(td.AtA0, Atb64, x64, pm, prm.eps, prm.eps_rel64,
+                                   prm.max_iter, st);
+    flags |= F_REFINED;
+    if (warm && st.outer != outer0) flags |= F_WARM_REJECT;
+    return chisq64<double, double, NS, NP>(td, td.Li0, s64, x64);
+}
+
+// The fast fit of one channel.  Iteration 1 is fast_first.  If the type's
+// pulse covariance is zero (or not used) the covariance never depends on x:
+// the reference's iteration 2 rebuilds the same normal equations, its FNNLS
+// starts at their optimum and stops, chi2 repeats and Minimize ends after 2
+// iterations with the same amplitudes -- so the fit ends here.  With a live
+// pulse covariance the later iterations run the reference's FP64 iteration
+// body (per-channel covariance, inplace_fnnls with warm x and reset passive
+// set); they are FP64 because the reference decides them on gradients near
+// its 1e-11 threshold (see minimize_from).
+template <int NS, int NP, bool LIVE, typename Park>
+MF_HD void fit_fast(const DevType &ty, const float (&s32)[10], const FitParams &prm,
+                    float (&x)[10], uint32_t pm, bool warm, Park &park, FitOut<float> &out)
+{
+    int flags = 0;
+    FnnlsStats st;
+    double s64[10], x64[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) s64[i] = (double)s32[i];
+    int nmin = 0;
+    double chi2 = 0.0;
+    if (!all_finite<float, NS>(s32)) {
+        flags |= F_NONFINITE;
+#pragma unroll
+        for (int i = 0; i < 10; ++i) x64[i] = 0.0;
+    } else if (prm.max_minimize <= 0) {
+        flags |= F_MINIMIZE_CAP;
+#pragma unroll
+        for (int i = 0; i < 10; ++i) x64[i] = 0.0;
+    } else {
+        chi2 = fast_first<NS, NP>(ty, s32, s64, prm, x, pm, warm, x64, st, flags);
+        nmin = 1;
+        if (!(fabs(chi2) < prm.chi2_tol)) {
+            if (LIVE && prm.use_pi)
+                nmin = minimize_from<double, NS, NP>(ty.d, s64, prm, park, x64, chi2, 1, st,
+                                                     flags);
+            else
+                nmin = prm.max_minimize >= 2 ? 2 : 1;
+            if (nmin < 2 && !(LIVE && prm.use_pi)) flags |= F_MINIMIZE_CAP;
+        }
+    }
+    if (st.hit_max) flags |= F_FNNLS_CAP;
+    if (st.llt_fail) flags |= F_LLT_FAIL;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) out.x[i] = (float)x64[i];
+    out.chi2 = (float)chi2;
     out.flags = flags;
     out.n_min = nmin;
     out.n_outer = st.outer;

@@ -86,6 +86,36 @@ void k_fit_plain(const DevType *__restrict__ types, int type_id,
     if (ito) ito[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
 }
 
+// Fast path (MF_PREC_FP32): FP32 bulk with FP64 polish, see fit_fast.
+// LIVE: the type's pulse covariance is live, later iterations run in FP64
+// with L^{-1} parked in LDS.
+template <typename TS, int N, bool LIVE>
+__global__ __launch_bounds__(256)
+void k_fit_fast(const DevType *__restrict__ types, int type_id, const int32_t *__restrict__ idx,
+                int64_t nitems, const TS *__restrict__ samples, int64_t nch,
+                float *__restrict__ xo, float *__restrict__ chi2o, int32_t *__restrict__ flo,
+                int32_t *__restrict__ ito, FitParams prm)
+{
+    extern __shared__ unsigned char lds_raw[];
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nitems) return;
+    const int64_t c = idx ? (int64_t)idx[g] : g;
+    float s[10], x[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        s[i] = i < N ? (float)samples[(int64_t)i * nch + c] : 0.0f;
+        x[i] = 0.0f;
+    }
+    LdsPark<double> park{reinterpret_cast<double *>(lds_raw) + threadIdx.x, (int)blockDim.x};
+    FitOut<float> out;
+    fit_fast<N, N, LIVE>(types[type_id], s, prm, x, 0u, false, park, out);
+#pragma unroll
+    for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = out.x[p];
+    chi2o[c] = out.chi2;
+    flo[c] = out.flags;
+    if (ito) ito[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
+}
+
 // Partition of a mixed batch by channel type: counts, offsets, index lists.
 // part layout: counts[256] (int64), then cursors[256] (int64).
 __global__ void k_type_count(const uint8_t *__restrict__ type_of, int64_t nch,
@@ -358,6 +388,42 @@ static mf_status launch_fit(mf_context *ctx, int type_id, const int32_t *idx,
     return MF_OK;
 }
 
+template <typename TS, int N, bool LIVE>
+static mf_status launch_fast(mf_context *ctx, int type_id, const int32_t *idx, int64_t nitems,
+                             const void *samples, int64_t nch, const mf_result *out,
+                             const FitParams &prm, int block, hipStream_t st)
+{
+    if (nitems <= 0) return MF_OK;
+    const int64_t grid = (nitems + block - 1) / block;
+    const size_t lds = LIVE ? sizeof(double) * 55 * (size_t)block : 0;
+    hipLaunchKernelGGL((k_fit_fast<TS, N, LIVE>), dim3((unsigned)grid), dim3(block), lds, st,
+                       ctx->d_types, type_id, idx, nitems, static_cast<const TS *>(samples), nch,
+                       static_cast<float *>(out->x), static_cast<float *>(out->chi2), out->flags,
+                       out->iters, prm);
+    HIP_TRY(hipGetLastError());
+    return MF_OK;
+}
+
+template <typename TS>
+static mf_status dispatch_fast(mf_context *ctx, int type_id, const int32_t *idx, int64_t nitems,
+                               const mf_batch *in, const mf_result *out, const FitParams &prm,
+                               int block, hipStream_t st)
+{
+    const mf_chtype &t = ctx->types[(size_t)type_id];
+    bool has_pi = false;
+    for (int i = 0; i < MF_FULL * MF_FULL && !has_pi; ++i) has_pi = t.fullpulsecov[i] != 0.0;
+    const bool live = has_pi && prm.use_pi;
+    if (t.nsamples == 10)
+        return live ? launch_fast<TS, 10, true>(ctx, type_id, idx, nitems, in->samples, in->nch,
+                                                out, prm, block, st)
+                    : launch_fast<TS, 10, false>(ctx, type_id, idx, nitems, in->samples, in->nch,
+                                                 out, prm, block, st);
+    return live ? launch_fast<TS, 8, true>(ctx, type_id, idx, nitems, in->samples, in->nch, out,
+                                           prm, block, st)
+                : launch_fast<TS, 8, false>(ctx, type_id, idx, nitems, in->samples, in->nch, out,
+                                            prm, block, st);
+}
+
 template <typename T>
 static mf_status dispatch_fit(mf_context *ctx, int type_id, const int32_t *idx,
                               const int64_t *count, int64_t nitems, const mf_batch *in,
@@ -407,11 +473,18 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
     prm.max_minimize = opts->max_minimize;
     prm.use_pi = opts->use_pulse_cov;
     prm.solver = opts->solver;
+    prm.eps_rel32 = 1e-6f;  // FP32 phase: allowance on w (about 16 ulp of its terms)
+    prm.tau32 = 1e-3f;      // FP32 phase: pivot keeping < 0.1% of its diagonal -> FP64
+    prm.eps_rel64 = 1e-15;  // FP64 phase: allowance on w (about 4 ulp of its terms)
+    if (opts->precision == MF_PREC_FP32 && opts->solver != MF_SOLVER_INPLACE_FNNLS)
+        return fail(MF_ERR_UNSUPPORTED, "the FP32 path implements inplace_fnnls only");
     const int ntypes = (int)ctx->types.size();
     auto go = [&](int type_id, const int32_t *idx, const int64_t *count, int64_t nitems) {
-        return opts->precision == MF_PREC_FP64
-                   ? dispatch_fit<double>(ctx, type_id, idx, count, nitems, in, out, prm, block, st)
-                   : dispatch_fit<float>(ctx, type_id, idx, count, nitems, in, out, prm, block, st);
+        if (opts->precision == MF_PREC_FP64)
+            return dispatch_fit<double>(ctx, type_id, idx, count, nitems, in, out, prm, block, st);
+        return in->sample_dtype == MF_DTYPE_F64
+                   ? dispatch_fast<double>(ctx, type_id, idx, nitems, in, out, prm, block, st)
+                   : dispatch_fast<float>(ctx, type_id, idx, nitems, in, out, prm, block, st);
     };
     if (!in->type_of || ntypes == 1) return go(0, nullptr, nullptr, in->nch);
     // mixed batch: partition channel indices by type on the device

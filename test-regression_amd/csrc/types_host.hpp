@@ -2,6 +2,7 @@
 // tables the fit kernels read (mf::DevType).  Host code.
 #pragma once
 
+#include <math.h>
 #include <string.h>
 
 #include "../../include/mf_abi.h"
@@ -42,8 +43,55 @@ inline const char *expand_type(const mf_chtype &t, DevType &d)
                 d.d.Pi[p][i][j] = t.fullpulsecov[(i + o) * MF_FULL + (j + o)];
         }
     }
+    // x = 0 problem: L0 = chol(C0), Li0 = L0^{-1}, A0 = Li0 P, AtA0, G0
+    {
+        double L[10][10] = {{0}}, Z[10][10] = {{0}};
+        for (int i = 0; i < ns; ++i)
+            for (int j = 0; j <= i; ++j) {
+                double v = d.d.C0[i][j];
+                for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
+                if (i == j) {
+                    if (!(v > 0)) return "noise covariance pederr^2 * noisecor is not positive definite";
+                    L[i][i] = sqrt(v);
+                } else {
+                    L[i][j] = v / L[j][j];
+                }
+            }
+        for (int i = 0; i < ns; ++i) {
+            Z[i][i] = 1.0 / L[i][i];
+            for (int j = 0; j < i; ++j) {
+                double t = 0;
+                for (int k = j; k < i; ++k) t += L[i][k] * Z[k][j];
+                Z[i][j] = -t / L[i][i];
+            }
+        }
+        for (int i = 0; i < ns; ++i)
+            for (int j = 0; j < ns; ++j) d.d.Li0[i][j] = Z[i][j];
+        for (int i = 0; i < ns; ++i)
+            for (int p = 0; p < np; ++p) {
+                double t = 0;
+                for (int k = 0; k <= i; ++k) t += Z[i][k] * d.d.P[k][p];
+                d.d.A0[i][p] = t;
+            }
+        for (int p = 0; p < np; ++p)
+            for (int q = 0; q < np; ++q) {
+                double t = 0;
+                for (int i = 0; i < ns; ++i) t += d.d.A0[i][p] * d.d.A0[i][q];
+                d.d.AtA0[p][q] = t;
+            }
+        for (int p = 0; p < np; ++p)
+            for (int k = 0; k < ns; ++k) {
+                double t = 0;
+                for (int i = k; i < ns; ++i) t += d.d.A0[i][p] * Z[i][k];
+                d.d.G0[p][k] = t;
+            }
+    }
     for (int i = 0; i < 10; ++i)
         for (int j = 0; j < 10; ++j) {
+            d.f.Li0[i][j] = (float)d.d.Li0[i][j];
+            d.f.A0[i][j] = (float)d.d.A0[i][j];
+            d.f.AtA0[i][j] = (float)d.d.AtA0[i][j];
+            d.f.G0[i][j] = (float)d.d.G0[i][j];
             d.f.P[i][j] = (float)d.d.P[i][j];
             d.f.C0[i][j] = (float)d.d.C0[i][j];
             for (int p = 0; p < 10; ++p) d.f.Pi[p][i][j] = (float)d.d.Pi[p][i][j];

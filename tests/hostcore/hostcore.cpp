@@ -14,6 +14,25 @@
 
 using namespace mf;
 
+template <int N, bool LIVE>
+static void run_fast(const DevType &dt, const double *samples, int64_t nch, int64_t c,
+                     const FitParams &prm, double *x, double *chi2, int32_t *flags,
+                     int32_t *iters)
+{
+    float s[10], x0[10];
+    for (int i = 0; i < 10; ++i) {
+        s[i] = i < N ? (float)samples[i * nch + c] : 0.0f;
+        x0[i] = 0.0f;
+    }
+    RegPark<float> park;
+    FitOut<float> out;
+    fit_fast<N, N, LIVE>(dt, s, prm, x0, 0u, false, park, out);
+    for (int p = 0; p < 10; ++p) x[p * nch + c] = p < N ? (double)out.x[p] : 0.0;
+    chi2[c] = (double)out.chi2;
+    flags[c] = out.flags;
+    iters[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
+}
+
 template <typename T, int N>
 static void run(const DevType &dt, const double *samples, int64_t nch, int64_t c,
                 const FitParams &prm, double *x, double *chi2, int32_t *flags, int32_t *iters)
@@ -27,6 +46,15 @@ static void run(const DevType &dt, const double *samples, int64_t nch, int64_t c
     chi2[c] = (double)out.chi2;
     flags[c] = out.flags;
     iters[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
+}
+
+static float eps_rel32 = 1e-6f, tau32 = 1e-3f;
+static double eps_rel64 = 1e-15;
+extern "C" void hc_set_tolerances(float e32, float t32, double e64)
+{
+    eps_rel32 = e32;
+    tau32 = t32;
+    eps_rel64 = e64;
 }
 
 extern "C" int hc_fit_batch(const mf_chtype *types, int ntypes, const uint8_t *type_of,
@@ -43,15 +71,21 @@ extern "C" int hc_fit_batch(const mf_chtype *types, int ntypes, const uint8_t *t
     prm.max_minimize = o->max_minimize;
     prm.use_pi = o->use_pulse_cov;
     prm.solver = o->solver;
+    prm.eps_rel32 = eps_rel32;
+    prm.tau32 = tau32;
+    prm.eps_rel64 = eps_rel64;
     for (int64_t c = 0; c < nch; ++c) {
         const DevType &d = dt[type_of ? type_of[c] : 0];
         const bool f64 = o->precision == MF_PREC_FP64;
+        const bool live = d.h.has_pi && o->use_pulse_cov;
         if (d.h.ns == 10) {
             if (f64) run<double, 10>(d, samples, nch, c, prm, x, chi2, flags, iters);
-            else run<float, 10>(d, samples, nch, c, prm, x, chi2, flags, iters);
+            else if (live) run_fast<10, true>(d, samples, nch, c, prm, x, chi2, flags, iters);
+            else run_fast<10, false>(d, samples, nch, c, prm, x, chi2, flags, iters);
         } else {
             if (f64) run<double, 8>(d, samples, nch, c, prm, x, chi2, flags, iters);
-            else run<float, 8>(d, samples, nch, c, prm, x, chi2, flags, iters);
+            else if (live) run_fast<8, true>(d, samples, nch, c, prm, x, chi2, flags, iters);
+            else run_fast<8, false>(d, samples, nch, c, prm, x, chi2, flags, iters);
         }
     }
     return 0;

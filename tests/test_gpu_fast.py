@@ -1,0 +1,100 @@
+"""GPU parity of the fast path (MF_PREC_FP32) against the FP64 oracle.
+
+Bar (FP32 bulk, north-star accuracy): identical active sets on >= 99.9% of
+channels; amplitudes within 1e-4 of the oracle relative to the channel's
+largest amplitude (floor 1 ADC) on >= 99.9% of channels; chi2 within 1e-4
+relative (floor 1); Minimize iteration counts equal on >= 99.9%.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle_binding import oracle_fit, oracle_opts
+
+pytestmark = pytest.mark.gpu
+
+TOL_X = 1e-4
+TOL_CHI2 = 1e-4
+
+
+def _fit_fast(cfg, pulse_cov, n, sample_dtype=np.float32, **kw):
+    from test_regression_amd import engine, workloads, _abi
+    w = workloads.config(cfg, pulse_cov)
+    ct = w.chtypes()
+    s, t, _ = w.generate(n)
+    s_in = s.astype(sample_dtype)
+    eng = engine.Engine(list(ct))
+    print(f"[gpu-fast] cfg={cfg} pulse_cov={pulse_cov} n={n} dtype={sample_dtype.__name__}",
+          flush=True)
+    r = eng.fit(torch.from_numpy(s_in).cuda(),
+                torch.from_numpy(t).cuda() if t is not None else None,
+                _abi.default_opts(precision=_abi.MF_PREC_FP32, **kw))
+    torch.cuda.synchronize()
+    # the oracle sees exactly the samples the fast path fitted (FP32-rounded)
+    s_ref = s_in.astype(np.float32).astype(np.float64)
+    ref = oracle_fit(ct, s_ref, t, oracle_opts(), nthreads=8)
+    return r, ref
+
+
+def _check(r, ref):
+    x = r.x.double().cpu().numpy()
+    xr = ref["x"].T
+    scale = np.maximum(np.abs(xr).max(axis=0), 1.0)
+    err = (np.abs(x - xr) / scale).max(axis=0)
+    aset = ((x > 0) == (xr > 0)).all(axis=0)
+    chi2 = r.chi2.double().cpu().numpy()
+    cerr = np.abs(chi2 - ref["chi2"]) / np.maximum(ref["chi2"], 1.0)
+    nmin = (r.iters.cpu().numpy() & 255) == ref["n_minimize"]
+    print(f"err max {err.max():.2e} p999 {np.quantile(err, 0.999):.2e} aset {aset.mean():.5f} "
+          f"chi2 p999 {np.quantile(cerr, 0.999):.2e} nmin {nmin.mean():.5f}")
+    assert aset.mean() >= 0.999
+    assert (err <= TOL_X).mean() >= 0.999
+    assert (cerr <= TOL_CHI2).mean() >= 0.999
+    assert nmin.mean() >= 0.999
+    return err, aset
+
+
+@pytest.mark.parametrize("cfg,pulse_cov,n", [
+    ("c1", "zero", 10_000),
+    ("c2", "zero", 50_000),
+    ("c2", "live", 20_000),
+    ("c3", "zero", 20_000),
+    ("c3", "live", 20_000),
+    ("c5", "zero", 20_000),
+    ("c5", "live", 10_000),
+])
+def test_fast_fit_matches_oracle(pkg, gpu, cfg, pulse_cov, n):
+    r, ref = _fit_fast(cfg, pulse_cov, n)
+    _check(r, ref)
+    fl = r.flags.cpu().numpy()
+    from test_regression_amd import _abi
+    assert not np.any(fl & _abi.FLAGS["nonfinite"])
+
+
+def test_fast_fit_fp64_samples(pkg, gpu):
+    """FP64 sample storage is rounded to FP32 on load."""
+    r, ref = _fit_fast("c2", "zero", 5000, sample_dtype=np.float64)
+    _check(r, ref)
+
+
+def test_fast_fit_edge_cases(pkg, gpu):
+    from test_regression_amd import engine, workloads, _abi
+    w = workloads.config("c1")
+    ct = w.chtypes()
+    eng = engine.Engine(list(ct))
+    s = np.zeros((10, 5), dtype=np.float32)
+    s[:, 1] = np.nan
+    s[:, 2] = 1e5 * np.exp(-0.5 * (np.arange(10) - 5) ** 2)
+    s[:, 3] = -5.0
+    s[4, 4] = np.inf
+    r = eng.fit(torch.from_numpy(s).cuda(), None, _abi.default_opts(precision=_abi.MF_PREC_FP32))
+    torch.cuda.synchronize()
+    x = r.x.cpu().numpy()
+    fl = r.flags.cpu().numpy()
+    assert np.all(x[:, 0] == 0) and r.chi2[0].item() == 0.0
+    assert fl[1] & _abi.FLAGS["nonfinite"] and fl[4] & _abi.FLAGS["nonfinite"]
+    ref = oracle_fit(ct, s[:, [0, 2, 3]].astype(np.float64), None, oracle_opts())
+    xr = ref["x"].T
+    scale = np.maximum(np.abs(xr).max(axis=0), 1.0)
+    assert np.all(np.abs(x[:, [0, 2, 3]] - xr) / scale <= TOL_X)
+    assert np.all((x[:, [0, 2, 3]] > 0) == (xr > 0))

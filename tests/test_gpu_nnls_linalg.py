@@ -55,6 +55,8 @@ def test_nnls_batch_matches_oracle_and_scipy(pkg, gpu, solver, m):
         worst = max(worst, np.abs(xg[c, :m] - xr[:m]).max() / scale)
         if c < 300:
             xs, _ = scipy_nnls(A[c, :m, :m].T, b[c, :m])
+            if solver == "fnnls" and np.count_nonzero(xg[c, :m] == 0) == 1:
+                continue  # fnnls.cu:113 never tests the last free index (see test_oracle)
             assert np.allclose(xg[c, :m], xs, rtol=1e-7, atol=1e-7 * max(1, np.abs(xs).max()))
     assert worst <= 1e-10
     assert np.all(xg[:, m:] == 0)
@@ -117,9 +119,15 @@ def test_linalg_batch(pkg, gpu, dtype):
     M2 = T(Ms)
     engine.linalg_batch(_abi.MF_LA_SWAP, M=M2, full=full, view=view, i=2, j=5)
     M2g = M2.double().cpu().numpy()
+    # the reference's swap (linalgebra/interface/common.hpp:11-42) visits
+    # elem == j after swapping the diagonal: (i, j) and (j, j) end up holding
+    # M[i][i] and M[i][j]; everything else is the symmetric permutation
     p = list(range(view))
     p[2], p[5] = 5, 2
-    assert np.allclose(M2g[0, :view, :view], Ms[0][np.ix_(p, p)])
+    for c in range(0, n, 37):
+        want = Ms[c][:view, :view][np.ix_(p, p)].copy()
+        want[2, 5], want[5, 5] = Ms[c][2, 2], Ms[c][2, 5]
+        assert np.allclose(M2g[c, :view, :view], want, rtol=tol, atol=tol)
     # row/column addition: factor 7x7, then append row 7 -> 8x8
     L7 = T(np.zeros_like(Ms))
     engine.linalg_batch(_abi.MF_LA_CHOLESKY, M=M, L=L7, full=full, view=view - 1)
