@@ -84,6 +84,9 @@ typedef struct mf_opts {
     int32_t warm_start;     /* FP32 only: 1 = learned warm start (needs weights) */
     int32_t persistent;     /* 1: persistent kernel + device work queue       */
     int32_t block;          /* threads per block (0 = tuned default)          */
+    int32_t compute_errors; /* FP64 only: MINOS-like in-time uncertainty
+                               (PulseChiSqSNNLS.cpp:73-146) into mf_result.err */
+    int32_t pad;
 } mf_opts;
 
 /* Per-channel flags (mf_result.flags). */
@@ -112,12 +115,15 @@ typedef struct mf_batch {
  * (double for MF_PREC_FP64, float for MF_PREC_FP32): x[p*nch + c] amplitude
  * of pulse p (pulse order of the channel's type, the BXs() order of the
  * reference), chi2[c], flags[c] (MF_F_*), and optionally iters[c] =
- * n_minimize | n_fnnls_outer << 8 | n_fnnls_inner << 16 (NULL: skipped). */
+ * n_minimize | n_fnnls_outer << 8 | n_fnnls_inner << 16 (NULL: skipped) and
+ * err[p*nch + c] (Errors(): only the in-time pulse's entry is nonzero; needed
+ * when compute_errors is set, else ignored). */
 typedef struct mf_result {
     void     *x;
     void     *chi2;
     int32_t  *flags;
     int32_t  *iters;
+    void     *err;
 } mf_result;
 
 typedef struct mf_context mf_context;
@@ -132,9 +138,10 @@ mf_status mf_context_destroy(mf_context *ctx);
 mf_status mf_set_types(mf_context *ctx, const mf_chtype *types, int32_t ntypes);
 mf_status mf_default_opts(mf_opts *opts);
 
-/* Fits every channel of the batch: DoFit with error computation disabled,
- * as every reference driver runs it (multifit_cpu/src/multifit.cpp:200,
- * multifit_gpu/src/PulseChiSqSNNLS.cu:234).  Asynchronous on `stream`. */
+/* Fits every channel of the batch: DoFit, by default with error computation
+ * disabled as every reference driver runs it (multifit_cpu/src/multifit.cpp:200,
+ * multifit_gpu/src/PulseChiSqSNNLS.cu:234).  Asynchronous on `stream`, except
+ * that a batch mixing channel types synchronizes once to size its partition. */
 mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_result *out,
                        const mf_opts *opts, void *stream);
 

@@ -42,6 +42,7 @@ class Opts(C.Structure):
         ("max_minimize", C.c_int32), ("eps", C.c_double), ("chi2_tol", C.c_double),
         ("use_pulse_cov", C.c_int32), ("warm_start", C.c_int32),
         ("persistent", C.c_int32), ("block", C.c_int32),
+        ("compute_errors", C.c_int32), ("pad", C.c_int32),
     ]
 
 
@@ -55,7 +56,7 @@ class Batch(C.Structure):
 class Result(C.Structure):
     _fields_ = [
         ("x", C.c_void_p), ("chi2", C.c_void_p), ("flags", C.c_void_p),
-        ("iters", C.c_void_p),
+        ("iters", C.c_void_p), ("err", C.c_void_p),
     ]
 
 

@@ -254,9 +254,12 @@ MF_HD void build_cov(const TypeMats<T> &tm, bool use_pi, const T (&x)[10], T (&C
 // Normal equations of the whitened problem A = L^{-1} P, b = L^{-1} s
 // (PulseChiSqSNNLS.cpp:235-236, inplace_fnnls.cu:115-117), streamed by rows
 // of A so that A is never held whole: row i of A needs only row i of L^{-1}.
+//
+// zc >= 0: column zc of the pulse matrix is taken as zero (DoFit's error
+// computation zeroes the in-time column, PulseChiSqSNNLS.cpp:104-105).
 template <typename T, int NS, int NP>
 MF_HD void normal_equations(const TypeMats<T> &tm, const T (&Li)[10][10], const T (&s)[10],
-                            T (&AtA)[10][10], T (&Atb)[10])
+                            T (&AtA)[10][10], T (&Atb)[10], int zc = -1)
 {
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
@@ -275,7 +278,7 @@ MF_HD void normal_equations(const TypeMats<T> &tm, const T (&Li)[10][10], const 
             T t = T(0);
 #pragma unroll
             for (int k = 0; k <= r; ++k) t = mfma(Li[r][k], tm.P[k][p], t);
-            a[p] = t;
+            a[p] = (p == zc) ? T(0) : t;
         }
 #pragma unroll
         for (int i = 0; i < NP; ++i) {
@@ -288,14 +291,15 @@ MF_HD void normal_equations(const TypeMats<T> &tm, const T (&Li)[10][10], const 
 
 // chi2 = |L^{-1} (P x - s)|^2 (ComputeChiSq, PulseChiSqSNNLS.cpp:196-200)
 template <typename T, int NS, int NP>
-MF_HD T chisq(const TypeMats<T> &tm, const T (&Li)[10][10], const T (&s)[10], const T (&x)[10])
+MF_HD T chisq(const TypeMats<T> &tm, const T (&Li)[10][10], const T (&s)[10], const T (&x)[10],
+              int zc = -1)
 {
     T r[10];
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
         T t = T(0);
 #pragma unroll
-        for (int p = 0; p < NP; ++p) t = mfma(tm.P[i][p], x[p], t);
+        for (int p = 0; p < NP; ++p) t = mfma(p == zc ? T(0) : tm.P[i][p], x[p], t);
         r[i] = t - s[i];
     }
     T q = T(0);
@@ -531,6 +535,7 @@ MF_HD void fnnls_mask(const T (&AtA)[10][10], const T (&Atb)[10], T (&x)[10], T 
 // ---------------------------------------------------------------------------
 template <typename T> struct FitOut {
     T x[10];
+    T err[10];
     T chi2;
     int flags;
     int n_min, n_outer, n_inner;
@@ -542,7 +547,7 @@ template <typename T> struct FitOut {
 // Park is any type with store(k, v) / load(k) for k < 55.  Returns chi2.
 template <typename T, int NS, int NP, typename Park>
 MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams &prm,
-                      Park &park, T (&x)[10], FnnlsStats &st, int &flags)
+                      Park &park, T (&x)[10], FnnlsStats &st, int &flags, int zc = -1)
 {
     const TypeMats<T> &tm = *opaque(&tm0);
     T Li[10][10], ri[10];
@@ -550,7 +555,7 @@ MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams 
     if (!cholesky_inplace<T, NS>(Li, ri)) flags |= F_LLT_FAIL;
     invert_lower<T, NS>(Li, ri);
     T AtA[10][10], Atb[10];
-    normal_equations<T, NS, NP>(tm, Li, s, AtA, Atb);
+    normal_equations<T, NS, NP>(tm, Li, s, AtA, Atb, zc);
 #pragma unroll
     for (int i = 0; i < NS; ++i)
 #pragma unroll
@@ -563,18 +568,19 @@ MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams 
     for (int i = 0; i < NS; ++i)
 #pragma unroll
         for (int j = 0; j <= i; ++j) Li[i][j] = park.load(i * (i + 1) / 2 + j);
-    return chisq<T, NS, NP>(tm, Li, s, x);
+    return chisq<T, NS, NP>(tm, Li, s, x, zc);
 }
 
 // Minimize (PulseChiSqSNNLS.cpp:149-166) from iteration `it0` on, with the
 // previous chi2 `chi2` and amplitudes x.  Returns the iteration count.
 template <typename T, int NS, int NP, typename Park>
 MF_HD int minimize_from(const TypeMats<T> &tm, const T (&s)[10], const FitParams &prm,
-                        Park &park, T (&x)[10], T &chi2, int it0, FnnlsStats &st, int &flags)
+                        Park &park, T (&x)[10], T &chi2, int it0, FnnlsStats &st, int &flags,
+                        int zc = -1)
 {
     int it = it0;
     for (; it < prm.max_minimize; ++it) {                       // :154
-        const T now = minimize_step<T, NS, NP>(tm, s, prm, park, x, st, flags);
+        const T now = minimize_step<T, NS, NP>(tm, s, prm, park, x, st, flags, zc);
         const T d = now - chi2;
         chi2 = now;
         if (mabs(d) < T(prm.chi2_tol)) return it + 1;           // :162
@@ -592,15 +598,80 @@ MF_HD bool all_finite(const T (&s)[10])
     return finite;
 }
 
-// Plain fit: the reference's algorithm in precision T, cold start.
-// Non-finite samples are not fitted (x = 0, F_NONFINITE).
+// MINOS-like uncertainty of the in-time amplitude (PulseChiSqSNNLS.cpp:73-146)
+// after a fit that ended with amplitudes x, chi2 and L^{-1} of its last
+// covariance in `park`.  ip: the first pulse with BX 0 (type constant).
+// approxerr = 1 / |L^{-1} p_ip| (ComputeApproxUncertainty, :202-207); the
+// in-time column is zeroed, its amplitude fixed at x0 + approxerr and the
+// samples reduced by it, Minimize continues from the current state, and
+// sigma+ = approxerr / sqrt(chi2+ - chi2_0); if x0 / sigma+ > 0.5 the same is
+// done at max(0, x0 - approxerr) and the two are averaged (:120-142).
 template <typename T, int NS, int NP, typename Park>
+MF_HD T minos_error(const TypeMats<T> &tm, const T (&s)[10], const FitParams &prm, Park &park,
+                    const T (&xfit)[10], T chi2_0, int ip, FnnlsStats &st, int &flags)
+{
+    T y = T(0), q = T(0);
+    {
+        T col[10];
+#pragma unroll
+        for (int i = 0; i < NS; ++i) col[i] = tm.P[i][ip];
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            T t = T(0);
+#pragma unroll
+            for (int k = 0; k <= i; ++k) t = mfma(park.load(i * (i + 1) / 2 + k), col[k], t);
+            q = mfma(t, t, q);
+        }
+        y = q;
+    }
+    const T approxerr = T(1) / sqrt(y);
+    const T x0 = xfit[ip];
+    T x[10], sr[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) x[i] = xfit[i];
+    T chi2 = chi2_0;
+    // upper point
+    const T xplus = x0 + approxerr;
+    x[ip] = xplus;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) sr[i] = i < NS ? s[i] - xplus * tm.P[i][ip] : T(0);
+    minimize_from<T, NS, NP>(tm, sr, prm, park, x, chi2, 0, st, flags, ip);
+    T Li[10][10];
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) Li[i][j] = park.load(i * (i + 1) / 2 + j);
+    const T chiplus = chisq<T, NS, NP>(tm, Li, sr, x, ip);      // :117
+    const T sigplus = mabs(xplus - x0) / sqrt(chiplus - chi2_0);
+    if (!(x0 / sigplus > T(0.5))) return sigplus;              // :122, :141-142
+    T xminus = x0 - approxerr;                                  // :130
+    if (xminus < T(0)) xminus = T(0);
+    x[ip] = xminus;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) sr[i] = i < NS ? s[i] - xminus * tm.P[i][ip] : T(0);
+    minimize_from<T, NS, NP>(tm, sr, prm, park, x, chi2, 0, st, flags, ip);
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) Li[i][j] = park.load(i * (i + 1) / 2 + j);
+    const T chiminus = chisq<T, NS, NP>(tm, Li, sr, x, ip);
+    const T sigminus = mabs(xminus - x0) / sqrt(chiminus - chi2_0);
+    return T(0.5) * (sigplus + sigminus);
+}
+
+// Plain fit: the reference's algorithm in precision T, cold start, and with
+// ERRORS the in-time uncertainty (ip >= 0: index of the in-time pulse).
+// Non-finite samples are not fitted (x = 0, F_NONFINITE).
+template <typename T, int NS, int NP, bool ERRORS = false, typename Park>
 MF_HD void fit_plain(const TypeMats<T> &tm, const T (&s)[10], const FitParams &prm,
-                     Park &park, FitOut<T> &out)
+                     Park &park, FitOut<T> &out, int ip = -1)
 {
     T x[10];
 #pragma unroll
-    for (int i = 0; i < 10; ++i) x[i] = T(0);
+    for (int i = 0; i < 10; ++i) {
+        x[i] = T(0);
+        out.err[i] = T(0);
+    }
     T chi2 = T(0);
     int flags = 0, nmin = 0;
     FnnlsStats st;
@@ -608,6 +679,14 @@ MF_HD void fit_plain(const TypeMats<T> &tm, const T (&s)[10], const FitParams &p
         nmin = minimize_from<T, NS, NP>(tm, s, prm, park, x, chi2, 0, st, flags);
     else
         flags |= F_NONFINITE;
+    if (ERRORS && ip >= 0 && !(flags & F_NONFINITE) && prm.max_minimize > 0) {
+        FnnlsStats st2;
+        int fl2 = 0;
+        const T e = minos_error<T, NS, NP>(tm, s, prm, park, x, chi2, ip, st2, fl2);
+#pragma unroll
+        for (int i = 0; i < 10; ++i)
+            if (i == ip) out.err[i] = e;
+    }
     if (st.hit_max) flags |= F_FNNLS_CAP;
     if (st.llt_fail) flags |= F_LLT_FAIL;
 #pragma unroll
@@ -838,7 +917,10 @@ MF_HD void fit_fast(const DevType &ty, const float (&s32)[10], const FitParams &
     if (st.hit_max) flags |= F_FNNLS_CAP;
     if (st.llt_fail) flags |= F_LLT_FAIL;
 #pragma unroll
-    for (int i = 0; i < 10; ++i) out.x[i] = (float)x64[i];
+    for (int i = 0; i < 10; ++i) {
+        out.x[i] = (float)x64[i];
+        out.err[i] = 0.0f;
+    }
     out.chi2 = (float)chi2;
     out.flags = flags;
     out.n_min = nmin;

@@ -57,15 +57,13 @@ template <typename T> struct LdsPark {
     __device__ __forceinline__ T load(int k) const { return base[k * stride]; }
 };
 
-template <typename T, typename TS, int N>
+template <typename T, typename TS, int N, bool ERRORS>
 __global__ __launch_bounds__(256)
-void k_fit_plain(const DevType *__restrict__ types, int type_id,
-                                                   const int32_t *__restrict__ idx,
-                                                   const int64_t *__restrict__ count,
-                                                   int64_t nitems, const TS *__restrict__ samples,
-                                                   int64_t nch, T *__restrict__ xo,
-                                                   T *__restrict__ chi2o, int32_t *__restrict__ flo,
-                                                   int32_t *__restrict__ ito, FitParams prm)
+void k_fit_plain(const DevType *__restrict__ types, int type_id, const int32_t *__restrict__ idx,
+                 const int64_t *__restrict__ count, int64_t nitems,
+                 const TS *__restrict__ samples, int64_t nch, T *__restrict__ xo,
+                 T *__restrict__ chi2o, int32_t *__restrict__ flo, int32_t *__restrict__ ito,
+                 T *__restrict__ erro, FitParams prm)
 {
     extern __shared__ unsigned char lds_raw[];
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -78,9 +76,13 @@ void k_fit_plain(const DevType *__restrict__ types, int type_id,
     for (int i = 0; i < 10; ++i) s[i] = i < N ? (T)samples[(int64_t)i * nch + c] : T(0);
     LdsPark<T> park{reinterpret_cast<T *>(lds_raw) + threadIdx.x, (int)blockDim.x};
     FitOut<T> out;
-    fit_plain<T, N, N>(tm, s, prm, park, out);
+    fit_plain<T, N, N, ERRORS>(tm, s, prm, park, out, types[type_id].h.intime);
 #pragma unroll
     for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = out.x[p];
+    if (ERRORS) {
+#pragma unroll
+        for (int p = 0; p < N; ++p) erro[(int64_t)p * nch + c] = out.err[p];
+    }
     chi2o[c] = out.chi2;
     flo[c] = out.flags;
     if (ito) ito[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
@@ -342,10 +344,10 @@ extern "C" mf_status mf_context_create(mf_context **ctx, int device)
 extern "C" mf_status mf_context_destroy(mf_context *ctx)
 {
     if (!ctx) return MF_OK;
-    hipSetDevice(ctx->device);
-    if (ctx->d_types) hipFree(ctx->d_types);
-    if (ctx->d_part) hipFree(ctx->d_part);
-    if (ctx->d_idx) hipFree(ctx->d_idx);
+    (void)hipSetDevice(ctx->device);
+    if (ctx->d_types) (void)hipFree(ctx->d_types);
+    if (ctx->d_part) (void)hipFree(ctx->d_part);
+    if (ctx->d_idx) (void)hipFree(ctx->d_idx);
     delete ctx;
     return MF_OK;
 }
@@ -375,15 +377,23 @@ template <typename T, typename TS, int N>
 static mf_status launch_fit(mf_context *ctx, int type_id, const int32_t *idx,
                             const int64_t *count, int64_t nitems, const void *samples,
                             int64_t nch, const mf_result *out, const FitParams &prm, int block,
-                            hipStream_t st)
+                            bool errors, hipStream_t st)
 {
     if (nitems <= 0) return MF_OK;
     const int64_t grid = (nitems + block - 1) / block;
     const size_t lds = sizeof(T) * 55 * (size_t)block;
-    hipLaunchKernelGGL((k_fit_plain<T, TS, N>), dim3((unsigned)grid), dim3(block), lds, st,
-                       ctx->d_types, type_id, idx, count, nitems,
-                       static_cast<const TS *>(samples), nch, static_cast<T *>(out->x),
-                       static_cast<T *>(out->chi2), out->flags, out->iters, prm);
+    if (errors)
+        hipLaunchKernelGGL((k_fit_plain<T, TS, N, true>), dim3((unsigned)grid), dim3(block), lds,
+                           st, ctx->d_types, type_id, idx, count, nitems,
+                           static_cast<const TS *>(samples), nch, static_cast<T *>(out->x),
+                           static_cast<T *>(out->chi2), out->flags, out->iters,
+                           static_cast<T *>(out->err), prm);
+    else
+        hipLaunchKernelGGL((k_fit_plain<T, TS, N, false>), dim3((unsigned)grid), dim3(block),
+                           lds, st, ctx->d_types, type_id, idx, count, nitems,
+                           static_cast<const TS *>(samples), nch, static_cast<T *>(out->x),
+                           static_cast<T *>(out->chi2), out->flags, out->iters,
+                           static_cast<T *>(out->err), prm);
     HIP_TRY(hipGetLastError());
     return MF_OK;
 }
@@ -427,20 +437,20 @@ static mf_status dispatch_fast(mf_context *ctx, int type_id, const int32_t *idx,
 template <typename T>
 static mf_status dispatch_fit(mf_context *ctx, int type_id, const int32_t *idx,
                               const int64_t *count, int64_t nitems, const mf_batch *in,
-                              const mf_result *out, const FitParams &prm, int block,
+                              const mf_result *out, const FitParams &prm, int block, bool errors,
                               hipStream_t st)
 {
     const int ns = ctx->types[(size_t)type_id].nsamples;
     const bool f64in = in->sample_dtype == MF_DTYPE_F64;
     if (ns == 10)
         return f64in ? launch_fit<T, double, 10>(ctx, type_id, idx, count, nitems, in->samples,
-                                                 in->nch, out, prm, block, st)
+                                                 in->nch, out, prm, block, errors, st)
                      : launch_fit<T, float, 10>(ctx, type_id, idx, count, nitems, in->samples,
-                                                in->nch, out, prm, block, st);
+                                                in->nch, out, prm, block, errors, st);
     return f64in ? launch_fit<T, double, 8>(ctx, type_id, idx, count, nitems, in->samples,
-                                            in->nch, out, prm, block, st)
+                                            in->nch, out, prm, block, errors, st)
                  : launch_fit<T, float, 8>(ctx, type_id, idx, count, nitems, in->samples, in->nch,
-                                           out, prm, block, st);
+                                           out, prm, block, errors, st);
 }
 
 extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_result *out,
@@ -478,10 +488,15 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
     prm.eps_rel64 = 1e-15;  // FP64 phase: allowance on w (about 4 ulp of its terms)
     if (opts->precision == MF_PREC_FP32 && opts->solver != MF_SOLVER_INPLACE_FNNLS)
         return fail(MF_ERR_UNSUPPORTED, "the FP32 path implements inplace_fnnls only");
+    if (opts->compute_errors && opts->precision != MF_PREC_FP64)
+        return fail(MF_ERR_UNSUPPORTED, "compute_errors needs MF_PREC_FP64");
+    if (opts->compute_errors && !out->err)
+        return fail(MF_ERR_INVALID, "compute_errors needs mf_result.err");
     const int ntypes = (int)ctx->types.size();
     auto go = [&](int type_id, const int32_t *idx, const int64_t *count, int64_t nitems) {
         if (opts->precision == MF_PREC_FP64)
-            return dispatch_fit<double>(ctx, type_id, idx, count, nitems, in, out, prm, block, st);
+            return dispatch_fit<double>(ctx, type_id, idx, count, nitems, in, out, prm, block,
+                                        opts->compute_errors != 0, st);
         return in->sample_dtype == MF_DTYPE_F64
                    ? dispatch_fast<double>(ctx, type_id, idx, nitems, in, out, prm, block, st)
                    : dispatch_fast<float>(ctx, type_id, idx, nitems, in, out, prm, block, st);

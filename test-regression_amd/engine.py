@@ -20,6 +20,7 @@ class FitResult:
     chi2: torch.Tensor     # [nch]
     flags: torch.Tensor    # [nch] int32 MF_F_* bits
     iters: torch.Tensor    # [nch] int32: n_minimize | outer << 8 | inner << 16
+    err: torch.Tensor | None = None  # [10, nch] in-time uncertainty (compute_errors)
 
     def n_minimize(self):
         return self.iters & 0xFF
@@ -75,6 +76,7 @@ class Engine:
             chi2=torch.zeros(nch, dtype=dt, device=dev),
             flags=torch.zeros(nch, dtype=torch.int32, device=dev),
             iters=torch.zeros(nch, dtype=torch.int32, device=dev),
+            err=torch.zeros((10, nch), dtype=dt, device=dev),
         )
 
     def fit(self, samples: torch.Tensor, type_of: torch.Tensor | None = None, opts=None,
@@ -103,7 +105,8 @@ class Engine:
         b = _abi.Batch(nch=nch, sample_dtype=sdt, pad=0, samples=samples.data_ptr(),
                        type_of=type_of.data_ptr() if type_of is not None else None)
         r = _abi.Result(x=out.x.data_ptr(), chi2=out.chi2.data_ptr(),
-                        flags=out.flags.data_ptr(), iters=out.iters.data_ptr())
+                        flags=out.flags.data_ptr(), iters=out.iters.data_ptr(),
+                        err=out.err.data_ptr() if out.err is not None else None)
         _abi.check(self._lib.mf_fit_batch(self._ctx, C.byref(b), C.byref(r), C.byref(opts),
                                           C.c_void_p(_stream_ptr(stream))), "mf_fit_batch")
         return out

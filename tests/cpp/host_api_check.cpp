@@ -1,0 +1,105 @@
+// host_api_check.cpp -- TEST PROGRAM for the C++ host library (mf_host.hpp).
+// Reads channels from a binary file written by tests/test_gpu_host_api.py,
+// fits them through PulseChiSqSNNLS::DoFit (one by one, errors on) and
+// doFitWrapper (batched), solves the same NNLS problems through
+// inplace_fnnls / fnnls_wrapper, and runs the linalgebra functions on the
+// reference's 3x3 literal (linalgebra/tests/test_cholesky_with_solvers.cpp:25-39);
+// writes everything to an output file for the test to compare with the oracle.
+//
+// input : int32 n, then per channel: 10 samples, 100 samplecor (row-major),
+//         pederr, 19 fullpulse, 361 fullpulsecov (row-major), 10 bx (as double)
+// output: per channel: DoFit X[10], Errors[10], chi2; doFitWrapper X[10], chi2,
+//         ampl; inplace_fnnls x[10] of (A = I + pulsemat, b = samples); then
+//         the 3x3 L (9), y (3), x (3) in double and in float.
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <vector>
+
+#include "../../include/mf_host.hpp"
+
+int main(int argc, char **argv)
+{
+    if (argc != 3) {
+        fprintf(stderr, "usage: %s in.bin out.bin\n", argv[0]);
+        return 2;
+    }
+    FILE *f = fopen(argv[1], "rb");
+    if (!f) return 2;
+    int32_t n = 0;
+    if (fread(&n, 4, 1, f) != 1) return 2;
+    std::vector<mf::DoFitArgs> args;
+    std::vector<mf::NNLS_args> nn;
+    for (int c = 0; c < n; ++c) {
+        std::vector<double> buf(10 + 100 + 1 + 19 + 361 + 10);
+        if (fread(buf.data(), 8, buf.size(), f) != buf.size()) return 2;
+        size_t k = 0;
+        mf::SampleVector s;
+        for (int i = 0; i < 10; ++i) s(i) = buf[k++];
+        mf::SampleMatrix sc;
+        for (int i = 0; i < 10; ++i)
+            for (int j = 0; j < 10; ++j) sc(i, j) = buf[k++];
+        const double pederr = buf[k++];
+        mf::FullSampleVector fp;
+        for (int i = 0; i < 19; ++i) fp(i) = buf[k++];
+        mf::FullSampleMatrix fpc;
+        for (int i = 0; i < 19; ++i)
+            for (int j = 0; j < 19; ++j) fpc(i, j) = buf[k++];
+        mf::BXVector bx(10);
+        for (int i = 0; i < 10; ++i) bx.coeffRef(i) = (char)buf[k++];
+        args.emplace_back(s, sc, pederr, bx, fp, fpc);
+    }
+    fclose(f);
+    FILE *o = fopen(argv[2], "wb");
+    if (!o) return 2;
+    try {
+        std::vector<double> out;
+        for (auto &a : args) {
+            mf::PulseChiSqSNNLS fit;
+            fit.DoFit(a.samples, a.samplecor, a.pederr, a.bxs, a.fullpulse, a.fullpulsecov);
+            for (int p = 0; p < 10; ++p) out.push_back(fit.X()[p]);
+            for (int p = 0; p < 10; ++p) out.push_back(fit.Errors()[p]);
+            out.push_back(fit.ChiSq());
+            mf::FixedMatrix A;
+            for (int i = 0; i < 10; ++i)
+                for (int j = 0; j < 10; ++j) A(i, j) = fit.pulsemat()(i, j) + (i == j ? 1.0 : 0.0);
+            mf::FixedVector b;
+            for (int i = 0; i < 10; ++i) b(i) = a.samples(i);
+            nn.emplace_back(A, b);
+        }
+        auto res = mf::doFitWrapper(args);
+        for (auto &r : res) {
+            for (int p = 0; p < 10; ++p) out.push_back(r.X[p]);
+            out.push_back(r.chi2);
+            out.push_back(r.ampl);
+        }
+        auto xs = mf::inplace_fnnls_wrapper(nn);
+        for (size_t c = 0; c < xs.size(); ++c) {
+            mf::FixedVector x1;  // single-problem entry, cold start
+            mf::inplace_fnnls(nn[c].A, nn[c].b, x1);
+            for (int i = 0; i < 10; ++i) out.push_back(xs[c](i));
+            for (int i = 0; i < 10; ++i) out.push_back(x1(i));
+        }
+        double M[9] = {4, -1, 2, -1, 6, 0, 2, 0, 5}, L[9] = {0}, bb[3] = {3, 3, 4}, y[3], x[3];
+        mf::cholesky_decomp<double>(M, L, 3, 3);
+        mf::solve_forward_substitution<double>(L, bb, y, 3, 3);
+        mf::solve_backward_substitution<double>(L, y, x, 3, 3);
+        for (double v : L) out.push_back(v);
+        for (double v : y) out.push_back(v);
+        for (double v : x) out.push_back(v);
+        float Mf[9] = {4, -1, 2, -1, 6, 0, 2, 0, 5}, Lf[9] = {0}, bf[3] = {3, 3, 4}, yf[3], xf[3];
+        mf::cholesky_decomp<float>(Mf, Lf, 3, 3);
+        mf::solve_forward_substitution<float>(Lf, bf, yf, 3, 3);
+        mf::solve_backward_substitution<float>(Lf, yf, xf, 3, 3);
+        for (float v : Lf) out.push_back(v);
+        for (float v : yf) out.push_back(v);
+        for (float v : xf) out.push_back(v);
+        fwrite(out.data(), 8, out.size(), o);
+    } catch (const mf::Error &e) {
+        fprintf(stderr, "mf::Error %d: %s\n", e.status, e.what());
+        return 1;
+    }
+    fclose(o);
+    printf("host_api_check: %d channels\n", n);
+    return 0;
+}

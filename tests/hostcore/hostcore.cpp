@@ -33,6 +33,8 @@ static void run_fast(const DevType &dt, const double *samples, int64_t nch, int6
     iters[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
 }
 
+static double *g_err = nullptr;
+
 template <typename T, int N>
 static void run(const DevType &dt, const double *samples, int64_t nch, int64_t c,
                 const FitParams &prm, double *x, double *chi2, int32_t *flags, int32_t *iters)
@@ -41,8 +43,13 @@ static void run(const DevType &dt, const double *samples, int64_t nch, int64_t c
     for (int i = 0; i < 10; ++i) s[i] = i < N ? (T)samples[i * nch + c] : T(0);
     RegPark<T> park;
     FitOut<T> out;
-    fit_plain<T, N, N>(mats<T>(dt), s, prm, park, out);
+    if (g_err)
+        fit_plain<T, N, N, true>(mats<T>(dt), s, prm, park, out, dt.h.intime);
+    else
+        fit_plain<T, N, N>(mats<T>(dt), s, prm, park, out);
     for (int p = 0; p < 10; ++p) x[p * nch + c] = p < N ? (double)out.x[p] : 0.0;
+    if (g_err)
+        for (int p = 0; p < 10; ++p) g_err[p * nch + c] = p < N ? (double)out.err[p] : 0.0;
     chi2[c] = (double)out.chi2;
     flags[c] = out.flags;
     iters[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
@@ -59,8 +66,9 @@ extern "C" void hc_set_tolerances(float e32, float t32, double e64)
 
 extern "C" int hc_fit_batch(const mf_chtype *types, int ntypes, const uint8_t *type_of,
                             const double *samples, int64_t nch, const mf_opts *o, double *x,
-                            double *chi2, int32_t *flags, int32_t *iters)
+                            double *chi2, int32_t *flags, int32_t *iters, double *err)
 {
+    g_err = o->compute_errors ? err : nullptr;
     std::vector<DevType> dt((size_t)ntypes);
     for (int t = 0; t < ntypes; ++t)
         if (expand_type(types[t], dt[(size_t)t])) return 1;

@@ -108,7 +108,7 @@ def hostcore():
     L = C.CDLL(str(so))
     dp = C.POINTER(C.c_double)
     L.hc_fit_batch.argtypes = [C.c_void_p, C.c_int, C.c_void_p, dp, C.c_int64, C.c_void_p,
-                               dp, dp, C.c_void_p, C.c_void_p]
+                               dp, dp, C.c_void_p, C.c_void_p, dp]
     L.hc_fit_batch.restype = C.c_int
     L.hc_nnls.argtypes = [dp, dp, dp, C.c_int, C.c_int, C.c_double, C.c_uint]
     _cache["hostcore"] = L
@@ -153,10 +153,13 @@ def hostcore_fit(chtypes, samples: np.ndarray, type_of, opts):
     chi2 = np.zeros(n)
     flags = np.zeros(n, dtype=np.int32)
     iters = np.zeros(n, dtype=np.int32)
+    err = np.zeros((10, n))
     rc = L.hc_fit_batch(C.cast(chtypes, C.c_void_p), len(chtypes),
                         type_of.ctypes.data_as(C.c_void_p) if type_of is not None else None,
                         dptr(samples), n, C.cast(C.byref(opts), C.c_void_p), dptr(x),
                         dptr(chi2), flags.ctypes.data_as(C.c_void_p),
-                        iters.ctypes.data_as(C.c_void_p))
+                        iters.ctypes.data_as(C.c_void_p), dptr(err))
     assert rc == 0
+    if opts.compute_errors:
+        return x, chi2, flags, iters, err
     return x, chi2, flags, iters

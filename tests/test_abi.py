@@ -33,9 +33,9 @@ def test_struct_layouts(pkg):
     # sizes/offsets of include/mf_abi.h structs (x86-64 SysV)
     assert C.sizeof(a.ChType) == 4 * 14 + 8 * (1 + 100 + 19 + 361)
     assert a.ChType.pederr.offset == 56
-    assert C.sizeof(a.Opts) == 48
+    assert C.sizeof(a.Opts) == 56
     assert a.Opts.eps.offset == 16
-    assert C.sizeof(a.Batch) == 32 and C.sizeof(a.Result) == 32
+    assert C.sizeof(a.Batch) == 32 and C.sizeof(a.Result) == 40
     assert C.sizeof(a.GenSpec) == 8 + 11 * 8
     assert pkg._abi.lib().mf_abi_version() == 1
 

@@ -93,8 +93,15 @@ def test_fast_fit_edge_cases(pkg, gpu):
     fl = r.flags.cpu().numpy()
     assert np.all(x[:, 0] == 0) and r.chi2[0].item() == 0.0
     assert fl[1] & _abi.FLAGS["nonfinite"] and fl[4] & _abi.FLAGS["nonfinite"]
-    ref = oracle_fit(ct, s[:, [0, 2, 3]].astype(np.float64), None, oracle_opts())
+    ref = oracle_fit(ct, s[:, [0, 3]].astype(np.float64), None, oracle_opts())
     xr = ref["x"].T
     scale = np.maximum(np.abs(xr).max(axis=0), 1.0)
-    assert np.all(np.abs(x[:, [0, 2, 3]] - xr) / scale <= TOL_X)
-    assert np.all((x[:, [0, 2, 3]] > 0) == (xr > 0))
+    assert np.all(np.abs(x[:, [0, 3]] - xr) / scale <= TOL_X)
+    assert np.all((x[:, [0, 3]] > 0) == (xr > 0))
+    # A 1e5 ADC pulse: FP64 rounding of w (~1e-16 |Atb|) exceeds the
+    # reference's eps 1e-11, so the reference's second FNNLS call decides its
+    # warm-start jump on rounding noise and may end away from the optimum.
+    # The fast path returns the NNLS optimum: its chi2 is never worse.
+    big = oracle_fit(ct, s[:, [2]].astype(np.float64), None, oracle_opts())
+    assert r.chi2[2].item() <= big["chi2"][0] * (1 + 1e-6)
+    assert np.all(x[:, 2] >= 0) and np.isfinite(x[:, 2]).all()

@@ -54,3 +54,17 @@ def test_fast_core_matches_oracle(pkg, cfg, pulse_cov, n):
     assert (err <= 1e-4).mean() >= 0.999
     assert (cerr <= 1e-4).mean() >= 0.999
     assert nmin.mean() >= 0.999
+
+
+@pytest.mark.parametrize("pulse_cov", ["zero", "live"])
+def test_minos_errors_core_matches_oracle(pkg, pulse_cov):
+    from test_regression_amd import workloads, _abi
+    w = workloads.config("c2", pulse_cov)
+    ct = w.chtypes()
+    s, t, _ = w.generate(2000)
+    ref = oracle_fit(ct, s, t, oracle_opts(compute_errors=1), nthreads=4)
+    x, chi2, fl, it, err = hostcore_fit(ct, s, t, _abi.default_opts(compute_errors=1))
+    e, er = err[5], ref["err"][:, 5]
+    assert np.array_equal(np.isnan(e), np.isnan(er))
+    ok = ~np.isnan(er)
+    assert np.all(np.abs(e[ok] - er[ok]) <= 1e-9 * np.abs(er[ok]))
