@@ -67,8 +67,11 @@ typedef struct mf_chtype {
 enum {
     MF_PREC_FP64 = 0,  /* plain cold-start FP64 fit with the reference's
                           semantics (multifit_gpu: FNNLS max_iter 1000)     */
-    MF_PREC_FP32 = 1   /* FP32 bulk, incremental Cholesky, FP64 pivot
-                          fallback and FP64 refinement residual            */
+    MF_PREC_FP32 = 1   /* fast path: active-set exploration in FP32 on the
+                          type's precomputed normal equations, finished in
+                          FP64 (refinement + KKT verification); later
+                          Minimize iterations of a live pulse covariance
+                          in FP64                                           */
 };
 enum { MF_SOLVER_INPLACE_FNNLS = 0, MF_SOLVER_FNNLS = 1 };
 enum { MF_DTYPE_F64 = 0, MF_DTYPE_F32 = 1 };
@@ -81,7 +84,8 @@ typedef struct mf_opts {
     double  eps;            /* FNNLS max-w threshold (reference: 1e-11)       */
     double  chi2_tol;       /* Minimize |dchi2| threshold (reference: 1e-3)   */
     int32_t use_pulse_cov;  /* 1: add x_j^2 * fullpulsecov blocks (updateCov) */
-    int32_t warm_start;     /* FP32 only: 1 = learned warm start (needs weights) */
+    int32_t warm_start;     /* FP32 only: 1 = start from the MLP's prediction
+                               (mf_mlp_*; every result is still KKT-verified) */
     int32_t persistent;     /* 1: persistent kernel + device work queue       */
     int32_t block;          /* threads per block (0 = tuned default)          */
     int32_t compute_errors; /* FP64 only: MINOS-like in-time uncertainty
@@ -174,6 +178,32 @@ enum {
 mf_status mf_linalg_batch(int32_t op, int32_t dtype, void *M, void *L, void *b,
                           void *y, int64_t n, int32_t full, int32_t view,
                           int32_t i, int32_t j, void *stream);
+
+/* ---- learned warm start ---------------------------------------------------
+ * No counterpart in the reference: a 10 -> 64 -> 64 -> 20 MLP (GELU) predicts
+ * a channel's passive set (logits 0..9 > 0) and amplitudes (10..19) from its
+ * samples scaled by the largest sample; mf_fit_batch with opts.warm_start = 1
+ * starts the fast path's active-set solve there.  Parameters: float32
+ * [W1 10x64, b1 64, W2 64x64, b2 64, W3 64x20, b3 20] row-major [in][out],
+ * used rounded to bfloat16 with float32 accumulation.  Device pointers
+ * unless stated; `samples` / `xref` are [10][ld] float32 (row i at i*ld). */
+#define MF_MLP_NPARAM 6164
+/* Xavier-uniform weights, zero biases, Adam state reset (seeded). */
+mf_status mf_mlp_init(mf_context *ctx, uint64_t seed);
+/* host arrays of MF_MLP_NPARAM floats; set also resets the Adam state */
+mf_status mf_mlp_set_params(mf_context *ctx, const float *params, int32_t n);
+mf_status mf_mlp_get_params(mf_context *ctx, float *params, int32_t n);
+/* gradient of the mean loss over B channels (grad: MF_MLP_NPARAM floats,
+ * loss: 1 float, may be NULL); loss per channel = mean_i BCE(logit_i,
+ * [xref_i > 0]) + mean_i (amp_i / m - xref_i / m)^2 */
+mf_status mf_mlp_grad(mf_context *ctx, const float *samples, int64_t ld, const float *xref,
+                      int64_t ldx, int64_t B, float *grad, float *loss, void *stream);
+/* one Adam step (lr) on that gradient; loss may be NULL */
+mf_status mf_mlp_train_step(mf_context *ctx, const float *samples, int64_t ld,
+                            const float *xref, int64_t ldx, int64_t B, float lr, float *loss,
+                            void *stream);
+/* inference for a batch: out [20][nch] float32 (logits, then amplitudes) */
+mf_status mf_mlp_predict(mf_context *ctx, const mf_batch *in, float *out, void *stream);
 
 #ifdef __cplusplus
 }

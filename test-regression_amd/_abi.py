@@ -17,6 +17,7 @@ MF_OK, MF_ERR_INVALID, MF_ERR_HIP, MF_ERR_NOMEM, MF_ERR_UNSUPPORTED = range(5)
 MF_PREC_FP64, MF_PREC_FP32 = 0, 1
 MF_SOLVER_INPLACE_FNNLS, MF_SOLVER_FNNLS = 0, 1
 MF_DTYPE_F64, MF_DTYPE_F32 = 0, 1
+MF_MLP_NPARAM = 6164
 (MF_LA_CHOLESKY, MF_LA_FORWARD, MF_LA_BACKWARD, MF_LA_SWAP, MF_LA_RCADDITION,
  MF_LA_INBETWEEN) = range(6)
 
@@ -93,6 +94,14 @@ SIGNATURES = [
     ("mf_linalg_batch", C.c_int, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
                                   C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
+    ("mf_mlp_init", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("mf_mlp_set_params", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
+    ("mf_mlp_get_params", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
+    ("mf_mlp_grad", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                              C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("mf_mlp_train_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                    C.c_int64, C.c_float, C.c_void_p, C.c_void_p]),
+    ("mf_mlp_predict", C.c_int, [C.c_void_p, C.POINTER(Batch), C.c_void_p, C.c_void_p]),
     ("mf_gen_type", C.c_int, [C.POINTER(GenSpec), C.POINTER(ChType)]),
     ("mf_gen_samples", C.c_int, [C.POINTER(GenSpec), C.c_int32, C.c_void_p, C.c_int64,
                                  C.c_uint64, C.c_void_p, C.c_void_p, C.c_int32]),

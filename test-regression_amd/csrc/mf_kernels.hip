@@ -19,6 +19,7 @@
 
 #include "../../include/mf_abi.h"
 #include "fit_core.hpp"
+#include "mlp.hpp"
 #include "types_host.hpp"
 
 using namespace mf;
@@ -91,26 +92,33 @@ void k_fit_plain(const DevType *__restrict__ types, int type_id, const int32_t *
 // Fast path (MF_PREC_FP32): FP32 bulk with FP64 polish, see fit_fast.
 // LIVE: the type's pulse covariance is live, later iterations run in FP64
 // with L^{-1} parked in LDS.
+// warm: null, or the MLP's prediction [20][nch] (logits, amplitudes): the
+// passive set {i: logit_i > 0} with its predicted amplitudes starts the solve.
 template <typename TS, int N, bool LIVE>
 __global__ __launch_bounds__(256)
 void k_fit_fast(const DevType *__restrict__ types, int type_id, const int32_t *__restrict__ idx,
                 int64_t nitems, const TS *__restrict__ samples, int64_t nch,
                 float *__restrict__ xo, float *__restrict__ chi2o, int32_t *__restrict__ flo,
-                int32_t *__restrict__ ito, FitParams prm)
+                int32_t *__restrict__ ito, const float *__restrict__ warm, FitParams prm)
 {
     extern __shared__ unsigned char lds_raw[];
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nitems) return;
     const int64_t c = idx ? (int64_t)idx[g] : g;
     float s[10], x[10];
+    uint32_t pm = 0;
 #pragma unroll
     for (int i = 0; i < 10; ++i) {
         s[i] = i < N ? (float)samples[(int64_t)i * nch + c] : 0.0f;
         x[i] = 0.0f;
+        if (warm && i < N && warm[(int64_t)i * nch + c] > 0.0f) {
+            pm |= 1u << i;
+            x[i] = fmaxf(warm[(int64_t)(10 + i) * nch + c], 1e-6f);
+        }
     }
     LdsPark<double> park{reinterpret_cast<double *>(lds_raw) + threadIdx.x, (int)blockDim.x};
     FitOut<float> out;
-    fit_fast<N, N, LIVE>(types[type_id], s, prm, x, 0u, false, park, out);
+    fit_fast<N, N, LIVE>(types[type_id], s, prm, x, pm, warm != nullptr, park, out);
 #pragma unroll
     for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = out.x[p];
     chi2o[c] = out.chi2;
@@ -312,6 +320,16 @@ struct mf_context {
     void *d_part = nullptr;  // counts[256], cursors[256], offsets[256] (int64)
     int32_t *d_idx = nullptr;
     int64_t idx_cap = 0;
+    // warm-start MLP: masters, bfloat16 copies (as floats and as bits),
+    // gradient, Adam moments; activations of the largest batch seen
+    float *mlp_p = nullptr, *mlp_pq = nullptr, *mlp_g = nullptr, *mlp_m = nullptr,
+          *mlp_v = nullptr;
+    uint16_t *mlp_bf = nullptr;
+    int mlp_t = 0;
+    bool mlp_ready = false;
+    mf::mlp::Scratch mlp_w;
+    float *pred = nullptr;  // [20][pred_cap] prediction for warm starts
+    int64_t pred_cap = 0;
 };
 
 extern "C" mf_status mf_default_opts(mf_opts *o)
@@ -348,6 +366,12 @@ extern "C" mf_status mf_context_destroy(mf_context *ctx)
     if (ctx->d_types) (void)hipFree(ctx->d_types);
     if (ctx->d_part) (void)hipFree(ctx->d_part);
     if (ctx->d_idx) (void)hipFree(ctx->d_idx);
+    for (void *p : {(void *)ctx->mlp_p, (void *)ctx->mlp_pq, (void *)ctx->mlp_g,
+                    (void *)ctx->mlp_m, (void *)ctx->mlp_v, (void *)ctx->mlp_bf,
+                    (void *)ctx->mlp_w.X, (void *)ctx->mlp_w.m, (void *)ctx->mlp_w.Z1,
+                    (void *)ctx->mlp_w.H1, (void *)ctx->mlp_w.Z2, (void *)ctx->mlp_w.H2,
+                    (void *)ctx->mlp_w.Z3, (void *)ctx->mlp_w.lossdummy, (void *)ctx->pred})
+        if (p) (void)hipFree(p);
     delete ctx;
     return MF_OK;
 }
@@ -401,7 +425,7 @@ static mf_status launch_fit(mf_context *ctx, int type_id, const int32_t *idx,
 template <typename TS, int N, bool LIVE>
 static mf_status launch_fast(mf_context *ctx, int type_id, const int32_t *idx, int64_t nitems,
                              const void *samples, int64_t nch, const mf_result *out,
-                             const FitParams &prm, int block, hipStream_t st)
+                             const FitParams &prm, int block, const float *warm, hipStream_t st)
 {
     if (nitems <= 0) return MF_OK;
     const int64_t grid = (nitems + block - 1) / block;
@@ -409,7 +433,7 @@ static mf_status launch_fast(mf_context *ctx, int type_id, const int32_t *idx, i
     hipLaunchKernelGGL((k_fit_fast<TS, N, LIVE>), dim3((unsigned)grid), dim3(block), lds, st,
                        ctx->d_types, type_id, idx, nitems, static_cast<const TS *>(samples), nch,
                        static_cast<float *>(out->x), static_cast<float *>(out->chi2), out->flags,
-                       out->iters, prm);
+                       out->iters, warm, prm);
     HIP_TRY(hipGetLastError());
     return MF_OK;
 }
@@ -417,7 +441,7 @@ static mf_status launch_fast(mf_context *ctx, int type_id, const int32_t *idx, i
 template <typename TS>
 static mf_status dispatch_fast(mf_context *ctx, int type_id, const int32_t *idx, int64_t nitems,
                                const mf_batch *in, const mf_result *out, const FitParams &prm,
-                               int block, hipStream_t st)
+                               int block, const float *warm, hipStream_t st)
 {
     const mf_chtype &t = ctx->types[(size_t)type_id];
     bool has_pi = false;
@@ -425,13 +449,13 @@ static mf_status dispatch_fast(mf_context *ctx, int type_id, const int32_t *idx,
     const bool live = has_pi && prm.use_pi;
     if (t.nsamples == 10)
         return live ? launch_fast<TS, 10, true>(ctx, type_id, idx, nitems, in->samples, in->nch,
-                                                out, prm, block, st)
+                                                out, prm, block, warm, st)
                     : launch_fast<TS, 10, false>(ctx, type_id, idx, nitems, in->samples, in->nch,
-                                                 out, prm, block, st);
+                                                 out, prm, block, warm, st);
     return live ? launch_fast<TS, 8, true>(ctx, type_id, idx, nitems, in->samples, in->nch, out,
-                                           prm, block, st)
+                                           prm, block, warm, st)
                 : launch_fast<TS, 8, false>(ctx, type_id, idx, nitems, in->samples, in->nch, out,
-                                            prm, block, st);
+                                            prm, block, warm, st);
 }
 
 template <typename T>
@@ -471,7 +495,10 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
         return fail(MF_ERR_INVALID, "max_minimize and max_iter must be >= 0");
     if (opts->solver != MF_SOLVER_INPLACE_FNNLS && opts->solver != MF_SOLVER_FNNLS)
         return fail(MF_ERR_INVALID, "unknown solver");
-    if (opts->warm_start) return fail(MF_ERR_UNSUPPORTED, "warm start is not available yet");
+    if (opts->warm_start && opts->precision != MF_PREC_FP32)
+        return fail(MF_ERR_UNSUPPORTED, "warm_start needs MF_PREC_FP32");
+    if (opts->warm_start && !ctx->mlp_ready)
+        return fail(MF_ERR_INVALID, "warm_start needs MLP parameters (mf_mlp_init / mf_mlp_set_params)");
     int block = opts->block > 0 ? opts->block : 64;
     if (block % 64 || block > 256) return fail(MF_ERR_INVALID, "block must be 64, 128, 192 or 256");
     HIP_TRY(hipSetDevice(ctx->device));
@@ -493,13 +520,31 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
     if (opts->compute_errors && !out->err)
         return fail(MF_ERR_INVALID, "compute_errors needs mf_result.err");
     const int ntypes = (int)ctx->types.size();
+    const float *warm = nullptr;
+    if (opts->warm_start) {
+        if (ctx->pred_cap < in->nch) {
+            if (ctx->pred) HIP_TRY(hipFree(ctx->pred));
+            ctx->pred = nullptr;
+            HIP_TRY(hipMalloc(&ctx->pred, sizeof(float) * 20 * (size_t)in->nch));
+            ctx->pred_cap = in->nch;
+        }
+        const int ns = ctx->types[0].nsamples > 8 ? 10 : 8;
+        if (in->sample_dtype == MF_DTYPE_F64)
+            mf::mlp::launch_infer<double>(in->samples, in->nch, ns, nullptr, in->nch, ctx->mlp_bf,
+                                          ctx->mlp_pq, ctx->pred, in->nch, st);
+        else
+            mf::mlp::launch_infer<float>(in->samples, in->nch, ns, nullptr, in->nch, ctx->mlp_bf,
+                                         ctx->mlp_pq, ctx->pred, in->nch, st);
+        HIP_TRY(hipGetLastError());
+        warm = ctx->pred;
+    }
     auto go = [&](int type_id, const int32_t *idx, const int64_t *count, int64_t nitems) {
         if (opts->precision == MF_PREC_FP64)
             return dispatch_fit<double>(ctx, type_id, idx, count, nitems, in, out, prm, block,
                                         opts->compute_errors != 0, st);
         return in->sample_dtype == MF_DTYPE_F64
-                   ? dispatch_fast<double>(ctx, type_id, idx, nitems, in, out, prm, block, st)
-                   : dispatch_fast<float>(ctx, type_id, idx, nitems, in, out, prm, block, st);
+                   ? dispatch_fast<double>(ctx, type_id, idx, nitems, in, out, prm, block, warm, st)
+                   : dispatch_fast<float>(ctx, type_id, idx, nitems, in, out, prm, block, warm, st);
     };
     if (!in->type_of || ntypes == 1) return go(0, nullptr, nullptr, in->nch);
     // mixed batch: partition channel indices by type on the device
@@ -540,6 +585,147 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
     }
     // hc/ho are host stack arrays: make sure the H2D copy of ho finished
     HIP_TRY(hipStreamSynchronize(st));
+    return MF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// warm-start MLP entry points
+// ---------------------------------------------------------------------------
+static mf_status mlp_alloc(mf_context *ctx)
+{
+    if (ctx->mlp_p) return MF_OK;
+    const size_t n = sizeof(float) * mf::mlp::NPARAM;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMalloc(&ctx->mlp_p, n));
+    HIP_TRY(hipMalloc(&ctx->mlp_pq, n));
+    HIP_TRY(hipMalloc(&ctx->mlp_g, n));
+    HIP_TRY(hipMalloc(&ctx->mlp_m, n));
+    HIP_TRY(hipMalloc(&ctx->mlp_v, n));
+    HIP_TRY(hipMalloc(&ctx->mlp_bf, sizeof(uint16_t) * mf::mlp::NPARAM));
+    return MF_OK;
+}
+
+static mf_status mlp_upload(mf_context *ctx, const float *params)
+{
+    mf_status s = mlp_alloc(ctx);
+    if (s != MF_OK) return s;
+    const size_t n = sizeof(float) * mf::mlp::NPARAM;
+    HIP_TRY(hipMemcpy(ctx->mlp_p, params, n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(ctx->mlp_m, 0, n));
+    HIP_TRY(hipMemset(ctx->mlp_v, 0, n));
+    HIP_TRY(mf::mlp::quantize(ctx->mlp_p, ctx->mlp_pq, ctx->mlp_bf, nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    ctx->mlp_t = 0;
+    ctx->mlp_ready = true;
+    return MF_OK;
+}
+
+extern "C" mf_status mf_mlp_init(mf_context *ctx, uint64_t seed)
+{
+    if (!ctx) return fail(MF_ERR_INVALID, "ctx is NULL");
+    // Xavier-uniform weights from a 64-bit LCG (Knuth MMIX constants), zero biases
+    std::vector<float> p((size_t)mf::mlp::NPARAM, 0.0f);
+    uint64_t st = seed * 6364136223846793005ull + 1442695040888963407ull;
+    auto uni = [&]() {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        return (double)(st >> 11) * (1.0 / 9007199254740992.0);
+    };
+    const int shapes[3][2] = {{10, 64}, {64, 64}, {64, 20}};
+    size_t k = 0;
+    for (auto &sh : shapes) {
+        const double lim = sqrt(6.0 / (sh[0] + sh[1]));
+        for (int i = 0; i < sh[0] * sh[1]; ++i) p[k++] = (float)((2.0 * uni() - 1.0) * lim);
+        k += (size_t)sh[1];  // bias stays 0
+    }
+    return mlp_upload(ctx, p.data());
+}
+
+extern "C" mf_status mf_mlp_set_params(mf_context *ctx, const float *params, int32_t n)
+{
+    if (!ctx || !params) return fail(MF_ERR_INVALID, "NULL argument");
+    if (n != mf::mlp::NPARAM) return fail(MF_ERR_INVALID, "expected %d parameters", mf::mlp::NPARAM);
+    return mlp_upload(ctx, params);
+}
+
+extern "C" mf_status mf_mlp_get_params(mf_context *ctx, float *params, int32_t n)
+{
+    if (!ctx || !params) return fail(MF_ERR_INVALID, "NULL argument");
+    if (n != mf::mlp::NPARAM) return fail(MF_ERR_INVALID, "expected %d parameters", mf::mlp::NPARAM);
+    if (!ctx->mlp_ready) return fail(MF_ERR_INVALID, "MLP parameters not set");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(params, ctx->mlp_p, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+    return MF_OK;
+}
+
+static mf_status mlp_scratch(mf_context *ctx, int64_t B)
+{
+    mf::mlp::Scratch &w = ctx->mlp_w;
+    if (w.cap >= B) return MF_OK;
+    for (float **p : {&w.X, &w.m, &w.Z1, &w.H1, &w.Z2, &w.H2, &w.Z3, &w.lossdummy})
+        if (*p) {
+            HIP_TRY(hipFree(*p));
+            *p = nullptr;
+        }
+    const size_t b = (size_t)B;
+    HIP_TRY(hipMalloc(&w.X, sizeof(float) * 16 * b));
+    HIP_TRY(hipMalloc(&w.m, sizeof(float) * b));
+    HIP_TRY(hipMalloc(&w.Z1, sizeof(float) * 64 * b));
+    HIP_TRY(hipMalloc(&w.H1, sizeof(float) * 64 * b));
+    HIP_TRY(hipMalloc(&w.Z2, sizeof(float) * 64 * b));
+    HIP_TRY(hipMalloc(&w.H2, sizeof(float) * 64 * b));
+    HIP_TRY(hipMalloc(&w.Z3, sizeof(float) * 20 * b));
+    HIP_TRY(hipMalloc(&w.lossdummy, sizeof(float)));
+    w.cap = B;
+    return MF_OK;
+}
+
+extern "C" mf_status mf_mlp_grad(mf_context *ctx, const float *samples, int64_t ld,
+                                 const float *xref, int64_t ldx, int64_t B, float *grad,
+                                 float *loss, void *stream)
+{
+    if (!ctx || !samples || !xref || !grad) return fail(MF_ERR_INVALID, "NULL argument");
+    if (B <= 0 || B > (1 << 22) || ld < B || ldx < B)
+        return fail(MF_ERR_INVALID, "need 0 < B <= 4194304 and ld, ldx >= B");
+    if (!ctx->mlp_ready) return fail(MF_ERR_INVALID, "MLP parameters not set");
+    HIP_TRY(hipSetDevice(ctx->device));
+    mf_status s = mlp_scratch(ctx, B);
+    if (s != MF_OK) return s;
+    HIP_TRY(mf::mlp::grad_step(ctx->mlp_w, samples, ld, xref, ldx, B, ctx->mlp_pq, grad, loss,
+                               static_cast<hipStream_t>(stream)));
+    return MF_OK;
+}
+
+extern "C" mf_status mf_mlp_train_step(mf_context *ctx, const float *samples, int64_t ld,
+                                       const float *xref, int64_t ldx, int64_t B, float lr,
+                                       float *loss, void *stream)
+{
+    if (!ctx) return fail(MF_ERR_INVALID, "ctx is NULL");
+    mf_status s = mf_mlp_grad(ctx, samples, ld, xref, ldx, B, ctx ? ctx->mlp_g : nullptr, loss,
+                              stream);
+    if (s != MF_OK) return s;
+    ctx->mlp_t += 1;
+    HIP_TRY(mf::mlp::adam_step(ctx->mlp_p, ctx->mlp_g, ctx->mlp_m, ctx->mlp_v, ctx->mlp_t, lr,
+                               ctx->mlp_pq, ctx->mlp_bf, static_cast<hipStream_t>(stream)));
+    return MF_OK;
+}
+
+extern "C" mf_status mf_mlp_predict(mf_context *ctx, const mf_batch *in, float *out,
+                                    void *stream)
+{
+    if (!ctx || !in || !out) return fail(MF_ERR_INVALID, "NULL argument");
+    if (!ctx->mlp_ready) return fail(MF_ERR_INVALID, "MLP parameters not set");
+    if (in->nch <= 0) return MF_OK;
+    if (!in->samples) return fail(MF_ERR_INVALID, "samples is NULL");
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (in->sample_dtype == MF_DTYPE_F64)
+        mf::mlp::launch_infer<double>(in->samples, in->nch, 10, nullptr, in->nch, ctx->mlp_bf,
+                                      ctx->mlp_pq, out, in->nch, st);
+    else
+        mf::mlp::launch_infer<float>(in->samples, in->nch, 10, nullptr, in->nch, ctx->mlp_bf,
+                                     ctx->mlp_pq, out, in->nch, st);
+    HIP_TRY(hipGetLastError());
     return MF_OK;
 }
 

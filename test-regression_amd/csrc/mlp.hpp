@@ -1,0 +1,34 @@
+// mlp.hpp -- host-side interface of the warm-start MLP kernels (mlp.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+namespace mf {
+namespace mlp {
+
+constexpr int NPARAM = 6164;  // [W1 10x64, b1 64, W2 64x64, b2 64, W3 64x20, b3 20]
+
+// activations of one training batch (device memory, batch-major)
+struct Scratch {
+    int64_t cap = 0;
+    float *X = nullptr, *m = nullptr, *Z1 = nullptr, *H1 = nullptr, *Z2 = nullptr,
+          *H2 = nullptr, *Z3 = nullptr, *lossdummy = nullptr;
+};
+
+// loss (device scalar, may be null) and gradient of one batch
+hipError_t grad_step(Scratch &w, const float *samples, int64_t lds, const float *xref,
+                     int64_t ldx, int64_t B, const float *pq, float *grad, float *loss,
+                     hipStream_t st);
+hipError_t adam_step(float *p, const float *g, float *m, float *v, int t, float lr, float *pq,
+                     uint16_t *pbf, hipStream_t st);
+hipError_t quantize(const float *p, float *pq, uint16_t *pbf, hipStream_t st);
+hipError_t zero(float *p, int64_t n, hipStream_t st);
+template <typename TS>
+void launch_infer(const void *s, int64_t ld, int ns, const int32_t *idx, int64_t n,
+                  const uint16_t *pbf, const float *pq, float *out, int64_t ldo, hipStream_t st);
+
+}  // namespace mlp
+}  // namespace mf
