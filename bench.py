@@ -2,26 +2,30 @@
 """bench.py -- fitted channels/s of the multifit engine on one MI355X.
 
 A step is one mf_fit_batch over one batch of synthetic channels already
-resident in HBM (structure of arrays).  Default workload: 1,000,000 ECAL-like
-channels, 10 samples x 10 BX, pileup 40, in-time amplitude U[1,500], pulse
-jitter N(0, 1 ns).  Reported:
+resident in HBM (structure of arrays).  Default workload (c2): 1,000,000
+ECAL-like channels, 10 samples x 10 BX, in-time amplitude U[1,500], per-BX
+Poisson(40/10) pileup of Exponential(2) ADC, N(0, 1 ns) jitter, zero pulse
+covariance (as the reference's drivers pass it).  Paths timed on the same
+GPU and the same batch:
 
-  value        channels/s of the fast path (FP32 bulk, warm active set,
-               FP64 refinement) -- or of the plain FP64 path while the fast
-               path is not built
-  plain_fp64   channels/s of the plain cold-start FP64 fit (reference
-               semantics, FNNLS max_iter 1000)
-  roofline     dominant kernel: algorithmic flops per launch (the plain
-               method's operation count, counted by the oracle on a sample of
-               the same workload) / its mean launch time from HIP events on
-               the launch stream, against the MI355X vector peak
-  cpu_baseline the oracle (a C restatement of multifit_cpu) on the host's
-               cores, on a bounded sample of the same workload
+  plain   plain cold-start FP64 fit, the reference's semantics
+          (multifit_gpu: inplace_fnnls with max_iter 1000)
+  fast    fast path, cold start: FP32 active-set exploration on the type's
+          precomputed normal equations, FP64 polish and KKT verification
+  warm    fast path started from the MLP's predicted passive set and
+          amplitudes (inference included in the step) -- the reported value
+
+Reported next to the value: the plain path's rate and the speedups, the
+error of every fitted amplitude and chi2 of the warm path against the plain
+FP64 fit over the whole batch (and against the FP64 oracle on a sample), the
+roofline of the dominant kernel (the fit kernel of the warm path: algorithmic
+flops per launch -- the plain method's operation count, counted by the oracle
+on a sample of the batch -- over the kernel's launch time from HIP events on
+its stream), and the oracle's throughput on the host cores.
 """
 from __future__ import annotations
 
 import argparse
-import ctypes as C
 import json
 import os
 import sys
@@ -33,52 +37,65 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "tests"))
 
-PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}
+PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def time_steps(fn, steps, warmup, stream):
+def time_steps(eng, fn, steps, warmup, stream):
     import torch
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
+    eng.set_timing(True)
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(steps)]
-    t0 = time.perf_counter()
+    kern = []
     for a, b in evs:
         a.record(stream)
         fn()
         b.record(stream)
+        kern.append(eng.timings())   # waits for this step's kernel events
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    eng.set_timing(False)
     per = [a.elapsed_time(b) for a, b in evs]
-    return wall, per
+    return per, kern
 
 
 def cpu_baseline(ct, s, t, seconds, threads):
-    """Oracle throughput on host cores over a bounded prefix of the batch."""
+    """Oracle throughput on host cores over bounded slices of the batch,
+    repeated until about `seconds` of CPU work."""
     from oracle_binding import oracle_fit, oracle_opts
-    n_probe = 2000
-    t0 = time.perf_counter()
-    oracle_fit(ct, s[:, :n_probe].copy(), t[:n_probe] if t is not None else None,
-               oracle_opts(), nthreads=threads)
-    rate = n_probe / max(time.perf_counter() - t0, 1e-6)
-    n = int(min(s.shape[1], max(n_probe, rate * seconds)))
-    t0 = time.perf_counter()
-    oracle_fit(ct, np.ascontiguousarray(s[:, :n]), t[:n] if t is not None else None,
-               oracle_opts(), nthreads=threads)
+    n = min(s.shape[1], 200_000)
+    sl = np.ascontiguousarray(s[:, :n])
+    tl = t[:n] if t is not None else None
+    done, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        oracle_fit(ct, sl, tl, oracle_opts(), nthreads=threads)
+        done += n
     dt = time.perf_counter() - t0
-    return n / dt, n, dt
+    return done / dt, n, done, dt
 
 
-def algorithmic_flops(ct, s, t, n=20000):
+def oracle_sample(ct, s, t, n):
     from oracle_binding import oracle_fit, oracle_opts
-    r = oracle_fit(ct, np.ascontiguousarray(s[:, :n]), t[:n] if t is not None else None,
-                   oracle_opts(count_flops=1), nthreads=min(16, os.cpu_count() or 1))
-    return float(r["aflops"].mean())
+    return oracle_fit(ct, np.ascontiguousarray(s[:, :n]), t[:n] if t is not None else None,
+                      oracle_opts(count_flops=1), nthreads=min(16, os.cpu_count() or 1))
+
+
+def errors(x, chi2, xr, chi2r):
+    """Relative errors (channel max amplitude floor 1 ADC, chi2 floor 1) and
+    the fraction of channels with identical active sets."""
+    scale = np.maximum(np.abs(xr).max(axis=0), 1.0)
+    e = (np.abs(x - xr) / scale).max(axis=0)
+    c = np.abs(chi2 - chi2r) / np.maximum(chi2r, 1.0)
+    aset = ((x > 0) == (xr > 0)).all(axis=0)
+    return {"amp_rel_err_max": float(e.max()), "amp_rel_err_p999": float(np.quantile(e, 0.999)),
+            "frac_amp_within_1e-4": float((e <= 1e-4).mean()),
+            "chi2_rel_err_max": float(c.max()), "chi2_rel_err_p999": float(np.quantile(c, 0.999)),
+            "identical_active_sets": float(aset.mean()), "channels": int(x.shape[1])}
 
 
 def main():
@@ -89,10 +106,13 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--pulse-cov", default="zero", choices=["zero", "live"])
     ap.add_argument("--nch", type=int, default=0)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", default="", help="plain|fast: time one path only")
+    ap.add_argument("--oracle-sample", type=int, default=20_000)
+    ap.add_argument("--mlp-weights", default="", help="parameters to load (default: the "
+                    "committed weights, else train on 4M c2-distribution channels)")
+    ap.add_argument("--save-mlp", default="", help="write the trained parameters here")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the multifit path has no cross-GPU exchange "
@@ -102,7 +122,7 @@ def main():
     import torch
     from oracle_binding import load_pkg
     load_pkg()
-    from test_regression_amd import _abi, engine, workloads
+    from test_regression_amd import _abi, engine, warmstart, workloads
 
     w = workloads.config(args.config, args.pulse_cov)
     nch = args.nch or w.nch
@@ -117,39 +137,54 @@ def main():
     tt = torch.from_numpy(t).to(dev) if t is not None else None
     stream = torch.cuda.current_stream()
 
-    res = {}
-    o64 = _abi.default_opts(precision=_abi.MF_PREC_FP64)
-    out64 = eng.alloc_result(nch, _abi.MF_PREC_FP64)
-    if args.only in ("", "plain"):
-        wall, per = time_steps(lambda: eng.fit(s64, tt, o64, out64), args.steps, args.warmup,
+    # learned warm start: load or train (not timed)
+    wpath = Path(args.mlp_weights) if args.mlp_weights else warmstart.WEIGHTS
+    t0 = time.perf_counter()
+    mlp_info = warmstart.ensure_trained(eng, path=wpath if wpath.exists() else None, log=log)
+    mlp_info["seconds"] = time.perf_counter() - t0
+    if args.save_mlp:
+        np.save(args.save_mlp, eng.mlp_get_params())
+    log(f"mlp: {mlp_info}")
+
+    O = _abi.default_opts
+    paths = {
+        "plain": (s64, O(precision=_abi.MF_PREC_FP64), _abi.MF_PREC_FP64),
+        "fast": (s32, O(precision=_abi.MF_PREC_FP32), _abi.MF_PREC_FP32),
+        "warm": (s32, O(precision=_abi.MF_PREC_FP32, warm_start=1), _abi.MF_PREC_FP32),
+    }
+    res, outs = {}, {}
+    for name, (x_in, o, prec) in paths.items():
+        out = eng.alloc_result(nch, prec)
+        per, kern = time_steps(eng, lambda: eng.fit(x_in, tt, o, out), args.steps, args.warmup,
                                stream)
-        res["plain"] = dict(wall=wall, per=per)
-        log(f"plain fp64: {np.mean(per):.3f} ms/step -> {nch / (np.mean(per) / 1e3):.3e} ch/s")
+        res[name] = {"ms": float(np.mean(per)), "fit_ms": float(np.mean([k["fit_ms"] for k in kern])),
+                     "mlp_ms": float(np.mean([k["mlp_ms"] for k in kern]))}
+        outs[name] = out
+        log(f"{name}: {res[name]['ms']:.3f} ms/step (fit kernel {res[name]['fit_ms']:.3f} ms, "
+            f"mlp {res[name]['mlp_ms']:.3f} ms) -> {nch / (res[name]['ms'] / 1e3):.3e} ch/s")
+    torch.cuda.synchronize()
 
-    fast_ok = False
-    if args.only in ("", "fast"):
-        try:
-            o32 = _abi.default_opts(precision=_abi.MF_PREC_FP32)
-            out32 = eng.alloc_result(nch, _abi.MF_PREC_FP32)
-            eng.fit(s32, tt, o32, out32)
-            torch.cuda.synchronize()
-            fast_ok = True
-            if fast_ok:
-                wall, per = time_steps(lambda: eng.fit(s32, tt, o32, out32), args.steps,
-                                       args.warmup, stream)
-                res["fast"] = dict(wall=wall, per=per)
-                log(f"fast fp32: {np.mean(per):.3f} ms/step -> "
-                    f"{nch / (np.mean(per) / 1e3):.3e} ch/s")
-        except _abi.MFError as e:
-            log(f"fast path unavailable: {e}")
+    # accuracy: warm (and cold) fast path vs the plain FP64 fit on the whole batch
+    xp = outs["plain"].x.cpu().numpy()
+    cp = outs["plain"].chi2.cpu().numpy()
+    acc = {k: errors(outs[k].x.double().cpu().numpy(), outs[k].chi2.double().cpu().numpy(), xp, cp)
+           for k in ("warm", "fast")}
+    # ... and vs the FP64 oracle on a sample; the same sample gives the flop count
+    orc = oracle_sample(ct, s.astype(np.float32).astype(np.float64), t, args.oracle_sample)
+    n_o = len(orc)
+    acc["warm_vs_oracle"] = errors(outs["warm"].x.double().cpu().numpy()[:, :n_o],
+                                   outs["warm"].chi2.double().cpu().numpy()[:n_o],
+                                   orc["x"].T, orc["chi2"])
+    af = float(orc["aflops"].mean())
+    it_w = outs["warm"].iters.cpu().numpy()
+    it_f = outs["fast"].iters.cpu().numpy()
+    mlp_eval = warmstart.evaluate(eng, s32, outs["plain"].x.float())
 
-    head = "fast" if fast_ok else "plain"
-    per = res[head]["per"]
-    ms = float(np.mean(per))
+    ms = res["warm"]["ms"]
     value = nch / (ms / 1e3)
-    af = algorithmic_flops(ct, s, t)
-    achieved = af * nch / (ms / 1e3) / 1e12
-    peak = PEAK_TFLOPS["fp32" if head == "fast" else "fp64"]
+    kern_ms = res["warm"]["fit_ms"]
+    achieved = af * nch / (kern_ms / 1e3) / 1e12
+    peak = PEAK_TFLOPS["f32"]
     line = {
         "metric": "fitted channels/sec (10 samples x 10 BX), warm-start FP32 vs plain FP64, same GPU",
         "value": value,
@@ -161,27 +196,38 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if head == "fast" else "f64",
+        "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{args.config}: {w.description}", "channels": nch,
                    "samples": 10, "pulses": 10, "pulse_cov": args.pulse_cov,
-                   "path": head},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                   "path": "warm-start fast path (MLP + FP32 active set + FP64 polish)"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": "k_fit_fast", "kernel_ms": kern_ms,
                      "aflops_per_channel": af},
+        "plain_fp64": {"channels_per_s": nch / (res["plain"]["ms"] / 1e3),
+                       "ms_per_step": res["plain"]["ms"]},
+        "fast_cold_fp32": {"channels_per_s": nch / (res["fast"]["ms"] / 1e3),
+                           "ms_per_step": res["fast"]["ms"]},
+        "speedup_warm_vs_plain_fp64": res["plain"]["ms"] / res["warm"]["ms"],
+        "speedup_cold_vs_plain_fp64": res["plain"]["ms"] / res["fast"]["ms"],
+        "mlp_ms_per_step": res["warm"]["mlp_ms"],
+        "accuracy": acc,
+        "solves_per_channel": {"warm": float(((it_w >> 16) & 0xFFFF).mean()),
+                               "cold": float(((it_f >> 16) & 0xFFFF).mean())},
+        "warm_start_rejected": float(((outs["warm"].flags.cpu().numpy() & 32) != 0).mean()),
+        "mlp": {"eval_on_batch": mlp_eval, **{k: v for k, v in mlp_info.items() if k != "report"}},
     }
-    if "plain" in res:
-        line["plain_fp64"] = {"channels_per_s": nch / (np.mean(res["plain"]["per"]) / 1e3),
-                              "ms_per_step": float(np.mean(res["plain"]["per"]))}
-    if "fast" in res and "plain" in res:
-        line["speedup_vs_plain_fp64"] = (np.mean(res["plain"]["per"]) / np.mean(res["fast"]["per"]))
+    if "report" in mlp_info:
+        line["mlp"]["train"] = mlp_info["report"]
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        rate, n, dt = cpu_baseline(ct, s, t, args.cpu_seconds, threads)
+        rate, n, done, dt = cpu_baseline(ct, s, t, args.cpu_seconds, threads)
         line["cpu_baseline"] = {"value": rate, "unit": "channels/s", "cores": threads,
                                 "kind": "port",
-                                "sample": f"first {n} channels of the same batch, {dt:.1f} s, "
-                                          f"FNNLS max_iter 1000, {threads} threads"}
+                                "sample": f"first {n} channels of the same batch fitted "
+                                          f"{done // n} times ({done} fits, {dt:.1f} s) by the "
+                                          f"oracle, FNNLS max_iter 1000, {threads} threads"}
     print(json.dumps(line), flush=True)
 
 

@@ -149,6 +149,18 @@ mf_status mf_default_opts(mf_opts *opts);
 mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_result *out,
                        const mf_opts *opts, void *stream);
 
+/* Opt-in timing of mf_fit_batch's kernels with HIP events recorded on the
+ * launch stream (no counterpart in the reference; used by the benchmark).
+ * mf_get_timings waits for the events of the last timed mf_fit_batch. */
+typedef struct mf_timings {
+    float mlp_ms;     /* warm-start inference kernel                       */
+    float fit_ms;     /* fit kernel(s), summed over the type partitions    */
+    int32_t fit_launches;
+    int32_t pad;
+} mf_timings;
+mf_status mf_set_timing(mf_context *ctx, int32_t enable);
+mf_status mf_get_timings(mf_context *ctx, mf_timings *out);
+
 /* Batched NNLS solvers, one problem per channel, FP64.
  * A: n matrices of m x m, column-major as the reference's FixedMatrix
  * (nnls/interface/data_types.h:25-26), A[c*100 + col*10 + row] (leading

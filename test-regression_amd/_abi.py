@@ -71,6 +71,11 @@ class GenSpec(C.Structure):
     ]
 
 
+class Timings(C.Structure):
+    _fields_ = [("mlp_ms", C.c_float), ("fit_ms", C.c_float), ("fit_launches", C.c_int32),
+                ("pad", C.c_int32)]
+
+
 class MFError(RuntimeError):
     pass
 
@@ -102,6 +107,8 @@ SIGNATURES = [
     ("mf_mlp_train_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
                                     C.c_int64, C.c_float, C.c_void_p, C.c_void_p]),
     ("mf_mlp_predict", C.c_int, [C.c_void_p, C.POINTER(Batch), C.c_void_p, C.c_void_p]),
+    ("mf_set_timing", C.c_int, [C.c_void_p, C.c_int32]),
+    ("mf_get_timings", C.c_int, [C.c_void_p, C.POINTER(Timings)]),
     ("mf_gen_type", C.c_int, [C.POINTER(GenSpec), C.POINTER(ChType)]),
     ("mf_gen_samples", C.c_int, [C.POINTER(GenSpec), C.c_int32, C.c_void_p, C.c_int64,
                                  C.c_uint64, C.c_void_p, C.c_void_p, C.c_int32]),

@@ -828,67 +828,47 @@ MF_HD double chisq64(const TypeMats<double> &td, const TL (&Li)[10][10], const d
 // First Minimize iteration of the fast fit.  The covariance is C0 for every
 // channel of the type at x = 0, so the normal equations are the type's AtA0
 // and Atb = G0 s (no per-channel covariance, Cholesky or whitening).
-//   phase 1 (FP32): active-set iterations from (x, pm) -- x = 0, pm = 0 for a
-//     cold start, a predicted passive set for a warm start -- until the FP32
-//     KKT test passes or a pivot of the passive block loses more than a
-//     fraction tau32 of its diagonal (F_FP64_PIVOT);
-//   phase 2 (FP64): the same iterations continue from that state on the FP64
-//     normal equations with the reference's threshold eps: the first solve
-//     refines the amplitudes to FP64, the KKT test verifies the active set
-//     (a warm start that needed more steps is flagged F_WARM_REJECT).
+//   phase 1 (FP32, fast_phase1): active-set iterations from (x, pm) -- x = 0,
+//     pm = 0 for a cold start, a predicted passive set for a warm start --
+//     until the FP32 KKT test passes or a pivot of the passive block loses
+//     more than a fraction tau32 of its diagonal (F_FP64_PIVOT);
+//   phase 2 (FP64, fast_phase2): the same iterations continue from that state
+//     on the FP64 normal equations with the reference's threshold eps: the
+//     first solve refines the amplitudes to FP64, the KKT test verifies the
+//     active set (a warm start that needed more steps is flagged
+//     F_WARM_REJECT).
 // The result is the NNLS optimum of the x = 0 problem, which is what the
-// reference's first (cold) FNNLS computes.  Returns chi2 (FP64); x64 holds
-// the amplitudes.
+// reference's first (cold) FNNLS computes.  The two phases are separate
+// functions so that a kernel can run them as separate launches: phase 1
+// needs about half the registers of phase 2.  Phase 2 takes the passive set
+// as the entries with x > 0.
 template <int NS, int NP>
-MF_HD double fast_first(const DevType &ty, const float (&s32)[10], const double (&s64)[10],
-                        const FitParams &prm, float (&x)[10], uint32_t &pm, bool warm,
-                        double (&x64)[10], FnnlsStats &st, int &flags)
+MF_HD void fast_phase1(const DevType &ty, const float (&s32)[10], const FitParams &prm,
+                       float (&x)[10], uint32_t &pm, FnnlsStats &st, int &flags)
 {
     const TypeMats<float> &tf = ty.f;
-    const TypeMats<double> &td = ty.d;
     float Atb[10];
-    double Atb64[10];
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
         float t = 0.0f;
-        double t64 = 0.0;
 #pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            t = mfma(tf.G0[p][k], s32[k], t);
-            t64 = fma(td.G0[p][k], s64[k], t64);
-        }
+        for (int k = 0; k < NS; ++k) t = mfma(tf.G0[p][k], s32[k], t);
         Atb[p] = t;
-        Atb64[p] = t64;
     }
     bool ill = false;
     fnnls_active<float, NP, true>(tf.AtA0, Atb, x, pm, (float)prm.eps, prm.eps_rel32,
                                   prm.max_iter, st, prm.tau32, &ill);
     if (ill) flags |= F_FP64_PIVOT;
-#pragma unroll
-    for (int i = 0; i < 10; ++i) x64[i] = ((pm >> i) & 1u) ? (double)x[i] : 0.0;
-    const int outer0 = st.outer;
-    fnnls_active<double, NP, true>(td.AtA0, Atb64, x64, pm, prm.eps, prm.eps_rel64,
-                                   prm.max_iter, st);
-    flags |= F_REFINED;
-    if (warm && st.outer != outer0) flags |= F_WARM_REJECT;
-    return chisq64<double, double, NS, NP>(td, td.Li0, s64, x64);
 }
 
-// The fast fit of one channel.  Iteration 1 is fast_first.  If the type's
-// pulse covariance is zero (or not used) the covariance never depends on x:
-// the reference's iteration 2 rebuilds the same normal equations, its FNNLS
-// starts at their optimum and stops, chi2 repeats and Minimize ends after 2
-// iterations with the same amplitudes -- so the fit ends here.  With a live
-// pulse covariance the later iterations run the reference's FP64 iteration
-// body (per-channel covariance, inplace_fnnls with warm x and reset passive
-// set); they are FP64 because the reference decides them on gradients near
-// its 1e-11 threshold (see minimize_from).
+// Phase 2 and the rest of the fit (see fit_fast).  x: phase 1's amplitudes
+// (passive set = entries > 0).  Fills out.
 template <int NS, int NP, bool LIVE, typename Park>
-MF_HD void fit_fast(const DevType &ty, const float (&s32)[10], const FitParams &prm,
-                    float (&x)[10], uint32_t pm, bool warm, Park &park, FitOut<float> &out)
+MF_HD void fast_phase2(const DevType &ty, const float (&s32)[10], const FitParams &prm,
+                       const float (&x)[10], bool warm, FnnlsStats &st, int flags, Park &park,
+                       FitOut<float> &out)
 {
-    int flags = 0;
-    FnnlsStats st;
+    const TypeMats<double> &td = ty.d;
     double s64[10], x64[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) s64[i] = (double)s32[i];
@@ -903,7 +883,34 @@ MF_HD void fit_fast(const DevType &ty, const float (&s32)[10], const FitParams &
 #pragma unroll
         for (int i = 0; i < 10; ++i) x64[i] = 0.0;
     } else {
-        chi2 = fast_first<NS, NP>(ty, s32, s64, prm, x, pm, warm, x64, st, flags);
+        double Atb64[10];
+        uint32_t pm = 0;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            double t64 = 0.0;
+#pragma unroll
+            for (int k = 0; k < NS; ++k) t64 = fma(td.G0[p][k], s64[k], t64);
+            Atb64[p] = t64;
+            x64[p] = x[p] > 0.0f ? (double)x[p] : 0.0;
+            if (x[p] > 0.0f) pm |= 1u << p;
+        }
+#pragma unroll
+        for (int p = NP; p < 10; ++p) x64[p] = 0.0;
+        const int outer0 = st.outer;
+        fnnls_active<double, NP, true>(td.AtA0, Atb64, x64, pm, prm.eps, prm.eps_rel64,
+                                       prm.max_iter, st);
+        flags |= F_REFINED;
+        if (warm && st.outer != outer0) flags |= F_WARM_REJECT;
+        chi2 = chisq64<double, double, NS, NP>(*opaque(&td), td.Li0, s64, x64);
+        // If the type's pulse covariance is zero (or not used) the covariance
+        // never depends on x: the reference's iteration 2 rebuilds the same
+        // normal equations, its FNNLS starts at their optimum and stops, chi2
+        // repeats and Minimize ends after 2 iterations with the same
+        // amplitudes -- so the fit ends here.  With a live pulse covariance
+        // the later iterations run the reference's FP64 iteration body
+        // (per-channel covariance, inplace_fnnls with warm x and reset passive
+        // set); they are FP64 because the reference decides them on gradients
+        // near its 1e-11 threshold.
         nmin = 1;
         if (!(fabs(chi2) < prm.chi2_tol)) {
             if (LIVE && prm.use_pi)
@@ -926,6 +933,18 @@ MF_HD void fit_fast(const DevType &ty, const float (&s32)[10], const FitParams &
     out.n_min = nmin;
     out.n_outer = st.outer;
     out.n_inner = st.inner;
+}
+
+// The fast fit of one channel, both phases in one call.
+template <int NS, int NP, bool LIVE, typename Park>
+MF_HD void fit_fast(const DevType &ty, const float (&s32)[10], const FitParams &prm,
+                    float (&x)[10], uint32_t pm, bool warm, Park &park, FitOut<float> &out)
+{
+    int flags = 0;
+    FnnlsStats st;
+    if (all_finite<float, NS>(s32) && prm.max_minimize > 0)
+        fast_phase1<NS, NP>(ty, s32, prm, x, pm, st, flags);
+    fast_phase2<NS, NP, LIVE>(ty, s32, prm, x, warm, st, flags, park, out);
 }
 
 // Register-array park for host builds and for kernels that keep L^{-1} live.

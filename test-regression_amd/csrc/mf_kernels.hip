@@ -89,19 +89,21 @@ void k_fit_plain(const DevType *__restrict__ types, int type_id, const int32_t *
     if (ito) ito[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
 }
 
-// Fast path (MF_PREC_FP32): FP32 bulk with FP64 polish, see fit_fast.
-// LIVE: the type's pulse covariance is live, later iterations run in FP64
-// with L^{-1} parked in LDS.
+// Fast path (MF_PREC_FP32), two launches (see fast_phase1 / fast_phase2):
+// k_fast_p1 runs the FP32 active-set exploration and leaves its amplitudes in
+// the output x (FP32), its flags and counters in flags / iters; k_fast_p2
+// reads them back, finishes the fit in FP64 and writes the final outputs.
+// Phase 1 needs about half the registers of phase 2, so splitting them lets
+// the long FP32 phase run at twice the occupancy.
 // warm: null, or the MLP's prediction [20][nch] (logits, amplitudes): the
-// passive set {i: logit_i > 0} with its predicted amplitudes starts the solve.
-template <typename TS, int N, bool LIVE>
+// passive set {i: logit_i > 0} with its predicted amplitudes starts phase 1.
+template <typename TS, int N>
 __global__ __launch_bounds__(256)
-void k_fit_fast(const DevType *__restrict__ types, int type_id, const int32_t *__restrict__ idx,
-                int64_t nitems, const TS *__restrict__ samples, int64_t nch,
-                float *__restrict__ xo, float *__restrict__ chi2o, int32_t *__restrict__ flo,
-                int32_t *__restrict__ ito, const float *__restrict__ warm, FitParams prm)
+void k_fast_p1(const DevType *__restrict__ types, int type_id, const int32_t *__restrict__ idx,
+               int64_t nitems, const TS *__restrict__ samples, int64_t nch,
+               float *__restrict__ xo, int32_t *__restrict__ flo, int32_t *__restrict__ ito,
+               const float *__restrict__ warm, FitParams prm)
 {
-    extern __shared__ unsigned char lds_raw[];
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nitems) return;
     const int64_t c = idx ? (int64_t)idx[g] : g;
@@ -116,14 +118,48 @@ void k_fit_fast(const DevType *__restrict__ types, int type_id, const int32_t *_
             x[i] = fmaxf(warm[(int64_t)(10 + i) * nch + c], 1e-6f);
         }
     }
+    int flags = 0;
+    FnnlsStats st;
+    if (all_finite<float, N>(s) && prm.max_minimize > 0)
+        fast_phase1<N, N>(types[type_id], s, prm, x, pm, st, flags);
+#pragma unroll
+    for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = ((pm >> p) & 1u) ? x[p] : 0.0f;
+    flo[c] = flags | (st.hit_max ? F_FNNLS_CAP : 0) | (st.llt_fail ? F_LLT_FAIL : 0);
+    ito[c] = (st.outer << 8) | (st.inner << 16);
+}
+
+template <typename TS, int N, bool LIVE>
+__global__ __launch_bounds__(256)
+void k_fast_p2(const DevType *__restrict__ types, int type_id, const int32_t *__restrict__ idx,
+               int64_t nitems, const TS *__restrict__ samples, int64_t nch,
+               float *__restrict__ xo, float *__restrict__ chi2o, int32_t *__restrict__ flo,
+               int32_t *__restrict__ ito, int warm, FitParams prm)
+{
+    extern __shared__ unsigned char lds_raw[];
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nitems) return;
+    const int64_t c = idx ? (int64_t)idx[g] : g;
+    float s[10], x[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        s[i] = i < N ? (float)samples[(int64_t)i * nch + c] : 0.0f;
+        x[i] = i < N ? xo[(int64_t)i * nch + c] : 0.0f;
+    }
+    const int f1 = flo[c], it1 = ito[c];
+    FnnlsStats st;
+    st.outer = (it1 >> 8) & 255;
+    st.inner = (it1 >> 16) & 0xFFFF;
+    st.hit_max = (f1 & F_FNNLS_CAP) != 0;
+    st.llt_fail = (f1 & F_LLT_FAIL) != 0;
     LdsPark<double> park{reinterpret_cast<double *>(lds_raw) + threadIdx.x, (int)blockDim.x};
     FitOut<float> out;
-    fit_fast<N, N, LIVE>(types[type_id], s, prm, x, pm, warm != nullptr, park, out);
+    fast_phase2<N, N, LIVE>(types[type_id], s, prm, x, warm != 0, st, f1 & F_FP64_PIVOT, park,
+                            out);
 #pragma unroll
     for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = out.x[p];
     chi2o[c] = out.chi2;
     flo[c] = out.flags;
-    if (ito) ito[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
+    ito[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
 }
 
 // Partition of a mixed batch by channel type: counts, offsets, index lists.
@@ -324,13 +360,23 @@ struct mf_context {
     // gradient, Adam moments; activations of the largest batch seen
     float *mlp_p = nullptr, *mlp_pq = nullptr, *mlp_g = nullptr, *mlp_m = nullptr,
           *mlp_v = nullptr;
-    uint16_t *mlp_bf = nullptr;
+    uint16_t *mlp_bf = nullptr, *mlp_frag = nullptr;
     int mlp_t = 0;
     bool mlp_ready = false;
     mf::mlp::Scratch mlp_w;
     float *pred = nullptr;  // [20][pred_cap] prediction for warm starts
     int64_t pred_cap = 0;
+    // opt-in kernel timing: events around the MLP and the fit launches
+    bool timing = false;
+    hipEvent_t ev[2 + 2 * 256] = {};
+    int n_fit_ev = 0;
+    bool mlp_timed = false;
 };
+
+static void time_mark(mf_context *ctx, int slot, hipStream_t st)
+{
+    if (ctx->timing) (void)hipEventRecord(ctx->ev[slot], st);
+}
 
 extern "C" mf_status mf_default_opts(mf_opts *o)
 {
@@ -366,7 +412,9 @@ extern "C" mf_status mf_context_destroy(mf_context *ctx)
     if (ctx->d_types) (void)hipFree(ctx->d_types);
     if (ctx->d_part) (void)hipFree(ctx->d_part);
     if (ctx->d_idx) (void)hipFree(ctx->d_idx);
-    for (void *p : {(void *)ctx->mlp_p, (void *)ctx->mlp_pq, (void *)ctx->mlp_g,
+    for (hipEvent_t &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (void *p : {(void *)ctx->mlp_frag, (void *)ctx->mlp_p, (void *)ctx->mlp_pq, (void *)ctx->mlp_g,
                     (void *)ctx->mlp_m, (void *)ctx->mlp_v, (void *)ctx->mlp_bf,
                     (void *)ctx->mlp_w.X, (void *)ctx->mlp_w.m, (void *)ctx->mlp_w.Z1,
                     (void *)ctx->mlp_w.H1, (void *)ctx->mlp_w.Z2, (void *)ctx->mlp_w.H2,
@@ -428,12 +476,17 @@ static mf_status launch_fast(mf_context *ctx, int type_id, const int32_t *idx, i
                              const FitParams &prm, int block, const float *warm, hipStream_t st)
 {
     if (nitems <= 0) return MF_OK;
+    if (!out->iters) return fail(MF_ERR_INVALID, "the FP32 path needs mf_result.iters");
     const int64_t grid = (nitems + block - 1) / block;
+    hipLaunchKernelGGL((k_fast_p1<TS, N>), dim3((unsigned)grid), dim3(block), 0, st,
+                       ctx->d_types, type_id, idx, nitems, static_cast<const TS *>(samples), nch,
+                       static_cast<float *>(out->x), out->flags, out->iters, warm, prm);
+    HIP_TRY(hipGetLastError());
     const size_t lds = LIVE ? sizeof(double) * 55 * (size_t)block : 0;
-    hipLaunchKernelGGL((k_fit_fast<TS, N, LIVE>), dim3((unsigned)grid), dim3(block), lds, st,
+    hipLaunchKernelGGL((k_fast_p2<TS, N, LIVE>), dim3((unsigned)grid), dim3(block), lds, st,
                        ctx->d_types, type_id, idx, nitems, static_cast<const TS *>(samples), nch,
                        static_cast<float *>(out->x), static_cast<float *>(out->chi2), out->flags,
-                       out->iters, warm, prm);
+                       out->iters, warm ? 1 : 0, prm);
     HIP_TRY(hipGetLastError());
     return MF_OK;
 }
@@ -529,22 +582,35 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
             ctx->pred_cap = in->nch;
         }
         const int ns = ctx->types[0].nsamples > 8 ? 10 : 8;
+        time_mark(ctx, 0, st);
+        ctx->mlp_timed = ctx->timing;
         if (in->sample_dtype == MF_DTYPE_F64)
-            mf::mlp::launch_infer<double>(in->samples, in->nch, ns, nullptr, in->nch, ctx->mlp_bf,
-                                          ctx->mlp_pq, ctx->pred, in->nch, st);
+            mf::mlp::launch_infer<double>(in->samples, in->nch, ns, nullptr, in->nch,
+                                          ctx->mlp_frag, ctx->mlp_pq, ctx->pred, in->nch, st);
         else
-            mf::mlp::launch_infer<float>(in->samples, in->nch, ns, nullptr, in->nch, ctx->mlp_bf,
-                                         ctx->mlp_pq, ctx->pred, in->nch, st);
+            mf::mlp::launch_infer<float>(in->samples, in->nch, ns, nullptr, in->nch,
+                                         ctx->mlp_frag, ctx->mlp_pq, ctx->pred, in->nch, st);
         HIP_TRY(hipGetLastError());
+        time_mark(ctx, 1, st);
         warm = ctx->pred;
+    } else {
+        ctx->mlp_timed = false;
     }
-    auto go = [&](int type_id, const int32_t *idx, const int64_t *count, int64_t nitems) {
+    auto go_fit = [&](int type_id, const int32_t *idx, const int64_t *count, int64_t nitems) {
         if (opts->precision == MF_PREC_FP64)
             return dispatch_fit<double>(ctx, type_id, idx, count, nitems, in, out, prm, block,
                                         opts->compute_errors != 0, st);
         return in->sample_dtype == MF_DTYPE_F64
                    ? dispatch_fast<double>(ctx, type_id, idx, nitems, in, out, prm, block, warm, st)
                    : dispatch_fast<float>(ctx, type_id, idx, nitems, in, out, prm, block, warm, st);
+    };
+    ctx->n_fit_ev = 0;
+    auto go = [&](int type_id, const int32_t *idx, const int64_t *count, int64_t nitems) {
+        const int k = ctx->n_fit_ev < 256 ? ctx->n_fit_ev++ : 255;
+        time_mark(ctx, 2 + 2 * k, st);
+        const mf_status r = go_fit(type_id, idx, count, nitems);
+        time_mark(ctx, 3 + 2 * k, st);
+        return r;
     };
     if (!in->type_of || ntypes == 1) return go(0, nullptr, nullptr, in->nch);
     // mixed batch: partition channel indices by type on the device
@@ -602,6 +668,7 @@ static mf_status mlp_alloc(mf_context *ctx)
     HIP_TRY(hipMalloc(&ctx->mlp_m, n));
     HIP_TRY(hipMalloc(&ctx->mlp_v, n));
     HIP_TRY(hipMalloc(&ctx->mlp_bf, sizeof(uint16_t) * mf::mlp::NPARAM));
+    HIP_TRY(hipMalloc(&ctx->mlp_frag, sizeof(uint16_t) * mf::mlp::FRAG_ELEMS));
     return MF_OK;
 }
 
@@ -614,6 +681,7 @@ static mf_status mlp_upload(mf_context *ctx, const float *params)
     HIP_TRY(hipMemset(ctx->mlp_m, 0, n));
     HIP_TRY(hipMemset(ctx->mlp_v, 0, n));
     HIP_TRY(mf::mlp::quantize(ctx->mlp_p, ctx->mlp_pq, ctx->mlp_bf, nullptr));
+    HIP_TRY(mf::mlp::pack(ctx->mlp_bf, ctx->mlp_frag, nullptr));
     HIP_TRY(hipDeviceSynchronize());
     ctx->mlp_t = 0;
     ctx->mlp_ready = true;
@@ -707,6 +775,7 @@ extern "C" mf_status mf_mlp_train_step(mf_context *ctx, const float *samples, in
     ctx->mlp_t += 1;
     HIP_TRY(mf::mlp::adam_step(ctx->mlp_p, ctx->mlp_g, ctx->mlp_m, ctx->mlp_v, ctx->mlp_t, lr,
                                ctx->mlp_pq, ctx->mlp_bf, static_cast<hipStream_t>(stream)));
+    HIP_TRY(mf::mlp::pack(ctx->mlp_bf, ctx->mlp_frag, static_cast<hipStream_t>(stream)));
     return MF_OK;
 }
 
@@ -720,12 +789,42 @@ extern "C" mf_status mf_mlp_predict(mf_context *ctx, const mf_batch *in, float *
     HIP_TRY(hipSetDevice(ctx->device));
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (in->sample_dtype == MF_DTYPE_F64)
-        mf::mlp::launch_infer<double>(in->samples, in->nch, 10, nullptr, in->nch, ctx->mlp_bf,
+        mf::mlp::launch_infer<double>(in->samples, in->nch, 10, nullptr, in->nch, ctx->mlp_frag,
                                       ctx->mlp_pq, out, in->nch, st);
     else
-        mf::mlp::launch_infer<float>(in->samples, in->nch, 10, nullptr, in->nch, ctx->mlp_bf,
+        mf::mlp::launch_infer<float>(in->samples, in->nch, 10, nullptr, in->nch, ctx->mlp_frag,
                                      ctx->mlp_pq, out, in->nch, st);
     HIP_TRY(hipGetLastError());
+    return MF_OK;
+}
+
+extern "C" mf_status mf_set_timing(mf_context *ctx, int32_t enable)
+{
+    if (!ctx) return fail(MF_ERR_INVALID, "ctx is NULL");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (enable && !ctx->ev[0])
+        for (hipEvent_t &e : ctx->ev) HIP_TRY(hipEventCreate(&e));
+    ctx->timing = enable != 0;
+    return MF_OK;
+}
+
+extern "C" mf_status mf_get_timings(mf_context *ctx, mf_timings *out)
+{
+    if (!ctx || !out) return fail(MF_ERR_INVALID, "NULL argument");
+    memset(out, 0, sizeof(*out));
+    if (!ctx->timing) return fail(MF_ERR_INVALID, "timing is not enabled (mf_set_timing)");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (ctx->mlp_timed) {
+        HIP_TRY(hipEventSynchronize(ctx->ev[1]));
+        HIP_TRY(hipEventElapsedTime(&out->mlp_ms, ctx->ev[0], ctx->ev[1]));
+    }
+    for (int k = 0; k < ctx->n_fit_ev; ++k) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventSynchronize(ctx->ev[3 + 2 * k]));
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[2 + 2 * k], ctx->ev[3 + 2 * k]));
+        out->fit_ms += ms;
+    }
+    out->fit_launches = ctx->n_fit_ev;
     return MF_OK;
 }
 

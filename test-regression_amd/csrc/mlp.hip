@@ -28,23 +28,29 @@
 namespace mf {
 namespace mlp {
 
-__device__ __forceinline__ float gelu(float x)
+// GELU, tanh form: 0.5 x (1 + tanh(u)), u = sqrt(2/pi) (x + 0.044715 x^3).
+// Since (1 + tanh(u)) / 2 = sigmoid(2u) this is x / (1 + exp(-2u)), one
+// hardware exp and one hardware reciprocal.
+__device__ __forceinline__ float sig2u(float x)
 {
-    const float c = 0.7978845608028654f;  // sqrt(2/pi)
-    const float u = c * (x + 0.044715f * x * x * x);
-    return 0.5f * x * (1.0f + tanhf(u));
+    const float c2 = 1.5957691216057308f;  // 2 sqrt(2/pi)
+    const float u2 = c2 * fmaf(0.044715f * x, x * x, x);
+    return __builtin_amdgcn_rcpf(1.0f + __expf(-u2));
 }
 
+__device__ __forceinline__ float gelu(float x) { return x * sig2u(x); }
+
+// d/dx [x sigma(2u)] = sigma + x sigma (1 - sigma) 2u'(x)
 __device__ __forceinline__ float gelu_grad(float x)
 {
-    const float c = 0.7978845608028654f;
-    const float t = tanhf(c * (x + 0.044715f * x * x * x));
-    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * x * x);
+    const float c2 = 1.5957691216057308f;
+    const float sg = sig2u(x);
+    return sg + x * sg * (1.0f - sg) * c2 * fmaf(3.0f * 0.044715f, x * x, 1.0f);
 }
 
 __device__ __forceinline__ float softplusf(float x)
 {
-    return fmaxf(x, 0.0f) + log1pf(expf(-fabsf(x)));
+    return fmaxf(x, 0.0f) + __logf(1.0f + __expf(-fabsf(x)));
 }
 
 // bfloat16 round to nearest even (NaN not expected: weights are finite)
@@ -246,27 +252,46 @@ __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d)
     return r;
 }
 
-// A-operand fragment of W^T (rows = outputs o, cols = inputs k) for the tile
-// (mt, kt): lane l holds W[kt*16 + 4*(l/16) + j][mt*16 + l%16], j = 0..3,
-// zero outside [nin x nout].  W: bfloat16 row-major [nin][nout].
-__device__ __forceinline__ bf16x4 wfrag(const uint16_t *__restrict__ W, int nin, int nout, int mt,
-                                        int kt, int lane)
+// A-operand fragments of W^T (rows = outputs o, cols = inputs k): for the tile
+// (mt, kt) lane l holds W[kt*16 + 4*(l/16) + j][mt*16 + l%16], j = 0..3, zero
+// outside [nin x nout].  They are packed once per parameter update
+// (k_mlp_pack) in the order the inference kernel reads them: fragment f of
+// lane l at frag[(f*64 + l)*4 .. +3], so a wavefront loads each fragment with
+// one contiguous 512-byte read.  Fragments: 4 of W1 (mt, kt 0), 16 of W2
+// (mt*4 + kt), 8 of W3 (mt*4 + kt).
+constexpr int NFRAG = 28;
+
+__global__ void k_mlp_pack(const uint16_t *__restrict__ pbf, uint16_t *__restrict__ frag)
 {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= NFRAG * 64) return;
+    const int f = t / 64, lane = t % 64;
+    const uint16_t *W;
+    int nin, nout, mt, kt;
+    if (f < 4) {
+        W = pbf; nin = 10; nout = 64; mt = f; kt = 0;
+    } else if (f < 20) {
+        W = pbf + 704; nin = 64; nout = 64; mt = (f - 4) / 4; kt = (f - 4) % 4;
+    } else {
+        W = pbf + 704 + 4160; nin = 64; nout = 20; mt = (f - 20) / 4; kt = (f - 20) % 4;
+    }
     const int o = mt * 16 + (lane & 15);
-    bf16x4 r;
-#pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int k = kt * 16 + 4 * (lane >> 4) + j;
-        r[j] = (k < nin && o < nout) ? (short)W[k * nout + o] : (short)0;
+        frag[t * 4 + j] = (k < nin && o < nout) ? W[k * nout + o] : (uint16_t)0;
     }
-    return r;
+}
+
+__device__ __forceinline__ bf16x4 wfrag(const uint16_t *__restrict__ frag, int f, int lane)
+{
+    return *reinterpret_cast<const bf16x4 *>(frag + ((size_t)f * 64 + lane) * 4);
 }
 
 // out: [20][ldo] float: logits 0..9, then amplitudes (softplus * m) 10..19.
 template <typename TS>
 __global__ __launch_bounds__(256) void k_mlp_infer(const TS *__restrict__ s, int64_t ld, int ns,
                                                    const int32_t *__restrict__ idx, int64_t n,
-                                                   const uint16_t *__restrict__ pbf,
+                                                   const uint16_t *__restrict__ pbf /* packed */,
                                                    const float *__restrict__ pq,
                                                    float *__restrict__ out, int64_t ldo)
 {
@@ -274,19 +299,19 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const TS *__restrict__ s, int
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t base = wave * 64;
     if (base >= n) return;  // whole wave exits together
-    const uint16_t *W1 = pbf, *W2 = pbf + 704, *W3 = pbf + 704 + 4160;
+    const uint16_t *frag = pbf;  // packed fragments (k_mlp_pack)
     const float *b1 = pq + 640, *b2 = pq + 704 + 4096, *b3 = pq + 704 + 4160 + 1280;
     bf16x4 a1[4], a2[4][4], a3[2][4];
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) a1[mt] = wfrag(W1, 10, 64, mt, 0, lane);
+    for (int mt = 0; mt < 4; ++mt) a1[mt] = wfrag(frag, mt, lane);
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-        for (int kt = 0; kt < 4; ++kt) a2[mt][kt] = wfrag(W2, 64, 64, mt, kt, lane);
+        for (int kt = 0; kt < 4; ++kt) a2[mt][kt] = wfrag(frag, 4 + mt * 4 + kt, lane);
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-        for (int kt = 0; kt < 4; ++kt) a3[mt][kt] = wfrag(W3, 64, 20, mt, kt, lane);
+        for (int kt = 0; kt < 4; ++kt) a3[mt][kt] = wfrag(frag, 20 + mt * 4 + kt, lane);
     const int r4 = 4 * (lane >> 4);
 #pragma unroll 1
     for (int nt = 0; nt < 4; ++nt) {
@@ -435,6 +460,12 @@ hipError_t adam_step(float *p, const float *g, float *m, float *v, int t, float 
     const float c1 = 1.0f - powf(0.9f, (float)t), c2 = 1.0f - powf(0.999f, (float)t);
     hipLaunchKernelGGL(k_adam, dim3(nblk(NPARAM, 256)), dim3(256), 0, st, p, g, m, v, NPARAM, lr,
                        c1, c2, pq, pbf);
+    return hipGetLastError();
+}
+
+hipError_t pack(const uint16_t *pbf, uint16_t *frag, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mlp_pack, dim3(nblk(NFRAG * 64, 256)), dim3(256), 0, st, pbf, frag);
     return hipGetLastError();
 }
 

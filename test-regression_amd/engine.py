@@ -112,6 +112,73 @@ class Engine:
         return out
 
 
+    # ---- kernel timing (HIP events on the launch stream) ----
+    def set_timing(self, enable: bool = True):
+        _abi.check(self._lib.mf_set_timing(self._ctx, 1 if enable else 0), "mf_set_timing")
+
+    def timings(self):
+        t = _abi.Timings()
+        _abi.check(self._lib.mf_get_timings(self._ctx, C.byref(t)), "mf_get_timings")
+        return {"mlp_ms": t.mlp_ms, "fit_ms": t.fit_ms, "fit_launches": t.fit_launches}
+
+    # ---- learned warm start (mf_mlp_*) ----
+    def mlp_init(self, seed: int = 0):
+        _abi.check(self._lib.mf_mlp_init(self._ctx, seed), "mf_mlp_init")
+
+    def mlp_set_params(self, params):
+        import numpy as np
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != _abi.MF_MLP_NPARAM:
+            raise ValueError(f"expected {_abi.MF_MLP_NPARAM} parameters")
+        _abi.check(self._lib.mf_mlp_set_params(self._ctx, p.ctypes.data, p.size),
+                   "mf_mlp_set_params")
+
+    def mlp_get_params(self):
+        import numpy as np
+        p = np.empty(_abi.MF_MLP_NPARAM, dtype=np.float32)
+        _abi.check(self._lib.mf_mlp_get_params(self._ctx, p.ctypes.data, p.size),
+                   "mf_mlp_get_params")
+        return p
+
+    @staticmethod
+    def _f32_2d(t, name):
+        if not (t.is_cuda and t.dtype == torch.float32 and t.dim() == 2 and t.shape[0] == 10
+                and t.stride(1) == 1):
+            raise ValueError(f"{name} must be a float32 device tensor [10, B] with unit column stride")
+
+    def mlp_grad(self, samples, xref, stream=None):
+        """(gradient [6164], loss) of the mean loss over the batch's columns."""
+        self._f32_2d(samples, "samples")
+        self._f32_2d(xref, "xref")
+        B = samples.shape[1]
+        g = torch.empty(_abi.MF_MLP_NPARAM, dtype=torch.float32, device=samples.device)
+        loss = torch.empty(1, dtype=torch.float32, device=samples.device)
+        _abi.check(self._lib.mf_mlp_grad(self._ctx, samples.data_ptr(), samples.stride(0),
+                                         xref.data_ptr(), xref.stride(0), B, g.data_ptr(),
+                                         loss.data_ptr(), C.c_void_p(_stream_ptr(stream))),
+                   "mf_mlp_grad")
+        return g, loss
+
+    def mlp_train_step(self, samples, xref, lr: float, loss=None, stream=None):
+        self._f32_2d(samples, "samples")
+        self._f32_2d(xref, "xref")
+        _abi.check(self._lib.mf_mlp_train_step(
+            self._ctx, samples.data_ptr(), samples.stride(0), xref.data_ptr(), xref.stride(0),
+            samples.shape[1], lr, loss.data_ptr() if loss is not None else None,
+            C.c_void_p(_stream_ptr(stream))), "mf_mlp_train_step")
+
+    def mlp_predict(self, samples, stream=None):
+        """[20, nch] float32: logits (passive if > 0), then amplitudes."""
+        samples = samples.contiguous()
+        nch = samples.shape[1]
+        out = torch.empty((20, nch), dtype=torch.float32, device=samples.device)
+        sdt = _abi.MF_DTYPE_F64 if samples.dtype == torch.float64 else _abi.MF_DTYPE_F32
+        b = _abi.Batch(nch=nch, sample_dtype=sdt, pad=0, samples=samples.data_ptr(), type_of=None)
+        _abi.check(self._lib.mf_mlp_predict(self._ctx, C.byref(b), out.data_ptr(),
+                                            C.c_void_p(_stream_ptr(stream))), "mf_mlp_predict")
+        return out
+
+
 def nnls_batch(A: torch.Tensor, b: torch.Tensor, x: torch.Tensor, m: int = 10,
                eps: float = 1e-11, max_iter: int = 1000, solver: str = "inplace_fnnls",
                stream=None) -> torch.Tensor:

@@ -1,0 +1,98 @@
+"""Training of the warm-start MLP on the GPU with the library's own kernels.
+
+Labels come from the plain method itself: the plain cold-start FP64 fit
+(mf_fit_batch, MF_PREC_FP64) of the training channels gives each channel's
+passive set and amplitudes.  Training is Adam (mf_mlp_train_step) over
+shuffled mini-batches; every step runs the forward, backward and update
+kernels of mlp.hip.  The trained parameters live in the engine's context and
+can be saved with numpy.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import _abi
+
+WEIGHTS = Path(__file__).resolve().parent / "weights" / "mlp_c2.npy"
+
+
+@dataclass
+class TrainReport:
+    n_train: int
+    steps: int
+    seconds: float
+    final_loss: float
+    label_seconds: float
+
+
+def label(eng, samples64: torch.Tensor, type_of=None) -> torch.Tensor:
+    """Plain FP64 fit of the channels -> amplitudes [10, n] float32."""
+    r = eng.fit(samples64, type_of, _abi.default_opts(precision=_abi.MF_PREC_FP64))
+    return r.x.float()
+
+
+def train(eng, samples32: torch.Tensor, xref: torch.Tensor, epochs: int = 3, batch: int = 8192,
+          lr: float = 2e-3, seed: int = 0, init: bool = True, log=None) -> TrainReport:
+    """samples32, xref: [10, n] float32 device tensors."""
+    n = samples32.shape[1]
+    if init:
+        eng.mlp_init(seed)
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    loss = torch.zeros(1, dtype=torch.float32, device=samples32.device)
+    steps = 0
+    total = epochs * (n // batch)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for ep in range(epochs):
+        perm = torch.randperm(n, generator=g).to(samples32.device)
+        xs = samples32[:, perm].contiguous()
+        xr = xref[:, perm].contiguous()
+        for b0 in range(0, n - batch + 1, batch):
+            # cosine decay to 5% of lr
+            frac = steps / max(1, total)
+            lr_t = lr * (0.05 + 0.95 * 0.5 * (1 + np.cos(np.pi * frac)))
+            eng.mlp_train_step(xs[:, b0:b0 + batch], xr[:, b0:b0 + batch], float(lr_t), loss)
+            steps += 1
+        if log:
+            torch.cuda.synchronize()
+            log(f"epoch {ep}: loss {loss.item():.5f} ({time.perf_counter() - t0:.1f}s)")
+    torch.cuda.synchronize()
+    return TrainReport(n, steps, time.perf_counter() - t0, float(loss.item()), 0.0)
+
+
+def evaluate(eng, samples32: torch.Tensor, xref: torch.Tensor) -> dict:
+    """Active-set accuracy of the prediction on held-out channels."""
+    pred = eng.mlp_predict(samples32)
+    pm = pred[:10] > 0
+    truth = xref > 0
+    exact = (pm == truth).all(dim=0).float().mean().item()
+    bit = (pm == truth).float().mean().item()
+    return {"exact_active_set": exact, "bit_accuracy": bit}
+
+
+def ensure_trained(eng, workload_cfg: str = "c4", n_train: int = 4_000_000, epochs: int = 3,
+                   path: Path | None = WEIGHTS, log=None) -> dict:
+    """Loads saved parameters, or trains on n_train channels of the c2
+    distribution (labelled by the plain FP64 fit) and saves them."""
+    from . import workloads
+    if path is not None and Path(path).exists():
+        eng.mlp_set_params(np.load(path))
+        return {"source": str(path)}
+    w = workloads.config(workload_cfg)
+    s, t, _ = w.generate(n_train)
+    s64 = torch.from_numpy(s).cuda()
+    t0 = time.perf_counter()
+    xref = label(eng, s64)
+    torch.cuda.synchronize()
+    lab = time.perf_counter() - t0
+    rep = train(eng, s64.float(), xref, epochs=epochs, log=log)
+    rep.label_seconds = lab
+    if path is not None:
+        Path(path).parent.mkdir(parents=True, exist_ok=True)
+        np.save(path, eng.mlp_get_params())
+    return {"source": "trained", "report": rep.__dict__}

@@ -1,18 +1,38 @@
 """GPU parity of the batched solver and linalgebra entry points.
 
 mf_inplace_fnnls_batch / mf_fnnls_batch against the oracle's inplace_fnnls /
-fnnls and against scipy's Lawson-Hanson NNLS; mf_linalg_batch against the
+fnnls, against exact brute force (m <= 7) and by the KKT conditions (a proof
+of optimality for this convex problem; scipy 1.15's nnls is not used as a
+judge: it returns non-optimal points on some of these inputs); mf_linalg_batch against the
 oracle's restatement of linalgebra/interface/*.hpp (same operation order:
 bit-identical in FP64 up to fma contraction, checked to 1e-13 relative).
 """
 import numpy as np
 import pytest
 import torch
-from scipy.optimize import nnls as scipy_nnls
+import itertools
 
 from oracle_binding import OrStats, dptr, oracle
 
 pytestmark = pytest.mark.gpu
+
+
+def _brute_nnls(A, b):
+    """Exact NNLS optimum: the best feasible least-squares fit over all supports."""
+    n = A.shape[1]
+    best, bx = np.inf, np.zeros(n)
+    for k in range(n + 1):
+        for S in itertools.combinations(range(n), k):
+            x = np.zeros(n)
+            if S:
+                xs, *_ = np.linalg.lstsq(A[:, S], b, rcond=None)
+                if np.any(xs < 0):
+                    continue
+                x[list(S)] = xs
+            r = np.sum((A @ x - b) ** 2)
+            if r < best - 1e-12:
+                best, bx = r, x
+    return bx
 
 
 def _problems(n, m, seed, kind="random"):
@@ -54,15 +74,17 @@ def test_nnls_batch_matches_oracle_and_scipy(pkg, gpu, solver, m):
         scale = max(np.abs(xr).max(), 1.0)
         worst = max(worst, np.abs(xg[c, :m] - xr[:m]).max() / scale)
         if c < 300:
-            Am = A[c, :m, :m].T
-            # the solvers work on A^T A: their error grows like cond(A)^2 eps,
-            # scipy's QR-based NNLS like cond(A) eps
-            if np.linalg.cond(Am) > 1e3:
-                continue
-            xs, _ = scipy_nnls(Am, b[c, :m])
-            if solver == "fnnls" and np.count_nonzero(xg[c, :m] == 0) == 1:
+            Am, bm, x = A[c, :m, :m].T, b[c, :m], xg[c, :m]
+            if solver == "fnnls" and np.count_nonzero(x == 0) == 1:
                 continue  # fnnls.cu:113 never tests the last free index (see test_oracle)
-            assert np.allclose(xg[c, :m], xs, rtol=1e-7, atol=1e-7 * max(1, np.abs(xs).max()))
+            # optimality proof: KKT of min |Am x - bm|^2, x >= 0 (convex)
+            wv = Am.T @ (bm - Am @ x)
+            tol = 1e-8 * (np.abs(Am.T @ bm).max() + 1) * np.linalg.cond(Am) ** 2
+            assert np.all(x >= 0)
+            assert np.all(wv[x == 0] <= tol) and np.all(np.abs(wv[x > 0]) <= tol), (c, x, wv)
+            if m <= 7:
+                xb = _brute_nnls(Am, bm)
+                assert np.allclose(x, xb, rtol=1e-7, atol=1e-7 * max(1, np.abs(xb).max()))
     assert worst <= 1e-10
     assert np.all(xg[:, m:] == 0)
 

@@ -5,7 +5,9 @@ Checks, per oracle function:
                      linalgebra headers (tests/golden/linalgebra_golden.npz),
                      numpy's Cholesky / solve, the 3x3 closed form
   inplace_fnnls      scipy's Lawson-Hanson NNLS, brute force over all passive
-                     sets, KKT conditions, and the reference's OpenCL
+                     sets, KKT conditions (scipy 1.15's nnls returns
+                     non-optimal points on some inputs, so brute force and
+                     KKT are the judges where they disagree), and the reference's OpenCL
                      incremental-Cholesky FNNLS (tests/golden/fnnls_cl_golden.npz)
   fnnls              scipy, brute force
   DoFit              closed forms: noise-free recovery, identity covariance
