@@ -101,12 +101,17 @@ template <> struct Lim<float> {
 // Opaque copy of a pointer: stops the compiler from hoisting loads of the
 // wave-uniform type tables (hundreds of values) out of the Minimize loop and
 // keeping them all live in scalar registers.
+// The copy is made in the constant address space, so that the loads through
+// it stay scalar (s_load) rather than per-lane flat loads.
 template <typename P> MF_HD const P *opaque(const P *p)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    asm volatile("" : "+s"(p));
-#endif
+    auto q = (const __attribute__((address_space(4))) P *)p;
+    asm volatile("" : "+s"(q));
+    return (const P *)q;
+#else
     return p;
+#endif
 }
 
 // 4-bit fields packed in a 64-bit word (permutation and its inverse)
@@ -142,10 +147,28 @@ struct TypeHdr {
     int32_t first[10];
     int32_t pad[2];
 };
+// Inverses of the passive blocks of the x = 0 normal matrix AtA0, one per
+// passive set m (bit mask over the pulses, 1 <= m < 2^np).  At x = 0 the
+// normal matrix is the same for every channel of a type, so the solve of
+// the passive system depends on the passive set alone: s_P = inv(AtA0[P,P])
+// Atb_P.  inv[m] holds that inverse embedded in n x n (zeros outside m),
+// its lower triangle packed by rows (entry (i, j), j <= i, at i(i+1)/2 + j),
+// computed on the host in extended precision and rounded to FP64; slot
+// MASK_PACK - 1 holds the smallest pivot ratio (M_kk - sum_j L_kj^2) / M_kk
+// over k in m of the FP64 Cholesky of the block (<= 0 or NaN: that
+// factorization fails; the inverse is then NaN).
+constexpr int MASK_PACK = 56;
+constexpr int MASK_SETS = 1024;
+struct MaskTab {
+    alignas(16) double inv[MASK_SETS][MASK_PACK];
+};
+
 struct DevType {
     TypeHdr h;
     TypeMats<double> d;
     TypeMats<float> f;
+    const MaskTab *tab;  // passive-set factors of AtA0 (same memory space as the DevType)
+    int64_t pad_;
 };
 template <typename T> MF_HD const TypeMats<T> &mats(const DevType &t);
 template <> MF_HD const TypeMats<double> &mats<double>(const DevType &t) { return t.d; }
@@ -358,6 +381,101 @@ MF_HD bool solve_passive(const T (&M)[10][10], const T (&b)[10], uint32_t pm, T 
     }
     if (ill) *ill = bad || !ok;
     return ok;
+}
+
+// The solve of solve_passive with the factor read from a MaskFactor: the
+// same two triangular solves, in the same order, with L streamed from memory
+// instead of being formed (and held) in registers.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T> using GlobalPtr = const __attribute__((address_space(1))) T *;
+typedef double Dbl2 __attribute__((ext_vector_type(2)));
+#else
+template <typename T> using GlobalPtr = const T *;
+struct Dbl2 {
+    double v[2];
+    double operator[](int e) const { return v[e]; }
+};
+#endif
+
+// (row, column) of packed lower-triangle entry e
+struct TriIndex {
+    int8_t i[MASK_PACK], j[MASK_PACK];
+    constexpr TriIndex() : i(), j()
+    {
+        int e = 0;
+        for (int r = 0; r < 10; ++r)
+            for (int c = 0; c <= r; ++c, ++e) {
+                i[e] = (int8_t)r;
+                j[e] = (int8_t)c;
+            }
+        i[e] = j[e] = -1;
+    }
+};
+constexpr TriIndex TRI{};
+
+// s = inv b for the passive set pm, inv read from a MaskTab row (see there):
+// s_i = sum_j inv_ij b_j over the packed lower triangle, each entry used for
+// both (i, j) and (j, i).  The row is streamed in 16-byte chunks, two stages
+// in flight, so a solve costs about one memory round trip.  Returns the
+// block's pivot ratio (the row's last slot).
+template <int N>
+MF_HD double solve_inverse(const double *row, const double (&b)[10], uint32_t pm,
+                           double (&s)[10])
+{
+    constexpr int NE = N * (N + 1) / 2;  // packed entries
+    constexpr int NC = MASK_PACK / 2;    // chunks in a row (the last one carries the ratio)
+    constexpr int NCU = (NE + 1) / 2;    // chunks holding entries (for N = 10 also the ratio)
+    constexpr int S = 7;                 // chunks per stage
+    constexpr int NST = (NCU + S - 1) / S;
+    static_assert(NE <= MASK_PACK - 1 && NCU <= NC, "packing");
+    const GlobalPtr<Dbl2> p = (GlobalPtr<Dbl2>)row;
+    double bm[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+        bm[k] = (k < N && ((pm >> k) & 1u)) ? b[k] : 0.0;
+        s[k] = 0.0;
+    }
+    Dbl2 buf[2][S];
+    double ratio = 0.0;
+#pragma unroll
+    for (int st = 0; st < 2 && st < NST; ++st)
+#pragma unroll
+        for (int c = 0; c < S; ++c)
+            if (st * S + c < NCU) buf[st][c] = p[st * S + c];
+#pragma unroll
+    for (int st = 0; st < NST; ++st) {
+#pragma unroll
+        for (int c = 0; c < S; ++c) {
+            const int ch = st * S + c;
+            if (ch >= NCU) continue;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int e = 2 * ch + h;
+                const double v = buf[st & 1][c][h];
+                if (e < NE) {
+                    const int i = TRI.i[e], j = TRI.j[e];
+                    s[i] = fma(v, bm[j], s[i]);
+                    if (i != j) s[j] = fma(v, bm[i], s[j]);
+                } else if (e == MASK_PACK - 1) {
+                    ratio = v;
+                }
+            }
+        }
+        if (st + 2 < NST) {
+            // the address of stage st + 2 depends on this stage's result, so
+            // its loads issue after this stage's use: at most two stages
+            // (14 chunks) are in flight per lane
+            GlobalPtr<Dbl2> q = p;
+#if defined(__HIP_DEVICE_COMPILE__)
+            asm volatile("" : "+v"(q) : "v"(s[0]));
+#endif
+#pragma unroll
+            for (int c = 0; c < S; ++c)
+                if ((st + 2) * S + c < NCU) buf[st & 1][c] = q[(st + 2) * S + c];
+        }
+    }
+    if (NCU < NC) ratio = ((GlobalPtr<double>)row)[MASK_PACK - 1];
+    return ratio;
 }
 
 template <typename T>
@@ -717,19 +835,31 @@ MF_HD void fit_plain(const TypeMats<T> &tm, const T (&s)[10], const FitParams &p
 // UNI: M lives in memory and is the same for every lane (a type's table): it
 // is re-read from the scalar cache on every pass instead of being held in
 // scalar registers across the loops.
-template <typename T, int N, bool UNI = false>
+//
+// TAB (FP64): M0 is the type's AtA0 and tab holds the inverses of its
+// passive blocks: each solve is a product with the set's inverse instead of
+// a Cholesky factorization, and the factorization's success is read from
+// the set's FP64 pivot ratio.
+template <typename T, int N, bool UNI = false, bool TAB = false>
 MF_HD void fnnls_active(const T (&M0)[10][10], const T (&Atb)[10], T (&x)[10], uint32_t &pm,
                         T eps, T eps_rel, int max_iter, FnnlsStats &st, T tau = T(0),
-                        bool *ill = nullptr)
+                        bool *ill = nullptr, const MaskTab *tab = nullptr)
 {
     bool solve = pm != 0;
     int adds = 0;
     for (;;) {
         while (solve && pm) {
             T s[10];
-            bool bad = false;
-            const T(&M)[10][10] = UNI ? *opaque(&M0) : M0;
-            const bool ok = solve_passive<T, N>(M, Atb, pm, s, tau, &bad);
+            bool bad = false, ok;
+            if constexpr (TAB) {
+                static_assert(sizeof(T) == 8, "the inverse tables are FP64");
+                const double r = solve_inverse<N>(tab->inv[pm], Atb, pm, s);
+                ok = r > 0.0;
+                bad = !(r >= (double)tau);
+            } else {
+                const T(&M)[10][10] = UNI ? *opaque(&M0) : M0;
+                ok = solve_passive<T, N>(M, Atb, pm, s, tau, &bad);
+            }
             st.inner++;
             if (tau > T(0) && bad) {
                 *ill = true;
@@ -863,15 +993,13 @@ MF_HD void fast_phase1(const DevType &ty, const float (&s32)[10], const FitParam
 
 // Phase 2 and the rest of the fit (see fit_fast).  x: phase 1's amplitudes
 // (passive set = entries > 0).  Fills out.
-template <int NS, int NP, bool LIVE, typename Park>
+template <int NS, int NP, bool LIVE, bool TAB = true, typename Park>
 MF_HD void fast_phase2(const DevType &ty, const float (&s32)[10], const FitParams &prm,
                        const float (&x)[10], bool warm, FnnlsStats &st, int flags, Park &park,
                        FitOut<float> &out)
 {
     const TypeMats<double> &td = ty.d;
-    double s64[10], x64[10];
-#pragma unroll
-    for (int i = 0; i < 10; ++i) s64[i] = (double)s32[i];
+    double x64[10];
     int nmin = 0;
     double chi2 = 0.0;
     if (!all_finite<float, NS>(s32)) {
@@ -889,7 +1017,7 @@ MF_HD void fast_phase2(const DevType &ty, const float (&s32)[10], const FitParam
         for (int p = 0; p < NP; ++p) {
             double t64 = 0.0;
 #pragma unroll
-            for (int k = 0; k < NS; ++k) t64 = fma(td.G0[p][k], s64[k], t64);
+            for (int k = 0; k < NS; ++k) t64 = fma(td.G0[p][k], (double)s32[k], t64);
             Atb64[p] = t64;
             x64[p] = x[p] > 0.0f ? (double)x[p] : 0.0;
             if (x[p] > 0.0f) pm |= 1u << p;
@@ -897,10 +1025,13 @@ MF_HD void fast_phase2(const DevType &ty, const float (&s32)[10], const FitParam
 #pragma unroll
         for (int p = NP; p < 10; ++p) x64[p] = 0.0;
         const int outer0 = st.outer;
-        fnnls_active<double, NP, true>(td.AtA0, Atb64, x64, pm, prm.eps, prm.eps_rel64,
-                                       prm.max_iter, st);
+        fnnls_active<double, NP, true, TAB>(td.AtA0, Atb64, x64, pm, prm.eps, prm.eps_rel64,
+                                            prm.max_iter, st, 0.0, nullptr, ty.tab);
         flags |= F_REFINED;
         if (warm && st.outer != outer0) flags |= F_WARM_REJECT;
+        double s64[10];   // formed here, not held through the solves
+#pragma unroll
+        for (int i = 0; i < 10; ++i) s64[i] = (double)s32[i];
         chi2 = chisq64<double, double, NS, NP>(*opaque(&td), td.Li0, s64, x64);
         // If the type's pulse covariance is zero (or not used) the covariance
         // never depends on x: the reference's iteration 2 rebuilds the same
@@ -936,7 +1067,7 @@ MF_HD void fast_phase2(const DevType &ty, const float (&s32)[10], const FitParam
 }
 
 // The fast fit of one channel, both phases in one call.
-template <int NS, int NP, bool LIVE, typename Park>
+template <int NS, int NP, bool LIVE, bool TAB = true, typename Park>
 MF_HD void fit_fast(const DevType &ty, const float (&s32)[10], const FitParams &prm,
                     float (&x)[10], uint32_t pm, bool warm, Park &park, FitOut<float> &out)
 {
@@ -944,7 +1075,7 @@ MF_HD void fit_fast(const DevType &ty, const float (&s32)[10], const FitParams &
     FnnlsStats st;
     if (all_finite<float, NS>(s32) && prm.max_minimize > 0)
         fast_phase1<NS, NP>(ty, s32, prm, x, pm, st, flags);
-    fast_phase2<NS, NP, LIVE>(ty, s32, prm, x, warm, st, flags, park, out);
+    fast_phase2<NS, NP, LIVE, TAB>(ty, s32, prm, x, warm, st, flags, park, out);
 }
 
 // Register-array park for host builds and for kernels that keep L^{-1} live.

@@ -352,6 +352,7 @@ struct mf_context {
     int device = 0;
     std::vector<mf_chtype> types;
     DevType *d_types = nullptr;
+    MaskTab *d_tabs = nullptr;     // one per type, pointed to by DevType::tab
     // partition scratch
     void *d_part = nullptr;  // counts[256], cursors[256], offsets[256] (int64)
     int32_t *d_idx = nullptr;
@@ -410,6 +411,7 @@ extern "C" mf_status mf_context_destroy(mf_context *ctx)
     if (!ctx) return MF_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->d_types) (void)hipFree(ctx->d_types);
+    if (ctx->d_tabs) (void)hipFree(ctx->d_tabs);
     if (ctx->d_part) (void)hipFree(ctx->d_part);
     if (ctx->d_idx) (void)hipFree(ctx->d_idx);
     for (hipEvent_t &e : ctx->ev)
@@ -433,11 +435,21 @@ extern "C" mf_status mf_set_types(mf_context *ctx, const mf_chtype *types, int32
         const char *e = expand_type(types[t], h[(size_t)t]);
         if (e) return fail(MF_ERR_INVALID, "type %d: %s", t, e);
     }
+    std::vector<MaskTab> tabs((size_t)ntypes);
+    for (int t = 0; t < ntypes; ++t) build_mask_tab(h[(size_t)t], tabs[(size_t)t]);
     HIP_TRY(hipSetDevice(ctx->device));
     if (ctx->d_types) {
         HIP_TRY(hipFree(ctx->d_types));
         ctx->d_types = nullptr;
     }
+    if (ctx->d_tabs) {
+        HIP_TRY(hipFree(ctx->d_tabs));
+        ctx->d_tabs = nullptr;
+    }
+    HIP_TRY(hipMalloc(&ctx->d_tabs, sizeof(MaskTab) * (size_t)ntypes));
+    HIP_TRY(hipMemcpy(ctx->d_tabs, tabs.data(), sizeof(MaskTab) * (size_t)ntypes,
+                      hipMemcpyHostToDevice));
+    for (int t = 0; t < ntypes; ++t) h[(size_t)t].tab = ctx->d_tabs + t;
     HIP_TRY(hipMalloc(&ctx->d_types, sizeof(DevType) * (size_t)ntypes));
     HIP_TRY(hipMemcpy(ctx->d_types, h.data(), sizeof(DevType) * (size_t)ntypes,
                       hipMemcpyHostToDevice));

@@ -425,8 +425,8 @@ hipError_t grad_step(Scratch &w, const float *samples, int64_t lds, const float 
           *gW3 = grad + 704 + 4160, *gb3 = grad + 704 + 4160 + 1280;
     // zero with kernels, not memset nodes, so that the step can be captured
     // in a graph and replayed
-    zero(grad, NPARAM, st);
-    if (loss) zero(loss, 1, st);
+    (void)zero(grad, NPARAM, st);
+    if (loss) (void)zero(loss, 1, st);
     hipLaunchKernelGGL(k_mlp_features<float>, dim3(nblk(B, 256)), dim3(256), 0, st, samples, lds,
                        B, 10, w.X, w.m);
     const int splits = (int)std::min<int64_t>(64, std::max<int64_t>(1, B / 256));

@@ -99,4 +99,81 @@ inline const char *expand_type(const mf_chtype &t, DevType &d)
     return 0;
 }
 
+// Fills t with the passive-block inverses of d's AtA0 (see MaskTab).  Each
+// inverse is formed in extended precision (long double) from the FP64
+// normal matrix the kernels use, then rounded; the pivot ratio is that of
+// the FP64 masked Cholesky factorization solve_passive performs, in its
+// operation order.
+inline void build_mask_tab(const DevType &d, MaskTab &t)
+{
+    const int n = d.h.np;
+    const double(&M)[10][10] = d.d.AtA0;
+    memset(&t, 0, sizeof(t));
+    for (int m = 1; m < (1 << n) && m < MASK_SETS; ++m) {
+        double L[10][10] = {{0}};
+        double ratio = INFINITY;
+        for (int k = 0; k < n; ++k) {
+            const bool ink = (m >> k) & 1;
+            double sq = 0.0;
+            for (int j = 0; j < k; ++j) sq = fma(L[k][j], L[k][j], sq);
+            const double dk = ink ? M[k][k] - sq : 1.0;
+            if (ink) {
+                const double r = dk / M[k][k];
+                if (!(r >= ratio)) ratio = r;   // keeps a NaN
+            }
+            const double rk = 1.0 / sqrt(dk);
+            for (int i = k + 1; i < n; ++i) {
+                const bool ini = (m >> i) & 1;
+                double u = 0.0;
+                for (int j = 0; j < k; ++j) u = fma(L[i][j], L[k][j], u);
+                L[i][k] = (ink && ini) ? (M[i][k] - u) * rk : 0.0;
+            }
+        }
+        int idx[10], k = 0;
+        for (int i = 0; i < n; ++i)
+            if ((m >> i) & 1) idx[k++] = i;
+        long double C[10][10] = {{0}}, W[10][10] = {{0}};
+        bool spd = ratio > 0.0;
+        for (int a = 0; a < k && spd; ++a)
+            for (int b = 0; b <= a; ++b) {
+                long double v = M[idx[a]][idx[b]];
+                for (int c = 0; c < b; ++c) v -= C[a][c] * C[b][c];
+                if (a == b) {
+                    if (!(v > 0)) {
+                        spd = false;
+                        break;
+                    }
+                    C[a][a] = sqrtl(v);
+                } else {
+                    C[a][b] = v / C[b][b];
+                }
+            }
+        double *row = t.inv[m];
+        if (spd) {
+            for (int a = 0; a < k; ++a) {   // W = C^{-1}, lower
+                W[a][a] = 1.0L / C[a][a];
+                for (int b = 0; b < a; ++b) {
+                    long double v = 0;
+                    for (int c = b; c < a; ++c) v += C[a][c] * W[c][b];
+                    W[a][b] = -v / C[a][a];
+                }
+            }
+            for (int a = 0; a < k; ++a)     // inverse = W^T W
+                for (int b = 0; b <= a; ++b) {
+                    long double v = 0;
+                    for (int c = a; c < k; ++c) v += W[c][a] * W[c][b];
+                    const int i = idx[a], j = idx[b];
+                    row[i * (i + 1) / 2 + j] = (double)v;
+                }
+        } else {
+            for (int a = 0; a < k; ++a)
+                for (int b = 0; b <= a; ++b) {
+                    const int i = idx[a], j = idx[b];
+                    row[i * (i + 1) / 2 + j] = NAN;
+                }
+        }
+        row[MASK_PACK - 1] = ratio;
+    }
+}
+
 }  // namespace mf

@@ -14,6 +14,8 @@
 
 using namespace mf;
 
+static bool use_tables = true;
+
 template <int N, bool LIVE>
 static void run_fast(const DevType &dt, const double *samples, int64_t nch, int64_t c,
                      const FitParams &prm, double *x, double *chi2, int32_t *flags,
@@ -26,7 +28,10 @@ static void run_fast(const DevType &dt, const double *samples, int64_t nch, int6
     }
     RegPark<float> park;
     FitOut<float> out;
-    fit_fast<N, N, LIVE>(dt, s, prm, x0, 0u, false, park, out);
+    if (use_tables)
+        fit_fast<N, N, LIVE, true>(dt, s, prm, x0, 0u, false, park, out);
+    else
+        fit_fast<N, N, LIVE, false>(dt, s, prm, x0, 0u, false, park, out);
     for (int p = 0; p < 10; ++p) x[p * nch + c] = p < N ? (double)out.x[p] : 0.0;
     chi2[c] = (double)out.chi2;
     flags[c] = out.flags;
@@ -57,6 +62,25 @@ static void run(const DevType &dt, const double *samples, int64_t nch, int64_t c
 
 static float eps_rel32 = 1e-6f, tau32 = 1e-3f;
 static double eps_rel64 = 1e-15;
+// Row m of the passive-block inverse table of the type's x = 0 normal
+// matrix, as the kernels read it (MASK_PACK values: packed lower triangle,
+// pivot ratio in the last slot), and the type's AtA0 (100 values,
+// row-major).  Returns nonzero on a bad type.
+extern "C" int hc_mask_inverse(const mf_chtype *type, int m, double *row, double *ata0)
+{
+    DevType d;
+    if (expand_type(*type, d)) return 1;
+    std::vector<MaskTab> t(1);
+    build_mask_tab(d, t[0]);
+    for (int e = 0; e < MASK_PACK; ++e) row[e] = t[0].inv[m][e];
+    for (int i = 0; i < 100; ++i) ata0[i] = d.d.AtA0[i / 10][i % 10];
+    return 0;
+}
+
+// 1: the FP64 phase of the fast path reads the passive-block inverse tables
+// (as the kernels do); 0: it forms every factor (solve_passive).
+extern "C" void hc_set_use_tables(int on) { use_tables = on != 0; }
+
 extern "C" void hc_set_tolerances(float e32, float t32, double e64)
 {
     eps_rel32 = e32;
@@ -70,8 +94,14 @@ extern "C" int hc_fit_batch(const mf_chtype *types, int ntypes, const uint8_t *t
 {
     g_err = o->compute_errors ? err : nullptr;
     std::vector<DevType> dt((size_t)ntypes);
-    for (int t = 0; t < ntypes; ++t)
+    std::vector<MaskTab> tabs((size_t)ntypes);
+    for (int t = 0; t < ntypes; ++t) {
         if (expand_type(types[t], dt[(size_t)t])) return 1;
+        if (use_tables) {
+            build_mask_tab(dt[(size_t)t], tabs[(size_t)t]);
+            dt[(size_t)t].tab = &tabs[(size_t)t];
+        }
+    }
     FitParams prm;
     prm.eps = o->eps;
     prm.chi2_tol = o->chi2_tol;

@@ -111,6 +111,9 @@ def hostcore():
                                dp, dp, C.c_void_p, C.c_void_p, dp]
     L.hc_fit_batch.restype = C.c_int
     L.hc_nnls.argtypes = [dp, dp, dp, C.c_int, C.c_int, C.c_double, C.c_uint]
+    L.hc_mask_inverse.argtypes = [C.c_void_p, C.c_int, dp, dp]
+    L.hc_mask_inverse.restype = C.c_int
+    L.hc_set_use_tables.argtypes = [C.c_int]
     _cache["hostcore"] = L
     return L
 

@@ -68,3 +68,99 @@ def test_minos_errors_core_matches_oracle(pkg, pulse_cov):
     assert np.array_equal(np.isnan(e), np.isnan(er))
     ok = ~np.isnan(er)
     assert np.all(np.abs(e[ok] - er[ok]) <= 1e-9 * np.abs(er[ok]))
+
+
+def _exact_inverse(B):
+    """Inverse of the FP64 matrix B in exact rational arithmetic (Gauss-Jordan)."""
+    from fractions import Fraction
+    n = B.shape[0]
+    A = [[Fraction(float(B[i, j])) for j in range(n)] + [Fraction(int(i == j)) for j in range(n)]
+         for i in range(n)]
+    for c in range(n):
+        p = next(r for r in range(c, n) if A[r][c] != 0)
+        A[c], A[p] = A[p], A[c]
+        piv = A[c][c]
+        A[c] = [v / piv for v in A[c]]
+        for r in range(n):
+            if r != c and A[r][c] != 0:
+                f = A[r][c]
+                A[r] = [a - f * b for a, b in zip(A[r], A[c])]
+    return np.array([[float(v) for v in row[n:]] for row in A])
+
+
+@pytest.mark.parametrize("cfg", ["c2", "c3"])
+def test_mask_inverse_tables(pkg, cfg):
+    """The passive-block inverse tables the FP64 phase of the fast path
+    reads.  AtA0 is recomputed here from the oracle's pulse matrix and numpy's
+    Cholesky of pederr^2 * noisecor.  For every passive set m the row holds
+    inv(AtA0[P,P]) packed as a lower triangle, zeros outside m: checked by its
+    residual (scaled by the block's condition number) and, on a sample of sets
+    that includes the full one, against the exact rational inverse of the same
+    FP64 block.  The last slot is the smallest pivot ratio of the block's FP64
+    Cholesky factorization."""
+    import ctypes as C
+    from oracle_binding import dptr, hostcore, oracle
+    from test_regression_amd import workloads
+    H, O = hostcore(), oracle()
+    tri = [(i, j) for i in range(10) for j in range(i + 1)]
+    rng = np.random.default_rng(5)
+    for ct in workloads.config(cfg).chtypes():
+        n = ct.nsamples
+        P = np.zeros(100)
+        O.or_build_pulsemat(C.byref(ct), dptr(P))
+        P = P.reshape(10, 10)[:n, :n]
+        C0 = ct.pederr ** 2 * np.array(ct.noisecor).reshape(10, 10)[:n, :n]
+        A = np.linalg.solve(np.linalg.cholesky(C0), P)
+        ata_np = A.T @ A
+        row, ata = np.zeros(56), np.zeros(100)
+        sample = set(rng.integers(1, 1 << n, 12).tolist()) | {(1 << n) - 1, 1, 1 << (n - 1)}
+        worst = 0.0
+        for m in range(1, 1 << n):
+            assert H.hc_mask_inverse(C.byref(ct), m, dptr(row), dptr(ata)) == 0
+            M = ata.reshape(10, 10)[:n, :n]
+            if m == 1:
+                assert np.allclose(M, ata_np, rtol=1e-10, atol=1e-12 * np.abs(ata_np).max())
+            Inv = np.zeros((n, n))
+            for e, (i, j) in enumerate(tri[:n * (n + 1) // 2]):
+                Inv[i, j] = Inv[j, i] = row[e]
+            assert np.all(row[n * (n + 1) // 2:55] == 0)
+            pas = np.array([(m >> k) & 1 for k in range(n)], bool)
+            assert np.all(Inv[~pas, :] == 0) and np.all(Inv[:, ~pas] == 0)
+            B = M[np.ix_(pas, pas)]
+            cond = np.linalg.cond(B)
+            res = np.abs(Inv[np.ix_(pas, pas)] @ B - np.eye(pas.sum())).max()
+            worst = max(worst, res / cond)
+            assert res <= 1e-14 * cond + 1e-13, (m, res, cond)
+            Lnp = np.linalg.cholesky(B)
+            want = (np.diag(Lnp) ** 2 / np.diag(B)).min()
+            assert abs(row[55] - want) <= 1e-6 * want + 1e-15 * cond, (m, row[55], want)
+            if m in sample:
+                X = _exact_inverse(B)
+                err = np.abs(Inv[np.ix_(pas, pas)] - X)
+                tol = 2.3e-16 * np.abs(X) + 1e-18 * cond * np.abs(X).max()
+                assert np.all(err <= tol), (m, (err / np.maximum(tol, 1e-300)).max())
+        print(f"{cfg} type n={n}: worst residual / cond = {worst:.2e}")
+
+
+@pytest.mark.parametrize("cfg,pulse_cov", [("c2", "zero"), ("c2", "live"), ("c3", "zero")])
+def test_fast_core_tables_match_formed_factors(pkg, cfg, pulse_cov):
+    """The fast path with table factors against the same path forming every
+    factor in registers (solve_passive): same active sets, same FP64 result."""
+    from oracle_binding import hostcore
+    from test_regression_amd import workloads, _abi
+    w = workloads.config(cfg, pulse_cov)
+    ct = w.chtypes()
+    s, t, _ = w.generate(4000)
+    s = s.astype(np.float32).astype(np.float64)
+    o = _abi.default_opts(precision=_abi.MF_PREC_FP32)
+    H = hostcore()
+    try:
+        H.hc_set_use_tables(0)
+        x0, c0, f0, i0 = hostcore_fit(ct, s, t, o)
+    finally:
+        H.hc_set_use_tables(1)
+    x1, c1, f1, i1 = hostcore_fit(ct, s, t, o)
+    scale = np.maximum(np.abs(x0).max(axis=0), 1.0)
+    assert ((x0 > 0) == (x1 > 0)).all(axis=0).mean() >= 0.9995
+    assert ((np.abs(x1 - x0) / scale).max(axis=0) <= 1e-6).mean() >= 0.9995
+    assert (np.abs(c1 - c0) / np.maximum(c0, 1) <= 1e-6).mean() >= 0.9995
