@@ -33,9 +33,11 @@ namespace mlp {
 // hardware exp and one hardware reciprocal.
 __device__ __forceinline__ float sig2u(float x)
 {
-    const float c2 = 1.5957691216057308f;  // 2 sqrt(2/pi)
-    const float u2 = c2 * fmaf(0.044715f * x, x * x, x);
-    return __builtin_amdgcn_rcpf(1.0f + __expf(-u2));
+    // 1 / (1 + exp(-2u)) = 1 / (1 + 2^(x (A + B x^2))), A = -2 sqrt(2/pi) log2(e),
+    // B = 0.044715 A: one v_exp_f32 and one v_rcp_f32
+    const float A = -1.5957691216057308f * 1.4426950408889634f;
+    const float B = A * 0.044715f;
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(x * fmaf(B, x * x, A)));
 }
 
 __device__ __forceinline__ float gelu(float x) { return x * sig2u(x); }
@@ -239,155 +241,221 @@ __global__ void k_quantize(const float *__restrict__ p, int n, float *__restrict
 }
 
 // ------------------------------------------------------------------ inference
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d)
-{
-    bf16x4 r;
-    r[0] = (short)f2bf(a);
-    r[1] = (short)f2bf(b);
-    r[2] = (short)f2bf(c);
-    r[3] = (short)f2bf(d);
-    return r;
-}
+// round to nearest even, in hardware (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ bf16x4 cvt4(f32x4 v) { return __builtin_convertvector(v, bf16x4); }
 
-// A-operand fragments of W^T (rows = outputs o, cols = inputs k): for the tile
-// (mt, kt) lane l holds W[kt*16 + 4*(l/16) + j][mt*16 + l%16], j = 0..3, zero
-// outside [nin x nout].  They are packed once per parameter update
-// (k_mlp_pack) in the order the inference kernel reads them: fragment f of
-// lane l at frag[(f*64 + l)*4 .. +3], so a wavefront loads each fragment with
-// one contiguous 512-byte read.  Fragments: 4 of W1 (mt, kt 0), 16 of W2
-// (mt*4 + kt), 8 of W3 (mt*4 + kt).
-constexpr int NFRAG = 28;
+// Weight fragments, packed once per parameter update (k_mlp_pack) and read by
+// the inference kernels from LDS.  Lane l = 16 g + n.
+//   layer 1 (v_mfma_f32_16x16x16_bf16), A = W1^T tile mt: lane l holds
+//     A[n][4g + j] = W1[4g + j][16 mt + n], j < 4 (0 for inputs >= 10);
+//   layers 2, 3 (v_mfma_f32_16x16x32_bf16), A = W^T tile (mt, kb): lane l
+//     holds A[n][8g + i] = W[h][16 mt + n], i < 8, with the hidden units of K
+//     block kb taken in the order h(kb, g, i) = 32 kb + 16 (i / 4) + 4 g + i % 4.
+// That order is the one in which a lane already holds them: the D fragment
+// of output tile mt gives lane l rows 16 mt + 4 g + j, so the D fragments of
+// tiles 2 kb and 2 kb + 1, after bias and GELU, are the B operand of K block
+// kb as they lie (a contraction does not depend on the order of its index).
+// Offsets (bfloat16 elements): W1 tile mt at 256 mt; W2 (mt, kb) at 1024 +
+// 512 (2 mt + kb); W3 (mt, kb) at 5120 + 512 (2 mt + kb); 7168 in all.
+constexpr int FRAG_W1 = 0, FRAG_W2 = 1024, FRAG_W3 = 5120;
+static_assert(FRAG_W3 + 4 * 512 == FRAG_ELEMS, "fragment size");
 
 __global__ void k_mlp_pack(const uint16_t *__restrict__ pbf, uint16_t *__restrict__ frag)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= NFRAG * 64) return;
-    const int f = t / 64, lane = t % 64;
-    const uint16_t *W;
-    int nin, nout, mt, kt;
+    if (t >= 16 * 64) return;
+    const int f = t / 64, lane = t % 64, g = lane >> 4, n = lane & 15;
     if (f < 4) {
-        W = pbf; nin = 10; nout = 64; mt = f; kt = 0;
-    } else if (f < 20) {
-        W = pbf + 704; nin = 64; nout = 64; mt = (f - 4) / 4; kt = (f - 4) % 4;
-    } else {
-        W = pbf + 704 + 4160; nin = 64; nout = 20; mt = (f - 20) / 4; kt = (f - 20) % 4;
+        const int o = f * 16 + n;
+        for (int j = 0; j < 4; ++j) {
+            const int k = 4 * g + j;
+            frag[FRAG_W1 + f * 256 + lane * 4 + j] = k < 10 ? pbf[k * 64 + o] : (uint16_t)0;
+        }
+        return;
     }
-    const int o = mt * 16 + (lane & 15);
-    for (int j = 0; j < 4; ++j) {
-        const int k = kt * 16 + 4 * (lane >> 4) + j;
-        frag[t * 4 + j] = (k < nin && o < nout) ? W[k * nout + o] : (uint16_t)0;
+    const bool l2 = f < 12;
+    const int q = l2 ? f - 4 : f - 12, mt = q / 2, kb = q % 2;
+    const uint16_t *W = l2 ? pbf + 704 : pbf + 704 + 4160;
+    const int nout = l2 ? 64 : 20, o = mt * 16 + n;
+    uint16_t *dst = frag + (l2 ? FRAG_W2 : FRAG_W3) + q * 512 + lane * 8;
+    for (int i = 0; i < 8; ++i) {
+        const int h = 32 * kb + 16 * (i / 4) + 4 * g + i % 4;
+        dst[i] = o < nout ? W[h * nout + o] : (uint16_t)0;
     }
 }
 
-__device__ __forceinline__ bf16x4 wfrag(const uint16_t *__restrict__ frag, int f, int lane)
+// LDS image of the network, shared by a block: fragments and biases
+// (b1 at 0, b2 at 64, b3 at 128, zero to 160).
+struct MlpLds {
+    alignas(16) uint16_t w[FRAG_ELEMS];
+    alignas(16) float b[160];
+};
+
+__device__ __forceinline__ void mlp_stage(MlpLds &L, const uint16_t *__restrict__ frag,
+                                          const float *__restrict__ pq, int tid, int nthr)
 {
-    return *reinterpret_cast<const bf16x4 *>(frag + ((size_t)f * 64 + lane) * 4);
+    const uint4 *src = reinterpret_cast<const uint4 *>(frag);
+    uint4 *dst = reinterpret_cast<uint4 *>(L.w);
+    for (int e = tid; e < FRAG_ELEMS / 8; e += nthr) dst[e] = src[e];
+    for (int e = tid; e < 160; e += nthr)
+        L.b[e] = e < 64 ? pq[640 + e] : e < 128 ? pq[704 + 4096 + e - 64]
+               : e < 148 ? pq[704 + 4160 + 1280 + e - 128] : 0.0f;
 }
 
-// out: [20][ldo] float: logits 0..9, then amplitudes (softplus * m) 10..19.
-template <typename TS>
-__global__ __launch_bounds__(256) void k_mlp_infer(const TS *__restrict__ s, int64_t ld, int ns,
-                                                   const int32_t *__restrict__ idx, int64_t n,
-                                                   const uint16_t *__restrict__ pbf /* packed */,
-                                                   const float *__restrict__ pq,
-                                                   float *__restrict__ out, int64_t ldo)
+// Features of one channel (this lane's), written where the tiles read them:
+// xs[lane][0..15] = s_i / m (bfloat16, zero past ns), m = max(max_i s_i, 1).
+__device__ __forceinline__ float mlp_features(const float (&v)[10], int ns, __bf16 (*xs)[16],
+                                              int lane)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t base = wave * 64;
-    if (base >= n) return;  // whole wave exits together
-    const uint16_t *frag = pbf;  // packed fragments (k_mlp_pack)
-    const float *b1 = pq + 640, *b2 = pq + 704 + 4096, *b3 = pq + 704 + 4160 + 1280;
-    bf16x4 a1[4], a2[4][4], a3[2][4];
+    float mx = -3.0e38f;
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) a1[mt] = wfrag(frag, mt, lane);
+    for (int i = 0; i < 10; ++i)
+        if (i < ns) mx = fmaxf(mx, v[i]);
+    const float sc = fmaxf(mx, 1.0f);
+    f32x4 q[4];
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
+    for (int i = 0; i < 16; ++i) q[i / 4][i % 4] = (i < 10 && i < ns) ? v[i] / sc : 0.0f;
 #pragma unroll
-        for (int kt = 0; kt < 4; ++kt) a2[mt][kt] = wfrag(frag, 4 + mt * 4 + kt, lane);
-#pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-        for (int kt = 0; kt < 4; ++kt) a3[mt][kt] = wfrag(frag, 20 + mt * 4 + kt, lane);
-    const int r4 = 4 * (lane >> 4);
+    for (int k = 0; k < 4; ++k) *reinterpret_cast<bf16x4 *>(&xs[lane][4 * k]) = cvt4(q[k]);
+    return sc;
+}
+
+// The network on the 64 channels of a wavefront, as 4 tiles of 16 channels.
+// xs: the wavefront's features (mlp_features), L: the staged network.
+// emit(t, o, z) is called for the lane's channel t (0..63) and outputs
+// o = 16 mt + 4 g + j < 20 with z = the output pre-activation (logits for
+// o < 10; the amplitude head's softplus is left to the caller).
+template <typename Emit>
+__device__ __forceinline__ void mlp_tiles(const MlpLds &L, const __bf16 (*xs)[16], int lane,
+                                          Emit &&emit)
+{
+    const int g = lane >> 4, n = lane & 15;
+    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll 1
     for (int nt = 0; nt < 4; ++nt) {
-        const int64_t g = base + nt * 16 + (lane & 15);
-        const bool live = g < n;
-        const int64_t c = live ? (idx ? (int64_t)idx[g] : g) : 0;
-        // features of channel c: this lane supplies inputs r4 .. r4+3
-        float v[10], mx = -3.0e38f;
-#pragma unroll
-        for (int i = 0; i < 10; ++i) {
-            v[i] = (live && i < ns) ? (float)s[(int64_t)i * ld + c] : 0.0f;
-            if (i < ns) mx = fmaxf(mx, v[i]);
-        }
-        const float sc = fmaxf(mx, 1.0f);
-        float f[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            float vv = 0.0f;
-#pragma unroll
-            for (int i = 0; i < 10; ++i)
-                if (i == r4 + j) vv = v[i];
-            f[j] = vv / sc;
-        }
-        const bf16x4 bx = pack4(f[0], f[1], f[2], f[3]);
-        // layer 1: Z1^T (64 x 16 channels) = W1^T X^T
-        bf16x4 b2v[4];
+        // an opaque zero offset per tile: the fragments are read from LDS
+        // where they are used, not hoisted into registers for all tiles
+        int w0 = 0;
+        asm volatile("" : "+s"(w0));
+        const bf16x4 bx = *reinterpret_cast<const bf16x4 *>(&xs[nt * 16 + n][4 * g]);
+        f32x4 d[4];
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
-            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[mt], bx, acc, 0, 0, 0);
-            float h[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) h[j] = gelu(acc[j] + b1[mt * 16 + r4 + j]);
-            b2v[mt] = pack4(h[0], h[1], h[2], h[3]);
+            const bf16x4 a = *reinterpret_cast<const bf16x4 *>(&L.w[w0 + FRAG_W1 + mt * 256 + lane * 4]);
+            d[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                __builtin_bit_cast(short __attribute__((ext_vector_type(4))), a),
+                __builtin_bit_cast(short __attribute__((ext_vector_type(4))), bx), zero, 0, 0, 0);
         }
-        // layer 2
-        bf16x4 b3v[4];
+        bf16x8 h[2];
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt) {
-            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int layer = 0; layer < 2; ++layer) {
+            // bias + GELU of the 4 tiles, regrouped as the 2 K blocks of the next layer
 #pragma unroll
-            for (int kt = 0; kt < 4; ++kt)
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[mt][kt], b2v[kt], acc, 0, 0, 0);
-            float h[4];
+            for (int kb = 0; kb < 2; ++kb) {
+                bf16x4 lo, hi;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) h[j] = gelu(acc[j] + b2[mt * 16 + r4 + j]);
-            b3v[mt] = pack4(h[0], h[1], h[2], h[3]);
+                for (int half = 0; half < 2; ++half) {
+                    const int mt = 2 * kb + half;
+                    const f32x4 bb = *reinterpret_cast<const f32x4 *>(&L.b[64 * layer + mt * 16 + 4 * g]);
+                    f32x4 v;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) v[j] = gelu(d[mt][j] + bb[j]);
+                    (half ? hi : lo) = cvt4(v);
+                }
+                h[kb] = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+            }
+            const int base = layer == 0 ? FRAG_W2 : FRAG_W3;
+            const int nmt = layer == 0 ? 4 : 2;
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                if (mt >= nmt) break;
+                f32x4 acc = zero;
+#pragma unroll
+                for (int kb = 0; kb < 2; ++kb) {
+                    const bf16x8 a = *reinterpret_cast<const bf16x8 *>(
+                        &L.w[w0 + base + (2 * mt + kb) * 512 + lane * 8]);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h[kb], acc, 0, 0, 0);
+                }
+                d[mt] = acc;
+            }
         }
-        // layer 3 and the head
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
-            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int kt = 0; kt < 4; ++kt)
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3[mt][kt], b3v[kt], acc, 0, 0, 0);
+            const f32x4 bb = *reinterpret_cast<const f32x4 *>(&L.b[128 + mt * 16 + 4 * g]);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int o = mt * 16 + r4 + j;
-                if (live && o < 20) {
-                    const float z = acc[j] + b3[o];
-                    out[(int64_t)o * ldo + g] = o < 10 ? z : softplusf(z) * sc;
-                }
+                const int o = mt * 16 + 4 * g + j;
+                if (o < 20) emit(nt * 16 + n, o, d[mt][j] + bb[j]);
             }
         }
     }
 }
 
+// out: [20][ldo] float: logits 0..9, then amplitudes (softplus * m) 10..19,
+// column = position in the channel list.  Grid-stride over 64-channel
+// wavefront tiles; the network is staged in LDS once per block.
+constexpr int INFER_BLOCK = 256;
+template <typename TS>
+__global__ __launch_bounds__(INFER_BLOCK) void k_mlp_infer(
+    const TS *__restrict__ s, int64_t ld, int ns, const int32_t *__restrict__ idx, int64_t n,
+    const uint16_t *__restrict__ frag, const float *__restrict__ pq, float *__restrict__ out,
+    int64_t ldo)
+{
+    __shared__ MlpLds L;
+    __shared__ __bf16 xs[INFER_BLOCK / 64][64][16];
+    __shared__ float ms[INFER_BLOCK / 64][64];
+    mlp_stage(L, frag, pq, threadIdx.x, INFER_BLOCK);
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int64_t base = ((int64_t)blockIdx.x * (INFER_BLOCK / 64) + wv) * 64; base < n;
+         base += (int64_t)gridDim.x * INFER_BLOCK) {
+        const int64_t g = base + lane;
+        const bool live = g < n;
+        const int64_t c = live ? (idx ? (int64_t)idx[g] : g) : 0;
+        float v[10];
+#pragma unroll
+        for (int i = 0; i < 10; ++i) v[i] = (live && i < ns) ? (float)s[(int64_t)i * ld + c] : 0.0f;
+        ms[wv][lane] = mlp_features(v, ns, xs[wv], lane);
+        // the tiles read other lanes' features: LDS operations of a wavefront
+        // complete in order, the fence keeps the compiler from reordering
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        mlp_tiles(L, xs[wv], lane, [&](int t, int o, float z) {
+            const int64_t gt = base + t;
+            if (gt < n) out[(int64_t)o * ldo + gt] = o < 10 ? z : softplusf(z) * ms[wv][t];
+        });
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
 // ------------------------------------------------------------------ host side
+static int infer_grid(int64_t n)
+{
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+    }
+    const int64_t need = (n + INFER_BLOCK - 1) / INFER_BLOCK;
+    return (int)std::min<int64_t>(need, (int64_t)cus * 8);
+}
+
 template <typename TS>
 void launch_infer(const void *s, int64_t ld, int ns, const int32_t *idx, int64_t n,
                   const uint16_t *pbf, const float *pq, float *out, int64_t ldo, hipStream_t st)
 {
     if (n <= 0) return;
-    const int64_t waves = (n + 63) / 64;
-    const int64_t blocks = (waves + 3) / 4;
-    hipLaunchKernelGGL(k_mlp_infer<TS>, dim3((unsigned)blocks), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_mlp_infer<TS>, dim3((unsigned)infer_grid(n)), dim3(INFER_BLOCK), 0, st,
                        static_cast<const TS *>(s), ld, ns, idx, n, pbf, pq, out, ldo);
 }
 template void launch_infer<float>(const void *, int64_t, int, const int32_t *, int64_t,
@@ -465,7 +533,7 @@ hipError_t adam_step(float *p, const float *g, float *m, float *v, int t, float 
 
 hipError_t pack(const uint16_t *pbf, uint16_t *frag, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_mlp_pack, dim3(nblk(NFRAG * 64, 256)), dim3(256), 0, st, pbf, frag);
+    hipLaunchKernelGGL(k_mlp_pack, dim3(nblk(16 * 64, 256)), dim3(256), 0, st, pbf, frag);
     return hipGetLastError();
 }
 

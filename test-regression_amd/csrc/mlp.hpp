@@ -26,10 +26,10 @@ hipError_t adam_step(float *p, const float *g, float *m, float *v, int t, float 
                      uint16_t *pbf, hipStream_t st);
 hipError_t quantize(const float *p, float *pq, uint16_t *pbf, hipStream_t st);
 hipError_t zero(float *p, int64_t n, hipStream_t st);
-// packs the bfloat16 weights into the inference kernel's fragment order
-// (frag: 28 * 64 * 4 uint16)
+// packs the bfloat16 weights into the inference kernels' fragment order
+// (frag: FRAG_ELEMS uint16)
 hipError_t pack(const uint16_t *pbf, uint16_t *frag, hipStream_t st);
-constexpr int FRAG_ELEMS = 28 * 64 * 4;
+constexpr int FRAG_ELEMS = 7168;
 // frag: packed fragments (pack), pq: bfloat16-rounded parameters as floats
 template <typename TS>
 void launch_infer(const void *s, int64_t ld, int ns, const int32_t *idx, int64_t n,
