@@ -161,6 +161,21 @@ typedef struct mf_timings {
 mf_status mf_set_timing(mf_context *ctx, int32_t enable);
 mf_status mf_get_timings(mf_context *ctx, mf_timings *out);
 
+/* Launch shape of the persistent fast-path kernels (opts.persistent = 1; no
+ * counterpart in the reference).  A lane that finishes its channel takes the
+ * next one from a device work queue; the grid holds a fixed number of
+ * resident wavefronts per compute unit. */
+typedef struct mf_tune {
+    int32_t block;      /* threads per block: 64, 128, 192 or 256             */
+    int32_t p1_wpc;     /* resident wavefronts per CU, FP32 phase (1..32)     */
+    int32_t p2_wpc;     /* resident wavefronts per CU, FP64 phase (1..32)     */
+    int32_t p1_refill;  /* idle lanes that trigger a refill, FP32 phase (1..64) */
+    int32_t p2_refill;  /* idle lanes that trigger a refill, FP64 phase (1..64) */
+    int32_t pad[3];
+} mf_tune;
+mf_status mf_get_tune(mf_context *ctx, mf_tune *out);
+mf_status mf_set_tune(mf_context *ctx, const mf_tune *tune);
+
 /* Batched NNLS solvers, one problem per channel, FP64.
  * A: n matrices of m x m, column-major as the reference's FixedMatrix
  * (nnls/interface/data_types.h:25-26), A[c*100 + col*10 + row] (leading

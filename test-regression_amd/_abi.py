@@ -76,6 +76,11 @@ class Timings(C.Structure):
                 ("pad", C.c_int32)]
 
 
+class Tune(C.Structure):
+    _fields_ = [("block", C.c_int32), ("p1_wpc", C.c_int32), ("p2_wpc", C.c_int32),
+                ("p1_refill", C.c_int32), ("p2_refill", C.c_int32), ("pad", C.c_int32 * 3)]
+
+
 class MFError(RuntimeError):
     pass
 
@@ -109,6 +114,8 @@ SIGNATURES = [
     ("mf_mlp_predict", C.c_int, [C.c_void_p, C.POINTER(Batch), C.c_void_p, C.c_void_p]),
     ("mf_set_timing", C.c_int, [C.c_void_p, C.c_int32]),
     ("mf_get_timings", C.c_int, [C.c_void_p, C.POINTER(Timings)]),
+    ("mf_get_tune", C.c_int, [C.c_void_p, C.POINTER(Tune)]),
+    ("mf_set_tune", C.c_int, [C.c_void_p, C.POINTER(Tune)]),
     ("mf_gen_type", C.c_int, [C.POINTER(GenSpec), C.POINTER(ChType)]),
     ("mf_gen_samples", C.c_int, [C.POINTER(GenSpec), C.c_int32, C.c_void_p, C.c_int64,
                                  C.c_uint64, C.c_void_p, C.c_void_p, C.c_int32]),

@@ -425,7 +425,7 @@ MF_HD double solve_inverse(const double *row, const double (&b)[10], uint32_t pm
     constexpr int NE = N * (N + 1) / 2;  // packed entries
     constexpr int NC = MASK_PACK / 2;    // chunks in a row (the last one carries the ratio)
     constexpr int NCU = (NE + 1) / 2;    // chunks holding entries (for N = 10 also the ratio)
-    constexpr int S = 7;                 // chunks per stage
+    constexpr int S = 4;                 // chunks per stage
     constexpr int NST = (NCU + S - 1) / S;
     static_assert(NE <= MASK_PACK - 1 && NCU <= NC, "packing");
     const GlobalPtr<Dbl2> p = (GlobalPtr<Dbl2>)row;
@@ -464,7 +464,7 @@ MF_HD double solve_inverse(const double *row, const double (&b)[10], uint32_t pm
         if (st + 2 < NST) {
             // the address of stage st + 2 depends on this stage's result, so
             // its loads issue after this stage's use: at most two stages
-            // (14 chunks) are in flight per lane
+            // (8 chunks) are in flight per lane
             GlobalPtr<Dbl2> q = p;
 #if defined(__HIP_DEVICE_COMPILE__)
             asm volatile("" : "+v"(q) : "v"(s[0]));
@@ -840,49 +840,64 @@ MF_HD void fit_plain(const TypeMats<T> &tm, const T (&s)[10], const FitParams &p
 // passive blocks: each solve is a product with the set's inverse instead of
 // a Cholesky factorization, and the factorization's success is read from
 // the set's FP64 pivot ratio.
+template <typename T> struct ActiveState {
+    T x[10];
+    uint32_t pm;  // passive set
+    bool solve;   // a solve of the passive system is due
+    int adds;     // indices added so far
+};
+
+// One step of fnnls_active.  If a solve is due: solve the passive system,
+// then either accept it (all positive) or step from x towards it until the
+// first passive value reaches zero and drop that index (a solve stays due).
+// If no solve is due afterwards: test the KKT conditions and add the free
+// index of largest w (a solve becomes due).  Returns true when the iteration
+// has ended: KKT met, max_iter adds, or (tau > 0) an ill-conditioned block
+// (*ill set, state left at the last feasible point).  A lane of a persistent
+// kernel runs one step per pass and takes a new channel when this returns
+// true; fnnls_active runs the steps to the end.
 template <typename T, int N, bool UNI = false, bool TAB = false>
-MF_HD void fnnls_active(const T (&M0)[10][10], const T (&Atb)[10], T (&x)[10], uint32_t &pm,
-                        T eps, T eps_rel, int max_iter, FnnlsStats &st, T tau = T(0),
-                        bool *ill = nullptr, const MaskTab *tab = nullptr)
+MF_HD bool fnnls_step(const T (&M0)[10][10], const T (&Atb)[10], ActiveState<T> &S, T eps,
+                      T eps_rel, int max_iter, FnnlsStats &st, T tau = T(0),
+                      bool *ill = nullptr, const MaskTab *tab = nullptr)
 {
-    bool solve = pm != 0;
-    int adds = 0;
-    for (;;) {
-        while (solve && pm) {
-            T s[10];
-            bool bad = false, ok;
-            if constexpr (TAB) {
-                static_assert(sizeof(T) == 8, "the inverse tables are FP64");
-                const double r = solve_inverse<N>(tab->inv[pm], Atb, pm, s);
-                ok = r > 0.0;
-                bad = !(r >= (double)tau);
-            } else {
-                const T(&M)[10][10] = UNI ? *opaque(&M0) : M0;
-                ok = solve_passive<T, N>(M, Atb, pm, s, tau, &bad);
-            }
-            st.inner++;
-            if (tau > T(0) && bad) {
-                *ill = true;
-                return;
-            }
-            if (!ok) st.llt_fail = true;
-            bool allpos = true;
+    if (S.solve && !S.pm) S.solve = false;
+    if (S.solve) {
+        const uint32_t pm = S.pm;
+        T s[10];
+        bool bad = false, ok;
+        if constexpr (TAB) {
+            static_assert(sizeof(T) == 8, "the inverse tables are FP64");
+            const double r = solve_inverse<N>(tab->inv[pm], Atb, pm, s);
+            ok = r > 0.0;
+            bad = !(r >= (double)tau);
+        } else {
+            const T(&M)[10][10] = UNI ? *opaque(&M0) : M0;
+            ok = solve_passive<T, N>(M, Atb, pm, s, tau, &bad);
+        }
+        st.inner++;
+        if (tau > T(0) && bad) {
+            *ill = true;
+            return true;
+        }
+        if (!ok) st.llt_fail = true;
+        bool allpos = true;
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if ((pm >> i) & 1u) allpos = allpos && (s[i] > T(0));
+        if (allpos) {
 #pragma unroll
             for (int i = 0; i < N; ++i)
-                if ((pm >> i) & 1u) allpos = allpos && (s[i] > T(0));
-            if (allpos) {
-#pragma unroll
-                for (int i = 0; i < N; ++i)
-                    if ((pm >> i) & 1u) x[i] = s[i];
-                break;
-            }
+                if ((pm >> i) & 1u) S.x[i] = s[i];
+            S.solve = false;
+        } else {
             T alpha = Lim<T>::max();
             int a = -1, afirst = -1;
 #pragma unroll
             for (int i = 0; i < N; ++i) {
                 if (((pm >> i) & 1u) && !(s[i] > T(0))) {
                     if (afirst < 0) afirst = i;
-                    const T ratio = mdiv(x[i], x[i] - s[i]);
+                    const T ratio = mdiv(S.x[i], S.x[i] - s[i]);
                     if (ratio < alpha) {
                         alpha = ratio;
                         a = i;
@@ -895,39 +910,59 @@ MF_HD void fnnls_active(const T (&M0)[10][10], const T (&Atb)[10], T (&x)[10], u
             }
 #pragma unroll
             for (int i = 0; i < N; ++i)
-                if ((pm >> i) & 1u) x[i] = (i == a) ? T(0) : mfma(alpha, s[i] - x[i], x[i]);
-            pm &= ~(1u << a);
+                if ((pm >> i) & 1u)
+                    S.x[i] = (i == a) ? T(0) : mfma(alpha, s[i] - S.x[i], S.x[i]);
+            S.pm = pm & ~(1u << a);
+            return false;
         }
-        solve = false;
-        if (adds >= max_iter) {
-            st.hit_max = true;
-            break;
-        }
-        int q = -1;
-        T best = T(0);
-        const T(&M)[10][10] = UNI ? *opaque(&M0) : M0;
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            T t = T(0), ta = T(0);
-#pragma unroll
-            for (int j = 0; j < N; ++j) {
-                const T a = sym(M, i, j);
-                t = mfma(a, x[j], t);
-                ta = mfma(mabs(a), mabs(x[j]), ta);
-            }
-            const T w = Atb[i] - t;
-            const T thr = eps + eps_rel * (mabs(Atb[i]) + ta);
-            if (!((pm >> i) & 1u) && w > thr && (q < 0 || w > best)) {
-                best = w;
-                q = i;
-            }
-        }
-        if (q < 0) break;
-        pm |= 1u << q;
-        ++adds;
-        st.outer++;
-        solve = true;
     }
+    if (S.adds >= max_iter) {
+        st.hit_max = true;
+        return true;
+    }
+    int q = -1;
+    T best = T(0);
+    const T(&M)[10][10] = UNI ? *opaque(&M0) : M0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        T t = T(0), ta = T(0);
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const T a = sym(M, i, j);
+            t = mfma(a, S.x[j], t);
+            ta = mfma(mabs(a), mabs(S.x[j]), ta);
+        }
+        const T w = Atb[i] - t;
+        const T thr = eps + eps_rel * (mabs(Atb[i]) + ta);
+        if (!((S.pm >> i) & 1u) && w > thr && (q < 0 || w > best)) {
+            best = w;
+            q = i;
+        }
+    }
+    if (q < 0) return true;
+    S.pm |= 1u << q;
+    ++S.adds;
+    st.outer++;
+    S.solve = true;
+    return false;
+}
+
+template <typename T, int N, bool UNI = false, bool TAB = false>
+MF_HD void fnnls_active(const T (&M0)[10][10], const T (&Atb)[10], T (&x)[10], uint32_t &pm,
+                        T eps, T eps_rel, int max_iter, FnnlsStats &st, T tau = T(0),
+                        bool *ill = nullptr, const MaskTab *tab = nullptr)
+{
+    ActiveState<T> S;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) S.x[i] = x[i];
+    S.pm = pm;
+    S.solve = pm != 0;
+    S.adds = 0;
+    while (!fnnls_step<T, N, UNI, TAB>(M0, Atb, S, eps, eps_rel, max_iter, st, tau, ill, tab)) {
+    }
+#pragma unroll
+    for (int i = 0; i < 10; ++i) x[i] = S.x[i];
+    pm = S.pm;
 }
 
 // chi2 with an FP64 residual: r = s - P x in FP64, y = Li r with the given
@@ -991,6 +1026,68 @@ MF_HD void fast_phase1(const DevType &ty, const float (&s32)[10], const FitParam
     if (ill) flags |= F_FP64_PIVOT;
 }
 
+// Start of phase 2 for one channel: the FP64 right side Atb = G0 s and the
+// FP32 phase's amplitudes as the starting state (passive set = entries > 0).
+// Returns false for a channel that is not fitted (non-finite samples, or
+// max_minimize <= 0; the flag is set and x stays 0).
+template <int NS, int NP>
+MF_HD bool fast_p2_begin(const DevType &ty, const float (&s32)[10], const FitParams &prm,
+                         const float (&x)[10], double (&Atb64)[10], ActiveState<double> &S,
+                         int &flags)
+{
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        S.x[i] = 0.0;
+        Atb64[i] = 0.0;
+    }
+    S.pm = 0;
+    S.solve = false;
+    S.adds = 0;
+    if (!all_finite<float, NS>(s32)) {
+        flags |= F_NONFINITE;
+        return false;
+    }
+    if (prm.max_minimize <= 0) {
+        flags |= F_MINIMIZE_CAP;
+        return false;
+    }
+    const TypeMats<double> &td = *opaque(&ty.d);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        double t64 = 0.0;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) t64 = fma(td.G0[p][k], (double)s32[k], t64);
+        Atb64[p] = t64;
+        S.x[p] = x[p] > 0.0f ? (double)x[p] : 0.0;
+        if (x[p] > 0.0f) S.pm |= 1u << p;
+    }
+    S.solve = S.pm != 0;
+    return true;
+}
+
+// chi2 of the phase-2 result, with the count of Minimize iterations when
+// the type's pulse covariance is zero or not used: the covariance then never
+// depends on x, the reference's iteration 2 rebuilds the same normal
+// equations, its FNNLS starts at their optimum and stops, chi2 repeats and
+// Minimize ends after 2 iterations with the same amplitudes -- so the fit
+// ends here.
+template <int NS, int NP>
+MF_HD double fast_p2_chi2(const DevType &ty, const float (&s32)[10], const double (&x64)[10])
+{
+    const TypeMats<double> &td = *opaque(&ty.d);
+    double s64[10];   // formed here, not held through the solves
+#pragma unroll
+    for (int i = 0; i < 10; ++i) s64[i] = (double)s32[i];
+    return chisq64<double, double, NS, NP>(td, td.Li0, s64, x64);
+}
+MF_HD int fixed_cov_nmin(double chi2, const FitParams &prm, int &flags)
+{
+    if (fabs(chi2) < prm.chi2_tol) return 1;
+    const int nmin = prm.max_minimize >= 2 ? 2 : 1;
+    if (nmin < 2) flags |= F_MINIMIZE_CAP;
+    return nmin;
+}
+
 // Phase 2 and the rest of the fit (see fit_fast).  x: phase 1's amplitudes
 // (passive set = entries > 0).  Fills out.
 template <int NS, int NP, bool LIVE, bool TAB = true, typename Park>
@@ -998,65 +1095,41 @@ MF_HD void fast_phase2(const DevType &ty, const float (&s32)[10], const FitParam
                        const float (&x)[10], bool warm, FnnlsStats &st, int flags, Park &park,
                        FitOut<float> &out)
 {
-    const TypeMats<double> &td = ty.d;
-    double x64[10];
+    double Atb64[10];
+    ActiveState<double> S;
     int nmin = 0;
     double chi2 = 0.0;
-    if (!all_finite<float, NS>(s32)) {
-        flags |= F_NONFINITE;
-#pragma unroll
-        for (int i = 0; i < 10; ++i) x64[i] = 0.0;
-    } else if (prm.max_minimize <= 0) {
-        flags |= F_MINIMIZE_CAP;
-#pragma unroll
-        for (int i = 0; i < 10; ++i) x64[i] = 0.0;
-    } else {
-        double Atb64[10];
-        uint32_t pm = 0;
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            double t64 = 0.0;
-#pragma unroll
-            for (int k = 0; k < NS; ++k) t64 = fma(td.G0[p][k], (double)s32[k], t64);
-            Atb64[p] = t64;
-            x64[p] = x[p] > 0.0f ? (double)x[p] : 0.0;
-            if (x[p] > 0.0f) pm |= 1u << p;
-        }
-#pragma unroll
-        for (int p = NP; p < 10; ++p) x64[p] = 0.0;
+    if (fast_p2_begin<NS, NP>(ty, s32, prm, x, Atb64, S, flags)) {
         const int outer0 = st.outer;
-        fnnls_active<double, NP, true, TAB>(td.AtA0, Atb64, x64, pm, prm.eps, prm.eps_rel64,
-                                            prm.max_iter, st, 0.0, nullptr, ty.tab);
+        while (!fnnls_step<double, NP, true, TAB>(ty.d.AtA0, Atb64, S, prm.eps, prm.eps_rel64,
+                                                  prm.max_iter, st, 0.0, nullptr, ty.tab)) {
+        }
         flags |= F_REFINED;
         if (warm && st.outer != outer0) flags |= F_WARM_REJECT;
-        double s64[10];   // formed here, not held through the solves
+        chi2 = fast_p2_chi2<NS, NP>(ty, s32, S.x);
+        // With a live pulse covariance the later iterations run the
+        // reference's FP64 iteration body (per-channel covariance,
+        // inplace_fnnls with warm x and reset passive set); they are FP64
+        // because the reference decides them on gradients near its 1e-11
+        // threshold.
+        if (LIVE && prm.use_pi) {
+            nmin = 1;
+            if (!(fabs(chi2) < prm.chi2_tol)) {
+                double s64[10];
 #pragma unroll
-        for (int i = 0; i < 10; ++i) s64[i] = (double)s32[i];
-        chi2 = chisq64<double, double, NS, NP>(*opaque(&td), td.Li0, s64, x64);
-        // If the type's pulse covariance is zero (or not used) the covariance
-        // never depends on x: the reference's iteration 2 rebuilds the same
-        // normal equations, its FNNLS starts at their optimum and stops, chi2
-        // repeats and Minimize ends after 2 iterations with the same
-        // amplitudes -- so the fit ends here.  With a live pulse covariance
-        // the later iterations run the reference's FP64 iteration body
-        // (per-channel covariance, inplace_fnnls with warm x and reset passive
-        // set); they are FP64 because the reference decides them on gradients
-        // near its 1e-11 threshold.
-        nmin = 1;
-        if (!(fabs(chi2) < prm.chi2_tol)) {
-            if (LIVE && prm.use_pi)
-                nmin = minimize_from<double, NS, NP>(ty.d, s64, prm, park, x64, chi2, 1, st,
+                for (int i = 0; i < 10; ++i) s64[i] = (double)s32[i];
+                nmin = minimize_from<double, NS, NP>(ty.d, s64, prm, park, S.x, chi2, 1, st,
                                                      flags);
-            else
-                nmin = prm.max_minimize >= 2 ? 2 : 1;
-            if (nmin < 2 && !(LIVE && prm.use_pi)) flags |= F_MINIMIZE_CAP;
+            }
+        } else {
+            nmin = fixed_cov_nmin(chi2, prm, flags);
         }
     }
     if (st.hit_max) flags |= F_FNNLS_CAP;
     if (st.llt_fail) flags |= F_LLT_FAIL;
 #pragma unroll
     for (int i = 0; i < 10; ++i) {
-        out.x[i] = (float)x64[i];
+        out.x[i] = (float)S.x[i];
         out.err[i] = 0.0f;
     }
     out.chi2 = (float)chi2;

@@ -95,29 +95,45 @@ void k_fit_plain(const DevType *__restrict__ types, int type_id, const int32_t *
 // reads them back, finishes the fit in FP64 and writes the final outputs.
 // Phase 1 needs about half the registers of phase 2, so splitting them lets
 // the long FP32 phase run at twice the occupancy.
-// warm: null, or the MLP's prediction [20][nch] (logits, amplitudes): the
-// passive set {i: logit_i > 0} with its predicted amplitudes starts phase 1.
+// Warm start (WarmIn): the MLP's predicted passive set pm[c] and amplitudes
+// x[i][c]; a predicted passive value starts at max(x, 1e-6) (phase 1 needs a
+// feasible point).  Null pointers: cold start.
+struct WarmIn {
+    const float *x;     // [10][nch]
+    const uint32_t *pm; // [nch]
+};
+template <int N>
+__device__ __forceinline__ uint32_t warm_load(const WarmIn &w, int64_t c, int64_t nch,
+                                              float (&x)[10])
+{
+    uint32_t pm = 0;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) x[i] = 0.0f;
+    if (w.pm) {
+        pm = w.pm[c] & ((1u << N) - 1u);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const float v = w.x[(int64_t)i * nch + c];
+            x[i] = ((pm >> i) & 1u) ? fmaxf(v, 1e-6f) : 0.0f;
+        }
+    }
+    return pm;
+}
+
 template <typename TS, int N>
 __global__ __launch_bounds__(256)
 void k_fast_p1(const DevType *__restrict__ types, int type_id, const int32_t *__restrict__ idx,
                int64_t nitems, const TS *__restrict__ samples, int64_t nch,
                float *__restrict__ xo, int32_t *__restrict__ flo, int32_t *__restrict__ ito,
-               const float *__restrict__ warm, FitParams prm)
+               WarmIn warm, FitParams prm)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nitems) return;
     const int64_t c = idx ? (int64_t)idx[g] : g;
     float s[10], x[10];
-    uint32_t pm = 0;
 #pragma unroll
-    for (int i = 0; i < 10; ++i) {
-        s[i] = i < N ? (float)samples[(int64_t)i * nch + c] : 0.0f;
-        x[i] = 0.0f;
-        if (warm && i < N && warm[(int64_t)i * nch + c] > 0.0f) {
-            pm |= 1u << i;
-            x[i] = fmaxf(warm[(int64_t)(10 + i) * nch + c], 1e-6f);
-        }
-    }
+    for (int i = 0; i < 10; ++i) s[i] = i < N ? (float)samples[(int64_t)i * nch + c] : 0.0f;
+    uint32_t pm = warm_load<N>(warm, c, nch, x);
     int flags = 0;
     FnnlsStats st;
     if (all_finite<float, N>(s) && prm.max_minimize > 0)
@@ -160,6 +176,243 @@ void k_fast_p2(const DevType *__restrict__ types, int type_id, const int32_t *__
     chi2o[c] = out.chi2;
     flo[c] = out.flags;
     ito[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
+}
+
+// ---------------------------------------------------------------------------
+// persistent fast-path kernels: lanes refill from a device work queue
+// ---------------------------------------------------------------------------
+// Active-set iteration counts vary from channel to channel (pileup), and a
+// wavefront runs as long as its slowest lane.  In these kernels a lane that
+// has finished its channel takes the next one from a work queue while the
+// other lanes go on: the grid is sized to the machine (persistent waves), and
+// each pass of a wavefront's loop runs one active-set step (fnnls_step) on
+// every lane that holds a channel.
+//
+// Queue claims of a wavefront: items are claimed 64 at a time (one atomic
+// add by lane 0), and the next range is claimed one range ahead.  Lane 0
+// keeps that claim's result unread until the range is needed, so the atomic
+// completes in the background.  rb, ru: wave-uniform.
+struct QueueCursor {
+    unsigned rb, ru;  // current range base, items used from it
+    unsigned nb0;     // lane 0: base of the next range (claimed ahead)
+    __device__ void start(unsigned *head)
+    {
+        unsigned a = 0;
+        nb0 = 0;
+        if (__lane_id() == 0) {
+            a = atomicAdd(head, 64u);
+            nb0 = atomicAdd(head, 64u);
+        }
+        rb = __shfl(a, 0);
+        ru = 0;
+    }
+    // Items for the k lanes of `empty` (all 64 lanes call): this lane's item,
+    // INT64_MAX for a lane outside `empty`.
+    __device__ int64_t take(unsigned *head, uint64_t empty)
+    {
+        const int k = __popcll(empty);
+        const int r = __popcll(empty & ((1ull << __lane_id()) - 1ull));
+        const unsigned t1 = min((unsigned)k, 64u - ru);
+        int64_t it = (int64_t)rb + ru + r;
+        if (ru + (unsigned)k >= 64u) {
+            // the current range runs out: move to the claimed-ahead one and
+            // claim the next
+            const unsigned nb = __shfl(nb0, 0);
+            if ((unsigned)r >= t1) it = (int64_t)nb + (r - t1);
+            if (__lane_id() == 0) nb0 = atomicAdd(head, 64u);
+            rb = nb;
+            ru = (unsigned)k - t1;
+        } else {
+            ru += (unsigned)k;
+        }
+        return ((empty >> __lane_id()) & 1ull) ? it : INT64_MAX;
+    }
+    __device__ bool exhausted(int64_t nitems) const { return (int64_t)rb + ru >= nitems; }
+};
+
+// Raw warm-start prediction of one channel (loaded unconditionally, so the
+// loads need not complete until the channel is started); warm_start_state
+// turns it into the starting point.
+template <int N>
+__device__ __forceinline__ void warm_fetch(const WarmIn &w, int64_t c, int64_t nch,
+                                           uint32_t &pm, float (&x)[10])
+{
+    pm = 0;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) x[i] = 0.0f;
+    if (w.pm) {
+        pm = w.pm[c];
+#pragma unroll
+        for (int i = 0; i < N; ++i) x[i] = w.x[(int64_t)i * nch + c];
+    }
+}
+template <int N>
+__device__ __forceinline__ uint32_t warm_start_state(uint32_t pm, const float (&xr)[10],
+                                                     float (&x)[10])
+{
+    pm &= (1u << N) - 1u;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) x[i] = (i < N && ((pm >> i) & 1u)) ? fmaxf(xr[i], 1e-6f) : 0.0f;
+    return pm;
+}
+
+// Phase 1, persistent.  Each lane holds its current channel and a prefetched
+// next one (samples and warm-start prediction, loaded while the current one
+// is worked on); a lane that finishes switches to its next channel at once,
+// and empty prefetch slots are refilled from the queue once at least `need`
+// of them are empty.
+template <typename TS, int N>
+__global__ __launch_bounds__(256)
+void k_fast_p1q(const DevType *__restrict__ types, int type_id, const int32_t *__restrict__ idx,
+                int64_t nitems, const TS *__restrict__ samples, int64_t nch,
+                float *__restrict__ xo, int32_t *__restrict__ flo, int32_t *__restrict__ ito,
+                WarmIn warm, FitParams prm, unsigned *__restrict__ head, int need)
+{
+    const DevType &ty = types[type_id];
+    int64_t c = -1, cn = -1;
+    float Atb[10], svn[10], xn[10];
+    uint32_t pmn = 0;
+    ActiveState<float> S;
+    FnnlsStats st;
+    bool ill = false;
+    QueueCursor q;
+    q.start(head);
+    for (;;) {
+        // refill empty prefetch slots
+        const uint64_t em = __ballot(cn < 0);
+        if (!q.exhausted(nitems) && __popcll(em) >= min(need, 64)) {
+            const int64_t it = q.take(head, em);
+            if (it < nitems) {
+                cn = idx ? (int64_t)idx[it] : it;
+#pragma unroll
+                for (int i = 0; i < 10; ++i) svn[i] = i < N ? (float)samples[(int64_t)i * nch + cn] : 0.0f;
+                warm_fetch<N>(warm, cn, nch, pmn, xn);
+            }
+        }
+        // a lane without a channel takes its prefetched one
+        if (c < 0 && cn >= 0) {
+            c = cn;
+            cn = -1;
+            S.pm = warm_start_state<N>(pmn, xn, S.x);
+            S.solve = S.pm != 0;
+            S.adds = 0;
+            st = FnnlsStats();
+            ill = false;
+            if (all_finite<float, N>(svn) && prm.max_minimize > 0) {
+                const TypeMats<float> &tf = *opaque(&ty.f);
+#pragma unroll
+                for (int p = 0; p < 10; ++p) {
+                    float t = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < N; ++k) t = mfma(tf.G0[p][k], svn[k], t);
+                    Atb[p] = p < N ? t : 0.0f;
+                }
+            } else {
+#pragma unroll
+                for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = 0.0f;
+                flo[c] = 0;
+                ito[c] = 0;
+                c = -1;
+            }
+        }
+        if (c >= 0 &&
+            fnnls_step<float, N, true>(ty.f.AtA0, Atb, S, (float)prm.eps, prm.eps_rel32,
+                                       prm.max_iter, st, prm.tau32, &ill)) {
+#pragma unroll
+            for (int p = 0; p < N; ++p)
+                xo[(int64_t)p * nch + c] = ((S.pm >> p) & 1u) ? S.x[p] : 0.0f;
+            flo[c] = (ill ? F_FP64_PIVOT : 0) | (st.hit_max ? F_FNNLS_CAP : 0) |
+                     (st.llt_fail ? F_LLT_FAIL : 0);
+            ito[c] = (st.outer << 8) | (st.inner << 16);
+            c = -1;
+        }
+        if (q.exhausted(nitems) && __ballot(c >= 0 || cn >= 0) == 0) break;
+    }
+}
+
+// Phase 2 (zero or unused pulse covariance), persistent, with the same
+// prefetch and refill scheme; the next channel's samples and phase-1 result
+// are loaded while the current one is worked on.
+template <typename TS, int N>
+__global__ __launch_bounds__(256)
+void k_fast_p2q(const DevType *__restrict__ types, int type_id, const int32_t *__restrict__ idx,
+                int64_t nitems, const TS *__restrict__ samples, int64_t nch,
+                float *__restrict__ xo, float *__restrict__ chi2o, int32_t *__restrict__ flo,
+                int32_t *__restrict__ ito, int warm, FitParams prm, unsigned *__restrict__ head,
+                int need)
+{
+    const DevType &ty = types[type_id];
+    int64_t c = -1, cn = -1;
+    double Atb[10];
+    ActiveState<double> S;
+    FnnlsStats st;
+    int flags = 0, outer0 = 0;
+    float svn[10], xn[10];
+    int fn = 0, itn = 0;
+    QueueCursor q;
+    q.start(head);
+    for (;;) {
+        const uint64_t em = __ballot(cn < 0);
+        if (!q.exhausted(nitems) && __popcll(em) >= min(need, 64)) {
+            const int64_t it = q.take(head, em);
+            if (it < nitems) {
+                cn = idx ? (int64_t)idx[it] : it;
+#pragma unroll
+                for (int i = 0; i < 10; ++i) {
+                    svn[i] = i < N ? (float)samples[(int64_t)i * nch + cn] : 0.0f;
+                    xn[i] = i < N ? xo[(int64_t)i * nch + cn] : 0.0f;
+                }
+                fn = flo[cn];
+                itn = ito[cn];
+            }
+        }
+        if (c < 0 && cn >= 0) {
+            c = cn;
+            cn = -1;
+            st = FnnlsStats();
+            st.outer = (itn >> 8) & 255;
+            st.inner = (itn >> 16) & 0xFFFF;
+            st.hit_max = (fn & F_FNNLS_CAP) != 0;
+            st.llt_fail = (fn & F_LLT_FAIL) != 0;
+            outer0 = st.outer;
+            flags = fn & F_FP64_PIVOT;
+            if (!fast_p2_begin<N, N>(ty, svn, prm, xn, Atb, S, flags)) {
+#pragma unroll
+                for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = 0.0f;
+                chi2o[c] = 0.0f;
+                flo[c] = flags | (st.hit_max ? F_FNNLS_CAP : 0) | (st.llt_fail ? F_LLT_FAIL : 0);
+                ito[c] = (st.outer << 8) | (st.inner << 16);
+                c = -1;
+            }
+        }
+        if (c >= 0 &&
+            fnnls_step<double, N, true, true>(ty.d.AtA0, Atb, S, prm.eps, prm.eps_rel64,
+                                              prm.max_iter, st, 0.0, nullptr, ty.tab)) {
+            flags |= F_REFINED;
+            if (warm && st.outer != outer0) flags |= F_WARM_REJECT;
+            // the samples are read again here rather than held through the steps
+            float sv[10];
+#pragma unroll
+            for (int i = 0; i < 10; ++i) sv[i] = i < N ? (float)samples[(int64_t)i * nch + c] : 0.0f;
+            const double chi2 = fast_p2_chi2<N, N>(ty, sv, S.x);
+            const int nmin = fixed_cov_nmin(chi2, prm, flags);
+            if (st.hit_max) flags |= F_FNNLS_CAP;
+            if (st.llt_fail) flags |= F_LLT_FAIL;
+#pragma unroll
+            for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = (float)S.x[p];
+            chi2o[c] = (float)chi2;
+            flo[c] = flags;
+            ito[c] = nmin | (st.outer << 8) | (st.inner << 16);
+            c = -1;
+        }
+        if (q.exhausted(nitems) && __ballot(c >= 0 || cn >= 0) == 0) break;
+    }
+}
+
+__global__ void k_zero_u32(unsigned *__restrict__ p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
 // Partition of a mixed batch by channel type: counts, offsets, index lists.
@@ -348,6 +601,22 @@ __global__ void k_linalg(int op, T *M, T *L, T *b, T *y, int64_t n, int full, in
 // ---------------------------------------------------------------------------
 // context and host entry points
 // ---------------------------------------------------------------------------
+// Launch shape of the persistent fast-path kernels (mf_autotune chooses it):
+// threads per block, resident wavefronts per compute unit for each phase,
+// and the number of idle lanes that triggers a refill from the queue.
+struct FastTune {
+    int block = 256;
+    int p1_wpc = 16, p2_wpc = 12;
+    int p1_need = 16, p2_need = 16;
+};
+static bool tune_valid(const mf_tune &t)
+{
+    return t.block >= 64 && t.block <= 256 && t.block % 64 == 0 && t.p1_wpc >= 1 &&
+           t.p1_wpc <= 32 && t.p2_wpc >= 1 && t.p2_wpc <= 32 && t.p1_refill >= 1 &&
+           t.p1_refill <= 64 && t.p2_refill >= 1 && t.p2_refill <= 64;
+}
+constexpr int QUEUE_SLOTS = 64;
+
 struct mf_context {
     int device = 0;
     std::vector<mf_chtype> types;
@@ -365,8 +634,12 @@ struct mf_context {
     int mlp_t = 0;
     bool mlp_ready = false;
     mf::mlp::Scratch mlp_w;
-    float *pred = nullptr;  // [20][pred_cap] prediction for warm starts
+    float *pred = nullptr;  // warm-start prediction: x [10][nch], then pm [nch] (uint32)
     int64_t pred_cap = 0;
+    // persistent kernels: compute units, queue heads, launch shape
+    int cus = 0;
+    unsigned *d_queue = nullptr;  // QUEUE_SLOTS heads
+    FastTune tune;
     // opt-in kernel timing: events around the MLP and the fit launches
     bool timing = false;
     hipEvent_t ev[2 + 2 * 256] = {};
@@ -400,9 +673,42 @@ extern "C" mf_status mf_context_create(mf_context **ctx, int device)
     HIP_TRY(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev)
         return fail(MF_ERR_INVALID, "device %d out of range (%d devices)", device, ndev);
+    HIP_TRY(hipSetDevice(device));
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    unsigned *q = nullptr;
+    HIP_TRY(hipMalloc(&q, sizeof(unsigned) * QUEUE_SLOTS));
     mf_context *c = new mf_context;
     c->device = device;
+    c->cus = cus > 0 ? cus : 1;
+    c->d_queue = q;
     *ctx = c;
+    return MF_OK;
+}
+
+extern "C" mf_status mf_get_tune(mf_context *ctx, mf_tune *out)
+{
+    if (!ctx || !out) return fail(MF_ERR_INVALID, "NULL argument");
+    memset(out, 0, sizeof(*out));
+    out->block = ctx->tune.block;
+    out->p1_wpc = ctx->tune.p1_wpc;
+    out->p2_wpc = ctx->tune.p2_wpc;
+    out->p1_refill = ctx->tune.p1_need;
+    out->p2_refill = ctx->tune.p2_need;
+    return MF_OK;
+}
+
+extern "C" mf_status mf_set_tune(mf_context *ctx, const mf_tune *t)
+{
+    if (!ctx || !t) return fail(MF_ERR_INVALID, "NULL argument");
+    if (!tune_valid(*t))
+        return fail(MF_ERR_INVALID, "tune out of range (block 64..256 step 64, wpc 1..32, "
+                                    "refill 1..64)");
+    ctx->tune.block = t->block;
+    ctx->tune.p1_wpc = t->p1_wpc;
+    ctx->tune.p2_wpc = t->p2_wpc;
+    ctx->tune.p1_need = t->p1_refill;
+    ctx->tune.p2_need = t->p2_refill;
     return MF_OK;
 }
 
@@ -412,6 +718,7 @@ extern "C" mf_status mf_context_destroy(mf_context *ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->d_types) (void)hipFree(ctx->d_types);
     if (ctx->d_tabs) (void)hipFree(ctx->d_tabs);
+    if (ctx->d_queue) (void)hipFree(ctx->d_queue);
     if (ctx->d_part) (void)hipFree(ctx->d_part);
     if (ctx->d_idx) (void)hipFree(ctx->d_idx);
     for (hipEvent_t &e : ctx->ev)
@@ -485,10 +792,31 @@ static mf_status launch_fit(mf_context *ctx, int type_id, const int32_t *idx,
 template <typename TS, int N, bool LIVE>
 static mf_status launch_fast(mf_context *ctx, int type_id, const int32_t *idx, int64_t nitems,
                              const void *samples, int64_t nch, const mf_result *out,
-                             const FitParams &prm, int block, const float *warm, hipStream_t st)
+                             const FitParams &prm, int block, WarmIn warm, bool persistent,
+                             hipStream_t st)
 {
     if (nitems <= 0) return MF_OK;
     if (!out->iters) return fail(MF_ERR_INVALID, "the FP32 path needs mf_result.iters");
+    if (persistent && !LIVE) {
+        const FastTune &tu = ctx->tune;
+        const int wpb = tu.block / 64;
+        const int64_t need_w = (nitems + 63) / 64;
+        const int64_t g1 = std::max<int64_t>(1, std::min<int64_t>(need_w, (int64_t)ctx->cus * tu.p1_wpc) / wpb);
+        const int64_t g2 = std::max<int64_t>(1, std::min<int64_t>(need_w, (int64_t)ctx->cus * tu.p2_wpc) / wpb);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, ctx->d_queue, 2);
+        hipLaunchKernelGGL((k_fast_p1q<TS, N>), dim3((unsigned)g1), dim3(tu.block), 0, st,
+                           ctx->d_types, type_id, idx, nitems, static_cast<const TS *>(samples),
+                           nch, static_cast<float *>(out->x), out->flags, out->iters, warm, prm,
+                           ctx->d_queue, tu.p1_need);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL((k_fast_p2q<TS, N>), dim3((unsigned)g2), dim3(tu.block), 0, st,
+                           ctx->d_types, type_id, idx, nitems, static_cast<const TS *>(samples),
+                           nch, static_cast<float *>(out->x), static_cast<float *>(out->chi2),
+                           out->flags, out->iters, warm.pm ? 1 : 0, prm, ctx->d_queue + 1,
+                           tu.p2_need);
+        HIP_TRY(hipGetLastError());
+        return MF_OK;
+    }
     const int64_t grid = (nitems + block - 1) / block;
     hipLaunchKernelGGL((k_fast_p1<TS, N>), dim3((unsigned)grid), dim3(block), 0, st,
                        ctx->d_types, type_id, idx, nitems, static_cast<const TS *>(samples), nch,
@@ -498,7 +826,7 @@ static mf_status launch_fast(mf_context *ctx, int type_id, const int32_t *idx, i
     hipLaunchKernelGGL((k_fast_p2<TS, N, LIVE>), dim3((unsigned)grid), dim3(block), lds, st,
                        ctx->d_types, type_id, idx, nitems, static_cast<const TS *>(samples), nch,
                        static_cast<float *>(out->x), static_cast<float *>(out->chi2), out->flags,
-                       out->iters, warm ? 1 : 0, prm);
+                       out->iters, warm.pm ? 1 : 0, prm);
     HIP_TRY(hipGetLastError());
     return MF_OK;
 }
@@ -506,7 +834,7 @@ static mf_status launch_fast(mf_context *ctx, int type_id, const int32_t *idx, i
 template <typename TS>
 static mf_status dispatch_fast(mf_context *ctx, int type_id, const int32_t *idx, int64_t nitems,
                                const mf_batch *in, const mf_result *out, const FitParams &prm,
-                               int block, const float *warm, hipStream_t st)
+                               int block, WarmIn warm, bool persistent, hipStream_t st)
 {
     const mf_chtype &t = ctx->types[(size_t)type_id];
     bool has_pi = false;
@@ -514,13 +842,13 @@ static mf_status dispatch_fast(mf_context *ctx, int type_id, const int32_t *idx,
     const bool live = has_pi && prm.use_pi;
     if (t.nsamples == 10)
         return live ? launch_fast<TS, 10, true>(ctx, type_id, idx, nitems, in->samples, in->nch,
-                                                out, prm, block, warm, st)
+                                                out, prm, block, warm, persistent, st)
                     : launch_fast<TS, 10, false>(ctx, type_id, idx, nitems, in->samples, in->nch,
-                                                 out, prm, block, warm, st);
+                                                 out, prm, block, warm, persistent, st);
     return live ? launch_fast<TS, 8, true>(ctx, type_id, idx, nitems, in->samples, in->nch, out,
-                                           prm, block, warm, st)
+                                           prm, block, warm, persistent, st)
                 : launch_fast<TS, 8, false>(ctx, type_id, idx, nitems, in->samples, in->nch, out,
-                                            prm, block, warm, st);
+                                            prm, block, warm, persistent, st);
 }
 
 template <typename T>
@@ -585,26 +913,28 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
     if (opts->compute_errors && !out->err)
         return fail(MF_ERR_INVALID, "compute_errors needs mf_result.err");
     const int ntypes = (int)ctx->types.size();
-    const float *warm = nullptr;
+    WarmIn warm{nullptr, nullptr};
     if (opts->warm_start) {
         if (ctx->pred_cap < in->nch) {
             if (ctx->pred) HIP_TRY(hipFree(ctx->pred));
             ctx->pred = nullptr;
-            HIP_TRY(hipMalloc(&ctx->pred, sizeof(float) * 20 * (size_t)in->nch));
+            HIP_TRY(hipMalloc(&ctx->pred, sizeof(float) * 11 * (size_t)in->nch));
             ctx->pred_cap = in->nch;
         }
+        float *px = ctx->pred;
+        uint32_t *ppm = reinterpret_cast<uint32_t *>(ctx->pred + 10 * (size_t)in->nch);
         const int ns = ctx->types[0].nsamples > 8 ? 10 : 8;
         time_mark(ctx, 0, st);
         ctx->mlp_timed = ctx->timing;
         if (in->sample_dtype == MF_DTYPE_F64)
-            mf::mlp::launch_infer<double>(in->samples, in->nch, ns, nullptr, in->nch,
-                                          ctx->mlp_frag, ctx->mlp_pq, ctx->pred, in->nch, st);
+            mf::mlp::launch_warm<double>(in->samples, in->nch, ns, in->nch, ctx->mlp_frag,
+                                         ctx->mlp_pq, px, ppm, st);
         else
-            mf::mlp::launch_infer<float>(in->samples, in->nch, ns, nullptr, in->nch,
-                                         ctx->mlp_frag, ctx->mlp_pq, ctx->pred, in->nch, st);
+            mf::mlp::launch_warm<float>(in->samples, in->nch, ns, in->nch, ctx->mlp_frag,
+                                        ctx->mlp_pq, px, ppm, st);
         HIP_TRY(hipGetLastError());
         time_mark(ctx, 1, st);
-        warm = ctx->pred;
+        warm = WarmIn{px, ppm};
     } else {
         ctx->mlp_timed = false;
     }
@@ -612,9 +942,12 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
         if (opts->precision == MF_PREC_FP64)
             return dispatch_fit<double>(ctx, type_id, idx, count, nitems, in, out, prm, block,
                                         opts->compute_errors != 0, st);
+        const bool pers = opts->persistent != 0;
         return in->sample_dtype == MF_DTYPE_F64
-                   ? dispatch_fast<double>(ctx, type_id, idx, nitems, in, out, prm, block, warm, st)
-                   : dispatch_fast<float>(ctx, type_id, idx, nitems, in, out, prm, block, warm, st);
+                   ? dispatch_fast<double>(ctx, type_id, idx, nitems, in, out, prm, block, warm,
+                                           pers, st)
+                   : dispatch_fast<float>(ctx, type_id, idx, nitems, in, out, prm, block, warm,
+                                          pers, st);
     };
     ctx->n_fit_ev = 0;
     auto go = [&](int type_id, const int32_t *idx, const int64_t *count, int64_t nitems) {

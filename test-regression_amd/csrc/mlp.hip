@@ -435,6 +435,47 @@ __global__ __launch_bounds__(INFER_BLOCK) void k_mlp_infer(
     }
 }
 
+// Warm-start form of the prediction, for the fit kernels: pm[c] = the
+// passive-set bits {i: logit_i > 0}, px[i][c] = the predicted amplitude
+// (softplus * m) of pulse i, for channels c = 0..n-1.
+template <typename TS>
+__global__ __launch_bounds__(INFER_BLOCK) void k_mlp_warm(
+    const TS *__restrict__ s, int64_t ld, int ns, int64_t n, const uint16_t *__restrict__ frag,
+    const float *__restrict__ pq, float *__restrict__ px, uint32_t *__restrict__ ppm)
+{
+    __shared__ MlpLds L;
+    __shared__ __bf16 xs[INFER_BLOCK / 64][64][16];
+    __shared__ float ms[INFER_BLOCK / 64][64];
+    __shared__ uint32_t pmw[INFER_BLOCK / 64][64];
+    mlp_stage(L, frag, pq, threadIdx.x, INFER_BLOCK);
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int64_t base = ((int64_t)blockIdx.x * (INFER_BLOCK / 64) + wv) * 64; base < n;
+         base += (int64_t)gridDim.x * INFER_BLOCK) {
+        const int64_t c = base + lane;
+        const bool live = c < n;
+        float v[10];
+#pragma unroll
+        for (int i = 0; i < 10; ++i) v[i] = (live && i < ns) ? (float)s[(int64_t)i * ld + c] : 0.0f;
+        ms[wv][lane] = mlp_features(v, ns, xs[wv], lane);
+        pmw[wv][lane] = 0u;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        mlp_tiles(L, xs[wv], lane, [&](int t, int o, float z) {
+            if (o < 10) {
+                if (z > 0.0f && o < ns) atomicOr(&pmw[wv][t], 1u << o);
+            } else if (base + t < n) {
+                px[(int64_t)(o - 10) * ld + base + t] = softplusf(z) * ms[wv][t];
+            }
+        });
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (live) ppm[c] = pmw[wv][lane];
+    }
+}
+
 // ------------------------------------------------------------------ host side
 static int infer_grid(int64_t n)
 {
@@ -462,6 +503,19 @@ template void launch_infer<float>(const void *, int64_t, int, const int32_t *, i
                                   const uint16_t *, const float *, float *, int64_t, hipStream_t);
 template void launch_infer<double>(const void *, int64_t, int, const int32_t *, int64_t,
                                    const uint16_t *, const float *, float *, int64_t, hipStream_t);
+
+template <typename TS>
+void launch_warm(const void *s, int64_t ld, int ns, int64_t n, const uint16_t *frag,
+                 const float *pq, float *px, uint32_t *ppm, hipStream_t st)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_mlp_warm<TS>, dim3((unsigned)infer_grid(n)), dim3(INFER_BLOCK), 0, st,
+                       static_cast<const TS *>(s), ld, ns, n, frag, pq, px, ppm);
+}
+template void launch_warm<float>(const void *, int64_t, int, int64_t, const uint16_t *,
+                                 const float *, float *, uint32_t *, hipStream_t);
+template void launch_warm<double>(const void *, int64_t, int, int64_t, const uint16_t *,
+                                  const float *, float *, uint32_t *, hipStream_t);
 
 static void gemm(bool ta, bool tb, int M, int N, int K, const float *A, int lda, const float *B,
                  int ldb, float *C, int ldc, int splits, hipStream_t st)

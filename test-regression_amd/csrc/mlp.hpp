@@ -35,5 +35,11 @@ template <typename TS>
 void launch_infer(const void *s, int64_t ld, int ns, const int32_t *idx, int64_t n,
                   const uint16_t *frag, const float *pq, float *out, int64_t ldo, hipStream_t st);
 
+// warm-start form over channels 0..n-1 of s ([10][ld]): passive-set bits
+// ppm[n] and predicted amplitudes px[10][ld]
+template <typename TS>
+void launch_warm(const void *s, int64_t ld, int ns, int64_t n, const uint16_t *frag,
+                 const float *pq, float *px, uint32_t *ppm, hipStream_t st);
+
 }  // namespace mlp
 }  // namespace mf

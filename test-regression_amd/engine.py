@@ -121,6 +121,16 @@ class Engine:
         _abi.check(self._lib.mf_get_timings(self._ctx, C.byref(t)), "mf_get_timings")
         return {"mlp_ms": t.mlp_ms, "fit_ms": t.fit_ms, "fit_launches": t.fit_launches}
 
+    def get_tune(self) -> dict:
+        """Launch shape of the persistent kernels (mf_tune)."""
+        t = _abi.Tune()
+        _abi.check(self._lib.mf_get_tune(self._ctx, C.byref(t)), "mf_get_tune")
+        return {k: getattr(t, k) for k in ("block", "p1_wpc", "p2_wpc", "p1_refill", "p2_refill")}
+
+    def set_tune(self, **kw):
+        t = _abi.Tune(**{**self.get_tune(), **kw})
+        _abi.check(self._lib.mf_set_tune(self._ctx, C.byref(t)), "mf_set_tune")
+
     # ---- learned warm start (mf_mlp_*) ----
     def mlp_init(self, seed: int = 0):
         _abi.check(self._lib.mf_mlp_init(self._ctx, seed), "mf_mlp_init")

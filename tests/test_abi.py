@@ -37,6 +37,7 @@ def test_struct_layouts(pkg):
     assert a.Opts.eps.offset == 16
     assert C.sizeof(a.Batch) == 32 and C.sizeof(a.Result) == 40
     assert C.sizeof(a.GenSpec) == 8 + 11 * 8
+    assert C.sizeof(a.Tune) == 32
     assert pkg._abi.lib().mf_abi_version() == 1
 
 

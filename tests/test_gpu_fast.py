@@ -105,3 +105,57 @@ def test_fast_fit_edge_cases(pkg, gpu):
     big = oracle_fit(ct, s[:, [2]].astype(np.float64), None, oracle_opts())
     assert r.chi2[2].item() <= big["chi2"][0] * (1 + 1e-6)
     assert np.all(x[:, 2] >= 0) and np.isfinite(x[:, 2]).all()
+
+
+@pytest.mark.parametrize("cfg,n", [("c1", 10_000), ("c2", 50_000), ("c3", 20_000),
+                                   ("c5", 20_000)])
+def test_persistent_fast_fit_matches_oracle(pkg, gpu, cfg, n):
+    """The persistent kernels (lanes refill from a device work queue) against
+    the oracle, same bar."""
+    r, ref = _fit_fast(cfg, "zero", n, persistent=1)
+    _check(r, ref)
+
+
+@pytest.mark.parametrize("warm", [0, 1])
+def test_persistent_equals_per_channel_launch(pkg, gpu, warm):
+    """Each lane of the persistent kernels runs the same step sequence on its
+    channel as the per-channel kernels: the outputs are identical, bit for
+    bit, cold and warm-started."""
+    from test_regression_amd import engine, workloads, _abi, warmstart
+    w = workloads.config("c2")
+    ct = w.chtypes()
+    s, _, _ = w.generate(100_000)
+    eng = engine.Engine(list(ct))
+    if warm:
+        warmstart.ensure_trained(eng)
+    x = torch.from_numpy(s.astype(np.float32)).cuda()
+    print(f"[gpu-fast] persistent vs per-channel warm={warm}", flush=True)
+    a = eng.fit(x, None, _abi.default_opts(precision=_abi.MF_PREC_FP32, warm_start=warm))
+    b = eng.fit(x, None, _abi.default_opts(precision=_abi.MF_PREC_FP32, warm_start=warm,
+                                           persistent=1))
+    torch.cuda.synchronize()
+    assert torch.equal(a.x, b.x) and torch.equal(a.chi2, b.chi2)
+    assert torch.equal(a.flags, b.flags) and torch.equal(a.iters, b.iters)
+
+
+def test_persistent_edge_cases(pkg, gpu):
+    """Non-finite and empty channels, and batches smaller than one wavefront
+    or not a multiple of it, through the queue."""
+    from test_regression_amd import engine, workloads, _abi
+    w = workloads.config("c1")
+    ct = w.chtypes()
+    eng = engine.Engine(list(ct))
+    for n in (1, 5, 63, 65, 1000):
+        s = np.abs(np.random.default_rng(n).normal(0, 1, (10, n))).astype(np.float32)
+        s[:, 0] += 50 * np.exp(-0.5 * (np.arange(10) - 5) ** 2)
+        if n >= 5:
+            s[:, 1] = np.nan
+            s[:, 3] = 0.0
+        r = eng.fit(torch.from_numpy(s).cuda(), None,
+                    _abi.default_opts(precision=_abi.MF_PREC_FP32, persistent=1))
+        q = eng.fit(torch.from_numpy(s).cuda(), None, _abi.default_opts(precision=_abi.MF_PREC_FP32))
+        torch.cuda.synchronize()
+        assert torch.equal(r.x, q.x) and torch.equal(r.flags, q.flags)
+        assert torch.equal(r.chi2, q.chi2) and torch.equal(r.iters, q.iters)
+        if n >= 5:
+            assert r.flags[1].item() & _abi.FLAGS["nonfinite"]
