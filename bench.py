@@ -39,6 +39,15 @@ sys.path.insert(0, str(ROOT / "tests"))
 
 PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
 
+# HBM bytes per launch of a warm step's fit kernels (k_fast_p1 + k_fast_p2)
+# on the default workload, from two rocprofv3 --pmc runs of
+# `bench.py --traffic-probe`: FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, the
+# factors that make the counters read a 256 MiB device copy as 256 MiB each
+# way.  Other workloads report null.
+MEASURED_TRAFFIC = {("c2", "zero", 1_000_000): {
+    "bytes": 273.3e6, "how": "rocprofv3 --pmc FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00), "
+                             "bench.py --traffic-probe, copy-calibrated"}}
+
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -118,6 +127,10 @@ def main():
     ap.add_argument("--mlp-weights", default="", help="parameters to load (default: the "
                     "committed weights, else train on 4M c2-distribution channels)")
     ap.add_argument("--save-mlp", default="", help="write the trained parameters here")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="for counter runs (rocprofv3 --pmc): a 256 MiB device copy, the "
+                         "calibration of the byte counters, then --steps steps of the warm and "
+                         "cold fast paths; no timing, no JSON")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the multifit path has no cross-GPU exchange "
@@ -152,6 +165,19 @@ def main():
     log(f"mlp: {mlp_info}")
 
     O = _abi.default_opts
+    if args.traffic_probe:
+        a = torch.ones(64 << 20, dtype=torch.float32, device=dev)   # 256 MiB
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        for name, o in (("warm", O(precision=_abi.MF_PREC_FP32, warm_start=1)),
+                        ("fast", O(precision=_abi.MF_PREC_FP32))):
+            out = eng.alloc_result(nch, _abi.MF_PREC_FP32)
+            for _ in range(args.steps):
+                eng.fit(s32, tt, o, out)
+            torch.cuda.synchronize()
+            log(f"traffic probe: {args.steps} steps of {name}")
+        return
     paths = {
         "plain": (s64, O(precision=_abi.MF_PREC_FP64), _abi.MF_PREC_FP64),
         "fast": (s32, O(precision=_abi.MF_PREC_FP32), _abi.MF_PREC_FP32),
@@ -209,7 +235,11 @@ def main():
                    "samples": 10, "pulses": 10, "pulse_cov": args.pulse_cov,
                    "path": "warm-start fast path (MLP + FP32 active set + FP64 polish)"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak,
+                     "traffic": (MEASURED_TRAFFIC.get((args.config, args.pulse_cov, nch)) or
+                                 {}).get("bytes"),
+                     "traffic_how": (MEASURED_TRAFFIC.get((args.config, args.pulse_cov, nch)) or
+                                     {}).get("how"),
                      "kernel": "k_fast_p1 + k_fast_p2 (the step's fit kernels)",
                      "kernel_ms": kern_ms,
                      "aflops_per_channel": af},
