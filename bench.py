@@ -112,6 +112,103 @@ def errors(x, chi2, xr, chi2r):
             "identical_active_sets": float(aset.mean()), "channels": int(x.shape[1])}
 
 
+def event_stream(args, eng, w, log):
+    """Event stream: `events` events back to back, each one fit call on one
+    event's channels (both types, given as channel ranges), replayed from a
+    captured graph; per-event latency from HIP events around each replay.
+    A pool of `event_pool` distinct generated events stays resident in HBM
+    and is cycled (event i uses pool slot i mod pool)."""
+    import torch
+    from test_regression_amd import _abi, warmstart
+    P, E = args.event_pool, args.events
+    nev = sum(w.event)
+    t0 = time.perf_counter()
+    s, _, _ = w.generate(events=P, nthreads=min(16, os.cpu_count() or 1))
+    log(f"generated {P} events of {nev} channels in {time.perf_counter() - t0:.1f}s")
+    pool = torch.from_numpy(np.ascontiguousarray(
+        s.reshape(10, P, nev).transpose(1, 0, 2)).astype(np.float32)).cuda()
+    ranges = np.concatenate([[0], np.cumsum(w.event)]).tolist()   # barrel, then endcap
+    wpath = Path(args.mlp_weights) if args.mlp_weights else warmstart.WEIGHTS
+    warmstart.ensure_trained(eng, path=wpath if wpath.exists() else None, log=log)
+    o = _abi.default_opts(precision=_abi.MF_PREC_FP32, warm_start=1)
+    outs = [eng.alloc_result(nev, _abi.MF_PREC_FP32) for _ in range(P)]
+    graphs = [eng.capture(pool[e], None, o, outs[e], type_ranges=ranges) for e in range(P)]
+    stream = torch.cuda.current_stream()
+
+    def run(launch):
+        for i in range(min(P, E)):
+            launch(i)
+        torch.cuda.synchronize()
+        st = [torch.cuda.Event(enable_timing=True) for _ in range(E)]
+        en = [torch.cuda.Event(enable_timing=True) for _ in range(E)]
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for i in range(E):
+            st[i].record(stream)
+            launch(i)
+            en[i].record(stream)
+        b.record(stream)
+        torch.cuda.synchronize()
+        lat = np.array([x.elapsed_time(y) for x, y in zip(st, en)])
+        return a.elapsed_time(b), lat
+
+    total, lat = run(lambda i: graphs[i % P].launch())
+    total_e, lat_e = run(lambda i: eng.fit(pool[i % P], None, o, outs[i % P], type_ranges=ranges))
+    log(f"graph: {total:.1f} ms for {E} events, p50 {np.median(lat):.4f} ms, "
+        f"p99 {np.quantile(lat, 0.99):.4f} ms; eager: {total_e:.1f} ms")
+    # accuracy of one event against the plain FP64 fit
+    ref = eng.fit(pool[0].double(), None, _abi.default_opts(precision=_abi.MF_PREC_FP64),
+                  type_ranges=ranges)
+    graphs[0].launch()
+    torch.cuda.synchronize()
+    acc = errors(outs[0].x.double().cpu().numpy(), outs[0].chi2.double().cpu().numpy(),
+                 ref.x.cpu().numpy(), ref.chi2.cpu().numpy())
+    # kernel time of an eager event (library events), for the roofline
+    eng.set_timing(True)
+    kern = []
+    for i in range(5):
+        eng.fit(pool[i % P], None, o, outs[i % P], type_ranges=ranges)
+        kern.append(eng.timings())
+    eng.set_timing(False)
+    for g in graphs:
+        g.close()
+    fit_ms = float(np.median([k["fit_ms"] for k in kern]))
+    ct = w.chtypes()
+    sample = np.ascontiguousarray(s[:, :nev]).astype(np.float32).astype(np.float64)
+    orc = oracle_sample(ct, sample, w.event_types(), nev)   # the whole event, both types
+    af = float(orc["aflops"].mean())
+    achieved = af * nev / (fit_ms / 1e3) / 1e12
+    value = E * nev / (total / 1e3)
+    line = {
+        "metric": "fitted channels/sec (10 samples x 10 BX), warm-start FP32 vs plain FP64, same GPU",
+        "value": value, "unit": "channels/s", "n_gpus": 1, "steps": E, "warmup": min(P, E),
+        "ms_per_step": total / E, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"c5: {E} events x {nev} channels ({w.event[0]} barrel PU40, "
+                               f"{w.event[1]} endcap PU80), warm-start FP32 path, one "
+                               f"graph-captured fit call per event",
+                   "events": E, "channels_per_event": nev, "event_pool": P},
+        "latency_ms": {"p50": float(np.median(lat)), "p99": float(np.quantile(lat, 0.99)),
+                       "mean": float(lat.mean()), "max": float(lat.max())},
+        "eager": {"channels_per_s": E * nev / (total_e / 1e3),
+                  "latency_ms_p50": float(np.median(lat_e)),
+                  "latency_ms_p99": float(np.quantile(lat_e, 0.99))},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["f32"],
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f32"], "traffic": None,
+                     "kernel": "k_fast_p1 + k_fast_p2, both types", "kernel_ms": fit_ms,
+                     "aflops_per_channel": af},
+        "accuracy_vs_plain_fp64": acc,
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        rate, n, done, dt = cpu_baseline(ct, sample, w.event_types(), args.cpu_seconds, threads)
+        line["cpu_baseline"] = {"value": rate, "unit": "channels/s", "cores": threads,
+                                "kind": "port",
+                                "sample": f"one event ({n} channels) fitted {done // n} times "
+                                          f"({dt:.1f} s) by the oracle, {threads} threads"}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +224,9 @@ def main():
     ap.add_argument("--mlp-weights", default="", help="parameters to load (default: the "
                     "committed weights, else train on 4M c2-distribution channels)")
     ap.add_argument("--save-mlp", default="", help="write the trained parameters here")
+    ap.add_argument("--events", type=int, default=2000, help="event stream (c5): events")
+    ap.add_argument("--event-pool", type=int, default=64,
+                    help="event stream (c5): distinct events resident in HBM, cycled")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for counter runs (rocprofv3 --pmc): a 256 MiB device copy, the "
                          "calibration of the byte counters, then --steps steps of the warm and "
@@ -143,6 +243,9 @@ def main():
     from test_regression_amd import _abi, engine, warmstart, workloads
 
     w = workloads.config(args.config, args.pulse_cov)
+    if w.event:
+        event_stream(args, engine.Engine(list(w.chtypes())), w, log)
+        return
     nch = args.nch or w.nch
     ct = w.chtypes()
     t0 = time.perf_counter()

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define MF_ABI_VERSION 1
+#define MF_ABI_VERSION 2
 #define MF_MAX_N 10   /* samples and pulses per channel */
 #define MF_FULL 19    /* FullSampleVector length (EigenMatrixTypes.h:7) */
 
@@ -101,18 +101,26 @@ enum {
     MF_F_NONFINITE    = 8,   /* non-finite input sample: not fitted, x = 0    */
     MF_F_FP64_PIVOT   = 16,  /* FP32 path: a pivot was recomputed in FP64     */
     MF_F_WARM_REJECT  = 32,  /* warm start failed its KKT check: cold refit   */
-    MF_F_REFINED      = 64   /* FP32 path: FP64 refinement step applied       */
+    MF_F_REFINED      = 64,  /* FP32 path: FP64 refinement step applied       */
+    MF_F_BAD_TYPE     = 128  /* type_of[c] >= ntypes: not fitted, x = 0       */
 };
 
 /* Device-resident batch, structure of arrays.  samples[i*nch + c] for sample
- * i of channel c (dtype MF_DTYPE_*).  type_of[c] indexes the table passed to
- * mf_set_types (NULL: every channel is type 0). */
+ * i of channel c (dtype MF_DTYPE_*).  The channel types, one of:
+ *   type_of (device, one byte per channel): index into the table passed to
+ *     mf_set_types; the batch is partitioned by type on the device;
+ *   ranges (host, nranges + 1 ascending channel numbers): channels
+ *     [ranges[r], ranges[r+1]) are of type r -- the layout of a readout
+ *     ordered by subdetector; no partition is needed and one launch covers
+ *     every range of the same shape (type_of must then be NULL);
+ *   neither: every channel is type 0. */
 typedef struct mf_batch {
     int64_t        nch;
     int32_t        sample_dtype;
-    int32_t        pad;
+    int32_t        nranges;
     const void    *samples;
     const uint8_t *type_of;
+    const int64_t *ranges;
 } mf_batch;
 
 /* Device-resident results, structure of arrays, in the compute precision
@@ -144,10 +152,28 @@ mf_status mf_default_opts(mf_opts *opts);
 
 /* Fits every channel of the batch: DoFit, by default with error computation
  * disabled as every reference driver runs it (multifit_cpu/src/multifit.cpp:200,
- * multifit_gpu/src/PulseChiSqSNNLS.cu:234).  Asynchronous on `stream`, except
- * that a batch mixing channel types synchronizes once to size its partition. */
+ * multifit_gpu/src/PulseChiSqSNNLS.cu:234).  Asynchronous on `stream`, with no
+ * host round trip (a batch with type_of is partitioned by type on the device),
+ * so that a call can be captured in a graph once its buffers exist (one call
+ * of the same size beforehand allocates them). */
 mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_result *out,
                        const mf_opts *opts, void *stream);
+
+/* A fit call captured in a HIP graph, for launch-bound repeats such as one
+ * event after another (no counterpart in the reference).  mf_graph_capture
+ * runs the call once eagerly (which sizes the context's scratch and checks
+ * the arguments), then records it; each mf_graph_launch repeats it on the
+ * same buffers, reading their current contents.  The buffers must outlive
+ * the graph, and the graphs must be destroyed before their context.  Graphs
+ * of one context share its scratch: do not run two of them, or a graph and
+ * an eager call, at the same time; while a graph exists, a call that would
+ * grow the scratch that graph uses (more channels than any call before) is
+ * refused. */
+typedef struct mf_graph mf_graph;
+mf_status mf_graph_capture(mf_context *ctx, const mf_batch *in, const mf_result *out,
+                           const mf_opts *opts, void *stream, mf_graph **graph);
+mf_status mf_graph_launch(mf_graph *graph, void *stream);
+mf_status mf_graph_destroy(mf_graph *graph);
 
 /* Opt-in timing of mf_fit_batch's kernels with HIP events recorded on the
  * launch stream (no counterpart in the reference; used by the benchmark).

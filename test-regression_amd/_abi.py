@@ -23,7 +23,7 @@ MF_MLP_NPARAM = 6164
 
 FLAGS = {
     "minimize_cap": 1, "llt_fail": 2, "fnnls_cap": 4, "nonfinite": 8,
-    "fp64_pivot": 16, "warm_reject": 32, "refined": 64,
+    "fp64_pivot": 16, "warm_reject": 32, "refined": 64, "bad_type": 128,
 }
 
 
@@ -49,8 +49,8 @@ class Opts(C.Structure):
 
 class Batch(C.Structure):
     _fields_ = [
-        ("nch", C.c_int64), ("sample_dtype", C.c_int32), ("pad", C.c_int32),
-        ("samples", C.c_void_p), ("type_of", C.c_void_p),
+        ("nch", C.c_int64), ("sample_dtype", C.c_int32), ("nranges", C.c_int32),
+        ("samples", C.c_void_p), ("type_of", C.c_void_p), ("ranges", C.c_void_p),
     ]
 
 
@@ -115,6 +115,10 @@ SIGNATURES = [
     ("mf_set_timing", C.c_int, [C.c_void_p, C.c_int32]),
     ("mf_get_timings", C.c_int, [C.c_void_p, C.POINTER(Timings)]),
     ("mf_get_tune", C.c_int, [C.c_void_p, C.POINTER(Tune)]),
+    ("mf_graph_capture", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.POINTER(C.c_void_p)]),
+    ("mf_graph_launch", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("mf_graph_destroy", C.c_int, [C.c_void_p]),
     ("mf_set_tune", C.c_int, [C.c_void_p, C.POINTER(Tune)]),
     ("mf_gen_type", C.c_int, [C.POINTER(GenSpec), C.POINTER(ChType)]),
     ("mf_gen_samples", C.c_int, [C.POINTER(GenSpec), C.c_int32, C.c_void_p, C.c_int64,

@@ -188,7 +188,7 @@ struct FitParams {
 
 enum {
     F_MINIMIZE_CAP = 1, F_LLT_FAIL = 2, F_FNNLS_CAP = 4, F_NONFINITE = 8,
-    F_FP64_PIVOT = 16, F_WARM_REJECT = 32, F_REFINED = 64
+    F_FP64_PIVOT = 16, F_WARM_REJECT = 32, F_REFINED = 64, F_BAD_TYPE = 128
 };
 
 struct FnnlsStats {

@@ -58,19 +58,70 @@ template <typename T> struct LdsPark {
     __device__ __forceinline__ T load(int k) const { return base[k * stride]; }
 };
 
+// The channels one launch fits: positions 0..count-1 of a list (idx; null:
+// channel = position).  A list built on the device (the type partition)
+// gives its length and start through device memory, so that launching on it
+// needs no host round trip; n is then an upper bound that sizes the grid.
+struct Seg {
+    const int32_t *idx;
+    const int64_t *dcount;   // device: number of items (null: n)
+    const int64_t *doffset;  // device: first item's position in idx (null: 0)
+    int64_t n;
+};
+__device__ __forceinline__ int64_t seg_count(const Seg &s) { return s.dcount ? *s.dcount : s.n; }
+__device__ __forceinline__ const int32_t *seg_list(const Seg &s)
+{
+    return s.idx ? s.idx + (s.doffset ? *s.doffset : 0) : nullptr;
+}
+
+// What one launch of a fit kernel covers: either one list of one type
+// (nrange == 0), or up to MAX_RANGES channel ranges, each of its own type,
+// laid out wave by wave: range r takes waves wave0[r] .. wave0[r+1]-1, so
+// that every wavefront holds channels of one type (its tables stay
+// wave-uniform) and one launch keeps the whole chip busy across types.
+constexpr int MAX_RANGES = 8;
+struct SegArg {
+    int nrange;
+    int type_id;   // single list: its type
+    Seg list;      // single list
+    int rtype[MAX_RANGES];
+    int64_t rfirst[MAX_RANGES], rcount[MAX_RANGES], wave0[MAX_RANGES + 1];
+};
+// This thread's channel c and type t; false: no channel.  t is wave-uniform.
+__device__ __forceinline__ bool seg_item(const SegArg &a, int &t, int64_t &c)
+{
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a.nrange == 0) {
+        t = a.type_id;
+        if (gt >= seg_count(a.list)) return false;
+        const int32_t *idx = seg_list(a.list);
+        c = idx ? (int64_t)idx[gt] : gt;
+        return true;
+    }
+    const int64_t w = gt >> 6;
+    int r = 0;
+#pragma unroll
+    for (int k = 1; k < MAX_RANGES; ++k)
+        if (k < a.nrange && w >= a.wave0[k]) r = k;
+    r = __builtin_amdgcn_readfirstlane(r);
+    t = __builtin_amdgcn_readfirstlane(a.rtype[r]);
+    const int64_t g = (w - a.wave0[r]) * 64 + (gt & 63);
+    if (w >= a.wave0[a.nrange] || g >= a.rcount[r]) return false;
+    c = a.rfirst[r] + g;
+    return true;
+}
+
 template <typename T, typename TS, int N, bool ERRORS>
 __global__ __launch_bounds__(256)
-void k_fit_plain(const DevType *__restrict__ types, int type_id, const int32_t *__restrict__ idx,
-                 const int64_t *__restrict__ count, int64_t nitems,
+void k_fit_plain(const DevType *__restrict__ types, SegArg seg,
                  const TS *__restrict__ samples, int64_t nch, T *__restrict__ xo,
                  T *__restrict__ chi2o, int32_t *__restrict__ flo, int32_t *__restrict__ ito,
                  T *__restrict__ erro, FitParams prm)
 {
     extern __shared__ unsigned char lds_raw[];
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t n = count ? *count : nitems;
-    if (g >= n) return;
-    const int64_t c = idx ? (int64_t)idx[g] : g;
+    int type_id;
+    int64_t c;
+    if (!seg_item(seg, type_id, c)) return;
     const TypeMats<T> &tm = mats<T>(types[type_id]);
     T s[10];
 #pragma unroll
@@ -122,14 +173,14 @@ __device__ __forceinline__ uint32_t warm_load(const WarmIn &w, int64_t c, int64_
 
 template <typename TS, int N>
 __global__ __launch_bounds__(256)
-void k_fast_p1(const DevType *__restrict__ types, int type_id, const int32_t *__restrict__ idx,
-               int64_t nitems, const TS *__restrict__ samples, int64_t nch,
+void k_fast_p1(const DevType *__restrict__ types, SegArg seg,
+               const TS *__restrict__ samples, int64_t nch,
                float *__restrict__ xo, int32_t *__restrict__ flo, int32_t *__restrict__ ito,
                WarmIn warm, FitParams prm)
 {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nitems) return;
-    const int64_t c = idx ? (int64_t)idx[g] : g;
+    int type_id;
+    int64_t c;
+    if (!seg_item(seg, type_id, c)) return;
     float s[10], x[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) s[i] = i < N ? (float)samples[(int64_t)i * nch + c] : 0.0f;
@@ -146,15 +197,15 @@ void k_fast_p1(const DevType *__restrict__ types, int type_id, const int32_t *__
 
 template <typename TS, int N, bool LIVE>
 __global__ __launch_bounds__(256)
-void k_fast_p2(const DevType *__restrict__ types, int type_id, const int32_t *__restrict__ idx,
-               int64_t nitems, const TS *__restrict__ samples, int64_t nch,
+void k_fast_p2(const DevType *__restrict__ types, SegArg seg,
+               const TS *__restrict__ samples, int64_t nch,
                float *__restrict__ xo, float *__restrict__ chi2o, int32_t *__restrict__ flo,
                int32_t *__restrict__ ito, int warm, FitParams prm)
 {
     extern __shared__ unsigned char lds_raw[];
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nitems) return;
-    const int64_t c = idx ? (int64_t)idx[g] : g;
+    int type_id;
+    int64_t c;
+    if (!seg_item(seg, type_id, c)) return;
     float s[10], x[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) {
@@ -263,12 +314,14 @@ __device__ __forceinline__ uint32_t warm_start_state(uint32_t pm, const float (&
 // of them are empty.
 template <typename TS, int N>
 __global__ __launch_bounds__(256)
-void k_fast_p1q(const DevType *__restrict__ types, int type_id, const int32_t *__restrict__ idx,
-                int64_t nitems, const TS *__restrict__ samples, int64_t nch,
+void k_fast_p1q(const DevType *__restrict__ types, int type_id, Seg seg,
+                const TS *__restrict__ samples, int64_t nch,
                 float *__restrict__ xo, int32_t *__restrict__ flo, int32_t *__restrict__ ito,
                 WarmIn warm, FitParams prm, unsigned *__restrict__ head, int need)
 {
     const DevType &ty = types[type_id];
+    const int64_t nitems = seg_count(seg);
+    const int32_t *idx = seg_list(seg);
     int64_t c = -1, cn = -1;
     float Atb[10], svn[10], xn[10];
     uint32_t pmn = 0;
@@ -335,13 +388,15 @@ void k_fast_p1q(const DevType *__restrict__ types, int type_id, const int32_t *_
 // are loaded while the current one is worked on.
 template <typename TS, int N>
 __global__ __launch_bounds__(256)
-void k_fast_p2q(const DevType *__restrict__ types, int type_id, const int32_t *__restrict__ idx,
-                int64_t nitems, const TS *__restrict__ samples, int64_t nch,
+void k_fast_p2q(const DevType *__restrict__ types, int type_id, Seg seg,
+                const TS *__restrict__ samples, int64_t nch,
                 float *__restrict__ xo, float *__restrict__ chi2o, int32_t *__restrict__ flo,
                 int32_t *__restrict__ ito, int warm, FitParams prm, unsigned *__restrict__ head,
                 int need)
 {
     const DevType &ty = types[type_id];
+    const int64_t nitems = seg_count(seg);
+    const int32_t *idx = seg_list(seg);
     int64_t c = -1, cn = -1;
     double Atb[10];
     ActiveState<double> S;
@@ -415,8 +470,16 @@ __global__ void k_zero_u32(unsigned *__restrict__ p, int n)
     if (i < n) p[i] = 0u;
 }
 
-// Partition of a mixed batch by channel type: counts, offsets, index lists.
-// part layout: counts[256] (int64), then cursors[256] (int64).
+// Partition of a mixed batch by channel type, on the device only (no host
+// round trip, so a fit call can be captured in a graph): counts per type,
+// their exclusive scan, and the index list of each type.  part layout:
+// counts[256], cursors[256], offsets[256] (64-bit).
+__global__ void k_zero_u64(unsigned long long *__restrict__ p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
 __global__ void k_type_count(const uint8_t *__restrict__ type_of, int64_t nch,
                              unsigned long long *__restrict__ counts)
 {
@@ -431,15 +494,60 @@ __global__ void k_type_count(const uint8_t *__restrict__ type_of, int64_t nch,
         if (loc[t]) atomicAdd(&counts[t], (unsigned long long)loc[t]);
 }
 
-__global__ void k_type_scatter(const uint8_t *__restrict__ type_of, int64_t nch,
-                               const int64_t *__restrict__ offsets,
-                               unsigned long long *__restrict__ cursors, int32_t *__restrict__ idx)
+// one block of 256 threads: offsets[t] = sum of counts[u], u < t
+__global__ void k_type_scan(const long long *__restrict__ counts, long long *__restrict__ offsets)
 {
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nch;
-         c += (int64_t)gridDim.x * blockDim.x) {
-        const int t = type_of[c];
-        const unsigned long long k = atomicAdd(&cursors[t], 1ull);
-        idx[offsets[t] + (int64_t)k] = (int32_t)c;
+    __shared__ long long v[256];
+    const int t = threadIdx.x;
+    v[t] = counts[t];
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        const long long a = t >= d ? v[t - d] : 0;
+        __syncthreads();
+        v[t] += a;
+        __syncthreads();
+    }
+    offsets[t] = v[t] - counts[t];
+}
+
+// Index lists of the types; a channel whose type id is not set (>= ntypes)
+// is not fitted: x = 0, chi2 = 0, flags = F_BAD_TYPE.  Slots are claimed per
+// wavefront: for each type present among its lanes, one atomic add on that
+// type's cursor, and the lanes of the type write at their rank (a cursor
+// shared by every channel would serialize the whole batch on one address).
+template <typename T>
+__global__ void k_type_scatter(const uint8_t *__restrict__ type_of, int64_t nch, int ntypes,
+                               const long long *__restrict__ offsets,
+                               unsigned long long *__restrict__ cursors, int32_t *__restrict__ idx,
+                               T *__restrict__ xo, T *__restrict__ chi2o,
+                               int32_t *__restrict__ flo, int32_t *__restrict__ ito)
+{
+    const int lane = __lane_id();
+    for (int64_t c0 = (int64_t)blockIdx.x * blockDim.x; c0 < nch;
+         c0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = c0 + threadIdx.x;
+        int t = c < nch ? (int)type_of[c] : -1;
+        if (t >= ntypes) {
+            for (int i = 0; i < 10; ++i) xo[(int64_t)i * nch + c] = T(0);
+            chi2o[c] = T(0);
+            flo[c] = F_BAD_TYPE;
+            if (ito) ito[c] = 0;
+            t = -1;
+        }
+        bool pending = t >= 0;
+        for (uint64_t pmask = __ballot(pending); pmask; pmask = __ballot(pending)) {
+            const int lead = __ffsll((unsigned long long)pmask) - 1;
+            const int tt = __shfl(t, lead);
+            const uint64_t mine = __ballot(pending && t == tt);
+            unsigned long long base = 0;
+            if (lane == lead) base = atomicAdd(&cursors[tt], (unsigned long long)__popcll(mine));
+            base = __shfl(base, lead);
+            if (pending && t == tt) {
+                const int r = __popcll(mine & ((1ull << lane) - 1ull));
+                idx[offsets[tt] + (long long)(base + (unsigned long long)r)] = (int32_t)c;
+                pending = false;
+            }
+        }
     }
 }
 
@@ -636,6 +744,9 @@ struct mf_context {
     mf::mlp::Scratch mlp_w;
     float *pred = nullptr;  // warm-start prediction: x [10][nch], then pm [nch] (uint32)
     int64_t pred_cap = 0;
+    // live mf_graph objects whose kernels point into the prediction / the
+    // partition scratch (that scratch must not move while they exist)
+    int graphs_pred = 0, graphs_part = 0;
     // persistent kernels: compute units, queue heads, launch shape
     int cus = 0;
     unsigned *d_queue = nullptr;  // QUEUE_SLOTS heads
@@ -764,110 +875,127 @@ extern "C" mf_status mf_set_types(mf_context *ctx, const mf_chtype *types, int32
     return MF_OK;
 }
 
+static int64_t seg_threads(const SegArg &a) { return a.nrange ? a.wave0[a.nrange] * 64 : a.list.n; }
+static int seg_type0(const SegArg &a) { return a.nrange ? a.rtype[0] : a.type_id; }
+
 template <typename T, typename TS, int N>
-static mf_status launch_fit(mf_context *ctx, int type_id, const int32_t *idx,
-                            const int64_t *count, int64_t nitems, const void *samples,
-                            int64_t nch, const mf_result *out, const FitParams &prm, int block,
-                            bool errors, hipStream_t st)
+static mf_status launch_fit(mf_context *ctx, const SegArg &seg, const void *samples, int64_t nch,
+                            const mf_result *out, const FitParams &prm, int block, bool errors,
+                            hipStream_t st)
 {
-    if (nitems <= 0) return MF_OK;
-    const int64_t grid = (nitems + block - 1) / block;
+    const int64_t nt = seg_threads(seg);
+    if (nt <= 0) return MF_OK;
+    const int64_t grid = (nt + block - 1) / block;
     const size_t lds = sizeof(T) * 55 * (size_t)block;
     if (errors)
         hipLaunchKernelGGL((k_fit_plain<T, TS, N, true>), dim3((unsigned)grid), dim3(block), lds,
-                           st, ctx->d_types, type_id, idx, count, nitems,
-                           static_cast<const TS *>(samples), nch, static_cast<T *>(out->x),
-                           static_cast<T *>(out->chi2), out->flags, out->iters,
-                           static_cast<T *>(out->err), prm);
+                           st, ctx->d_types, seg, static_cast<const TS *>(samples), nch,
+                           static_cast<T *>(out->x), static_cast<T *>(out->chi2), out->flags,
+                           out->iters, static_cast<T *>(out->err), prm);
     else
         hipLaunchKernelGGL((k_fit_plain<T, TS, N, false>), dim3((unsigned)grid), dim3(block),
-                           lds, st, ctx->d_types, type_id, idx, count, nitems,
-                           static_cast<const TS *>(samples), nch, static_cast<T *>(out->x),
-                           static_cast<T *>(out->chi2), out->flags, out->iters,
-                           static_cast<T *>(out->err), prm);
+                           lds, st, ctx->d_types, seg, static_cast<const TS *>(samples), nch,
+                           static_cast<T *>(out->x), static_cast<T *>(out->chi2), out->flags,
+                           out->iters, static_cast<T *>(out->err), prm);
     HIP_TRY(hipGetLastError());
     return MF_OK;
 }
 
 template <typename TS, int N, bool LIVE>
-static mf_status launch_fast(mf_context *ctx, int type_id, const int32_t *idx, int64_t nitems,
-                             const void *samples, int64_t nch, const mf_result *out,
-                             const FitParams &prm, int block, WarmIn warm, bool persistent,
-                             hipStream_t st)
+static mf_status launch_fast(mf_context *ctx, const SegArg &seg, const void *samples,
+                             int64_t nch, const mf_result *out, const FitParams &prm, int block,
+                             WarmIn warm, bool persistent, hipStream_t st)
 {
-    if (nitems <= 0) return MF_OK;
+    const int64_t nt = seg_threads(seg);
+    if (nt <= 0) return MF_OK;
     if (!out->iters) return fail(MF_ERR_INVALID, "the FP32 path needs mf_result.iters");
-    if (persistent && !LIVE) {
+    if (persistent && !LIVE && seg.nrange == 0) {
         const FastTune &tu = ctx->tune;
         const int wpb = tu.block / 64;
-        const int64_t need_w = (nitems + 63) / 64;
+        const int64_t need_w = (nt + 63) / 64;
         const int64_t g1 = std::max<int64_t>(1, std::min<int64_t>(need_w, (int64_t)ctx->cus * tu.p1_wpc) / wpb);
         const int64_t g2 = std::max<int64_t>(1, std::min<int64_t>(need_w, (int64_t)ctx->cus * tu.p2_wpc) / wpb);
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, ctx->d_queue, 2);
         hipLaunchKernelGGL((k_fast_p1q<TS, N>), dim3((unsigned)g1), dim3(tu.block), 0, st,
-                           ctx->d_types, type_id, idx, nitems, static_cast<const TS *>(samples),
+                           ctx->d_types, seg.type_id, seg.list, static_cast<const TS *>(samples),
                            nch, static_cast<float *>(out->x), out->flags, out->iters, warm, prm,
                            ctx->d_queue, tu.p1_need);
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL((k_fast_p2q<TS, N>), dim3((unsigned)g2), dim3(tu.block), 0, st,
-                           ctx->d_types, type_id, idx, nitems, static_cast<const TS *>(samples),
+                           ctx->d_types, seg.type_id, seg.list, static_cast<const TS *>(samples),
                            nch, static_cast<float *>(out->x), static_cast<float *>(out->chi2),
                            out->flags, out->iters, warm.pm ? 1 : 0, prm, ctx->d_queue + 1,
                            tu.p2_need);
         HIP_TRY(hipGetLastError());
         return MF_OK;
     }
-    const int64_t grid = (nitems + block - 1) / block;
+    const int64_t grid = (nt + block - 1) / block;
     hipLaunchKernelGGL((k_fast_p1<TS, N>), dim3((unsigned)grid), dim3(block), 0, st,
-                       ctx->d_types, type_id, idx, nitems, static_cast<const TS *>(samples), nch,
+                       ctx->d_types, seg, static_cast<const TS *>(samples), nch,
                        static_cast<float *>(out->x), out->flags, out->iters, warm, prm);
     HIP_TRY(hipGetLastError());
     const size_t lds = LIVE ? sizeof(double) * 55 * (size_t)block : 0;
     hipLaunchKernelGGL((k_fast_p2<TS, N, LIVE>), dim3((unsigned)grid), dim3(block), lds, st,
-                       ctx->d_types, type_id, idx, nitems, static_cast<const TS *>(samples), nch,
+                       ctx->d_types, seg, static_cast<const TS *>(samples), nch,
                        static_cast<float *>(out->x), static_cast<float *>(out->chi2), out->flags,
                        out->iters, warm.pm ? 1 : 0, prm);
     HIP_TRY(hipGetLastError());
     return MF_OK;
 }
 
-template <typename TS>
-static mf_status dispatch_fast(mf_context *ctx, int type_id, const int32_t *idx, int64_t nitems,
-                               const mf_batch *in, const mf_result *out, const FitParams &prm,
-                               int block, WarmIn warm, bool persistent, hipStream_t st)
+// Launch shape of a type: samples per channel and whether its pulse
+// covariance is live (one launch covers types of the same shape).
+static bool type_live(const mf_chtype &t, const FitParams &prm)
 {
-    const mf_chtype &t = ctx->types[(size_t)type_id];
     bool has_pi = false;
     for (int i = 0; i < MF_FULL * MF_FULL && !has_pi; ++i) has_pi = t.fullpulsecov[i] != 0.0;
-    const bool live = has_pi && prm.use_pi;
+    return has_pi && prm.use_pi;
+}
+
+template <typename TS>
+static mf_status dispatch_fast(mf_context *ctx, const SegArg &seg, const mf_batch *in,
+                               const mf_result *out, const FitParams &prm, int block,
+                               WarmIn warm, bool persistent, hipStream_t st)
+{
+    const mf_chtype &t = ctx->types[(size_t)seg_type0(seg)];
+    const bool live = type_live(t, prm);
     if (t.nsamples == 10)
-        return live ? launch_fast<TS, 10, true>(ctx, type_id, idx, nitems, in->samples, in->nch,
-                                                out, prm, block, warm, persistent, st)
-                    : launch_fast<TS, 10, false>(ctx, type_id, idx, nitems, in->samples, in->nch,
-                                                 out, prm, block, warm, persistent, st);
-    return live ? launch_fast<TS, 8, true>(ctx, type_id, idx, nitems, in->samples, in->nch, out,
-                                           prm, block, warm, persistent, st)
-                : launch_fast<TS, 8, false>(ctx, type_id, idx, nitems, in->samples, in->nch, out,
-                                            prm, block, warm, persistent, st);
+        return live ? launch_fast<TS, 10, true>(ctx, seg, in->samples, in->nch, out, prm, block,
+                                                warm, persistent, st)
+                    : launch_fast<TS, 10, false>(ctx, seg, in->samples, in->nch, out, prm, block,
+                                                 warm, persistent, st);
+    return live ? launch_fast<TS, 8, true>(ctx, seg, in->samples, in->nch, out, prm, block, warm,
+                                           persistent, st)
+                : launch_fast<TS, 8, false>(ctx, seg, in->samples, in->nch, out, prm, block, warm,
+                                            persistent, st);
 }
 
 template <typename T>
-static mf_status dispatch_fit(mf_context *ctx, int type_id, const int32_t *idx,
-                              const int64_t *count, int64_t nitems, const mf_batch *in,
+static mf_status dispatch_fit(mf_context *ctx, const SegArg &seg, const mf_batch *in,
                               const mf_result *out, const FitParams &prm, int block, bool errors,
                               hipStream_t st)
 {
-    const int ns = ctx->types[(size_t)type_id].nsamples;
+    const int ns = ctx->types[(size_t)seg_type0(seg)].nsamples;
     const bool f64in = in->sample_dtype == MF_DTYPE_F64;
     if (ns == 10)
-        return f64in ? launch_fit<T, double, 10>(ctx, type_id, idx, count, nitems, in->samples,
-                                                 in->nch, out, prm, block, errors, st)
-                     : launch_fit<T, float, 10>(ctx, type_id, idx, count, nitems, in->samples,
-                                                in->nch, out, prm, block, errors, st);
-    return f64in ? launch_fit<T, double, 8>(ctx, type_id, idx, count, nitems, in->samples,
-                                            in->nch, out, prm, block, errors, st)
-                 : launch_fit<T, float, 8>(ctx, type_id, idx, count, nitems, in->samples, in->nch,
-                                           out, prm, block, errors, st);
+        return f64in ? launch_fit<T, double, 10>(ctx, seg, in->samples, in->nch, out, prm, block,
+                                                 errors, st)
+                     : launch_fit<T, float, 10>(ctx, seg, in->samples, in->nch, out, prm, block,
+                                                errors, st);
+    return f64in ? launch_fit<T, double, 8>(ctx, seg, in->samples, in->nch, out, prm, block,
+                                            errors, st)
+                 : launch_fit<T, float, 8>(ctx, seg, in->samples, in->nch, out, prm, block,
+                                           errors, st);
+}
+
+static SegArg seg_single(int type_id, const Seg &list)
+{
+    SegArg a;
+    memset(&a, 0, sizeof(a));
+    a.nrange = 0;
+    a.type_id = type_id;
+    a.list = list;
+    return a;
 }
 
 extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_result *out,
@@ -882,6 +1010,17 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
         return fail(MF_ERR_INVALID, "samples, x, chi2 and flags are required");
     if (in->sample_dtype != MF_DTYPE_F64 && in->sample_dtype != MF_DTYPE_F32)
         return fail(MF_ERR_INVALID, "sample_dtype must be MF_DTYPE_F64 or MF_DTYPE_F32");
+    if (in->nranges < 0 || in->nranges > (int32_t)ctx->types.size())
+        return fail(MF_ERR_INVALID, "nranges must be in 0..ntypes");
+    if (in->nranges > 0) {
+        if (in->type_of) return fail(MF_ERR_INVALID, "give type_of or ranges, not both");
+        if (!in->ranges) return fail(MF_ERR_INVALID, "ranges is NULL");
+        if (in->ranges[0] != 0 || in->ranges[in->nranges] != in->nch)
+            return fail(MF_ERR_INVALID, "ranges must start at 0 and end at nch");
+        for (int r = 0; r < in->nranges; ++r)
+            if (in->ranges[r + 1] < in->ranges[r])
+                return fail(MF_ERR_INVALID, "ranges must not decrease");
+    }
     if (opts->precision != MF_PREC_FP64 && opts->precision != MF_PREC_FP32)
         return fail(MF_ERR_INVALID, "precision must be MF_PREC_FP64 or MF_PREC_FP32");
     if (opts->max_minimize < 0 || opts->max_iter < 0)
@@ -916,6 +1055,9 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
     WarmIn warm{nullptr, nullptr};
     if (opts->warm_start) {
         if (ctx->pred_cap < in->nch) {
+            if (ctx->graphs_pred)
+                return fail(MF_ERR_INVALID, "the scratch would grow under a captured graph; "
+                                            "destroy the context's graphs first");
             if (ctx->pred) HIP_TRY(hipFree(ctx->pred));
             ctx->pred = nullptr;
             HIP_TRY(hipMalloc(&ctx->pred, sizeof(float) * 11 * (size_t)in->nch));
@@ -938,27 +1080,59 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
     } else {
         ctx->mlp_timed = false;
     }
-    auto go_fit = [&](int type_id, const int32_t *idx, const int64_t *count, int64_t nitems) {
+    auto go_fit = [&](const SegArg &seg) {
         if (opts->precision == MF_PREC_FP64)
-            return dispatch_fit<double>(ctx, type_id, idx, count, nitems, in, out, prm, block,
-                                        opts->compute_errors != 0, st);
+            return dispatch_fit<double>(ctx, seg, in, out, prm, block, opts->compute_errors != 0,
+                                        st);
         const bool pers = opts->persistent != 0;
         return in->sample_dtype == MF_DTYPE_F64
-                   ? dispatch_fast<double>(ctx, type_id, idx, nitems, in, out, prm, block, warm,
-                                           pers, st)
-                   : dispatch_fast<float>(ctx, type_id, idx, nitems, in, out, prm, block, warm,
-                                          pers, st);
+                   ? dispatch_fast<double>(ctx, seg, in, out, prm, block, warm, pers, st)
+                   : dispatch_fast<float>(ctx, seg, in, out, prm, block, warm, pers, st);
     };
     ctx->n_fit_ev = 0;
-    auto go = [&](int type_id, const int32_t *idx, const int64_t *count, int64_t nitems) {
+    auto go = [&](const SegArg &seg) {
         const int k = ctx->n_fit_ev < 256 ? ctx->n_fit_ev++ : 255;
         time_mark(ctx, 2 + 2 * k, st);
-        const mf_status r = go_fit(type_id, idx, count, nitems);
+        const mf_status r = go_fit(seg);
         time_mark(ctx, 3 + 2 * k, st);
         return r;
     };
-    if (!in->type_of || ntypes == 1) return go(0, nullptr, nullptr, in->nch);
-    // mixed batch: partition channel indices by type on the device
+    if (in->nranges > 0) {
+        // type ranges: one launch per shape (samples, live pulse covariance),
+        // up to MAX_RANGES ranges each, every wavefront within one range
+        std::vector<int> pending;
+        for (int r = 0; r < in->nranges; ++r)
+            if (in->ranges[r + 1] > in->ranges[r]) pending.push_back(r);
+        while (!pending.empty()) {
+            const mf_chtype &t0 = ctx->types[(size_t)pending[0]];
+            const bool live0 = type_live(t0, prm);
+            SegArg a;
+            memset(&a, 0, sizeof(a));
+            std::vector<int> rest;
+            for (int r : pending) {
+                const mf_chtype &t = ctx->types[(size_t)r];
+                if (a.nrange < MAX_RANGES && t.nsamples == t0.nsamples && type_live(t, prm) == live0) {
+                    a.rtype[a.nrange] = r;
+                    a.rfirst[a.nrange] = in->ranges[r];
+                    a.rcount[a.nrange] = in->ranges[r + 1] - in->ranges[r];
+                    a.wave0[a.nrange + 1] = a.wave0[a.nrange] + (a.rcount[a.nrange] + 63) / 64;
+                    a.nrange++;
+                } else {
+                    rest.push_back(r);
+                }
+            }
+            const mf_status r = go(a);
+            if (r != MF_OK) return r;
+            pending.swap(rest);
+        }
+        return MF_OK;
+    }
+    if (!in->type_of) return go(seg_single(0, Seg{nullptr, nullptr, nullptr, in->nch}));
+    // mixed batch: partition channel indices by type on the device; each
+    // type's launch reads its count and offset from device memory
+    if ((!ctx->d_part || ctx->idx_cap < in->nch) && ctx->graphs_part)
+        return fail(MF_ERR_INVALID, "the scratch would grow under a captured graph; "
+                                    "destroy the context's graphs first");
     if (!ctx->d_part) HIP_TRY(hipMalloc(&ctx->d_part, sizeof(int64_t) * 768));
     if (ctx->idx_cap < in->nch) {
         if (ctx->d_idx) HIP_TRY(hipFree(ctx->d_idx));
@@ -968,34 +1142,113 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
     }
     int64_t *counts = static_cast<int64_t *>(ctx->d_part);
     int64_t *cursors = counts + 256, *offsets = counts + 512;
-    HIP_TRY(hipMemsetAsync(counts, 0, sizeof(int64_t) * 512, st));
+    hipLaunchKernelGGL(k_zero_u64, dim3(2), dim3(256), 0, st,
+                       reinterpret_cast<unsigned long long *>(counts), 512);
     const int pgrid = (int)std::min<int64_t>(2048, (in->nch + 255) / 256);
     hipLaunchKernelGGL(k_type_count, dim3(pgrid), dim3(256), 0, st, in->type_of, in->nch,
                        reinterpret_cast<unsigned long long *>(counts));
-    HIP_TRY(hipGetLastError());
-    // offsets from counts: ntypes is small; one host round trip keeps it simple
-    int64_t hc[256];
-    HIP_TRY(hipMemcpyAsync(hc, counts, sizeof(int64_t) * 256, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    int64_t ho[256], acc = 0;
-    for (int t = 0; t < 256; ++t) {
-        ho[t] = acc;
-        acc += hc[t];
-        if (hc[t] && t >= ntypes)
-            return fail(MF_ERR_INVALID, "type_of holds type %d but only %d types are set", t,
-                        ntypes);
-    }
-    HIP_TRY(hipMemcpyAsync(offsets, ho, sizeof(int64_t) * 256, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_type_scatter, dim3(pgrid), dim3(256), 0, st, in->type_of, in->nch,
-                       offsets, reinterpret_cast<unsigned long long *>(cursors), ctx->d_idx);
+    hipLaunchKernelGGL(k_type_scan, dim3(1), dim3(256), 0, st,
+                       reinterpret_cast<const long long *>(counts),
+                       reinterpret_cast<long long *>(offsets));
+    if (opts->precision == MF_PREC_FP64)
+        hipLaunchKernelGGL(k_type_scatter<double>, dim3(pgrid), dim3(256), 0, st, in->type_of,
+                           in->nch, ntypes, reinterpret_cast<const long long *>(offsets),
+                           reinterpret_cast<unsigned long long *>(cursors), ctx->d_idx,
+                           static_cast<double *>(out->x), static_cast<double *>(out->chi2),
+                           out->flags, out->iters);
+    else
+        hipLaunchKernelGGL(k_type_scatter<float>, dim3(pgrid), dim3(256), 0, st, in->type_of,
+                           in->nch, ntypes, reinterpret_cast<const long long *>(offsets),
+                           reinterpret_cast<unsigned long long *>(cursors), ctx->d_idx,
+                           static_cast<float *>(out->x), static_cast<float *>(out->chi2),
+                           out->flags, out->iters);
     HIP_TRY(hipGetLastError());
     for (int t = 0; t < ntypes; ++t) {
-        if (!hc[t]) continue;
-        mf_status s = go(t, ctx->d_idx + ho[t], nullptr, hc[t]);
-        if (s != MF_OK) return s;
+        mf_status r = go(seg_single(t, Seg{ctx->d_idx, counts + t, offsets + t, in->nch}));
+        if (r != MF_OK) return r;
     }
-    // hc/ho are host stack arrays: make sure the H2D copy of ho finished
-    HIP_TRY(hipStreamSynchronize(st));
+    return MF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// graph capture of a fit call
+// ---------------------------------------------------------------------------
+struct mf_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int device = 0;
+    mf_context *ctx = nullptr;
+    bool pred = false, part = false;  // scratch its kernels use
+};
+
+extern "C" mf_status mf_graph_capture(mf_context *ctx, const mf_batch *in, const mf_result *out,
+                                      const mf_opts *opts, void *stream, mf_graph **graph)
+{
+    if (!ctx || !graph) return fail(MF_ERR_INVALID, "NULL argument");
+    *graph = nullptr;
+    // eager run: checks the arguments and grows the scratch, so that nothing
+    // is allocated while capturing
+    mf_status r = mf_fit_batch(ctx, in, out, opts, stream);
+    if (r != MF_OK) return r;
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t st = static_cast<hipStream_t>(stream), own = nullptr;
+    if (!st) {  // the null stream cannot be captured
+        HIP_TRY(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+        HIP_TRY(hipDeviceSynchronize());
+        st = own;
+    }
+    const bool timing = ctx->timing;
+    ctx->timing = false;  // no event records inside the graph
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        r = mf_fit_batch(ctx, in, out, opts, st);
+        e = hipStreamEndCapture(st, &g);
+    }
+    ctx->timing = timing;
+    if (own) (void)hipStreamDestroy(own);
+    if (e != hipSuccess || r != MF_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return r != MF_OK ? r : fail(MF_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(MF_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    }
+    mf_graph *h = new mf_graph;
+    h->graph = g;
+    h->exec = x;
+    h->device = ctx->device;
+    h->ctx = ctx;
+    h->pred = opts->warm_start != 0;
+    h->part = in->type_of != nullptr && in->nranges == 0;
+    ctx->graphs_pred += h->pred;
+    ctx->graphs_part += h->part;
+    *graph = h;
+    return MF_OK;
+}
+
+extern "C" mf_status mf_graph_launch(mf_graph *graph, void *stream)
+{
+    if (!graph) return fail(MF_ERR_INVALID, "graph is NULL");
+    HIP_TRY(hipSetDevice(graph->device));
+    HIP_TRY(hipGraphLaunch(graph->exec, static_cast<hipStream_t>(stream)));
+    return MF_OK;
+}
+
+extern "C" mf_status mf_graph_destroy(mf_graph *graph)
+{
+    if (!graph) return MF_OK;
+    (void)hipSetDevice(graph->device);
+    if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+    if (graph->graph) (void)hipGraphDestroy(graph->graph);
+    if (graph->ctx) {
+        graph->ctx->graphs_pred -= graph->pred;
+        graph->ctx->graphs_part -= graph->part;
+    }
+    delete graph;
     return MF_OK;
 }
 

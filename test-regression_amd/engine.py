@@ -38,6 +38,30 @@ def _stream_ptr(stream) -> int | None:
     return stream.cuda_stream
 
 
+class FitGraph:
+    """A captured fit call (mf_graph_*): launch() repeats it on the same
+    buffers; result holds the outputs."""
+
+    def __init__(self, eng, handle, keep):
+        self._eng, self._h, self._keep = eng, handle, keep
+        self.result = keep[2]
+
+    def launch(self, stream=None):
+        _abi.check(self._eng._lib.mf_graph_launch(self._h, C.c_void_p(_stream_ptr(stream))),
+                   "mf_graph_launch")
+
+    def close(self):
+        if self._h:
+            self._eng._lib.mf_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Engine:
     """One device context holding the channel-type table (mf_context)."""
 
@@ -80,7 +104,8 @@ class Engine:
         )
 
     def fit(self, samples: torch.Tensor, type_of: torch.Tensor | None = None, opts=None,
-            out: FitResult | None = None, stream=None) -> FitResult:
+            out: FitResult | None = None, stream=None, capture: bool = False,
+            type_ranges=None):
         """samples: [>=nsamples, nch] float64/float32 on the device (row i = sample i)."""
         if opts is None:
             opts = _abi.default_opts()
@@ -102,14 +127,34 @@ class Engine:
             raise ValueError("samples must be float64 or float32")
         if out is None:
             out = self.alloc_result(nch, opts.precision)
-        b = _abi.Batch(nch=nch, sample_dtype=sdt, pad=0, samples=samples.data_ptr(),
-                       type_of=type_of.data_ptr() if type_of is not None else None)
+        rng = None
+        if type_ranges is not None:
+            # channels [type_ranges[t], type_ranges[t+1]) are of type t
+            rng = (C.c_int64 * len(type_ranges))(*[int(v) for v in type_ranges])
+        b = _abi.Batch(nch=nch, sample_dtype=sdt,
+                       nranges=len(type_ranges) - 1 if type_ranges is not None else 0,
+                       samples=samples.data_ptr(),
+                       type_of=type_of.data_ptr() if type_of is not None else None,
+                       ranges=C.cast(rng, C.c_void_p) if rng is not None else None)
         r = _abi.Result(x=out.x.data_ptr(), chi2=out.chi2.data_ptr(),
                         flags=out.flags.data_ptr(), iters=out.iters.data_ptr(),
                         err=out.err.data_ptr() if out.err is not None else None)
+        if capture:
+            g = C.c_void_p()
+            _abi.check(self._lib.mf_graph_capture(self._ctx, C.byref(b), C.byref(r),
+                                                  C.byref(opts), C.c_void_p(_stream_ptr(stream)),
+                                                  C.byref(g)), "mf_graph_capture")
+            return FitGraph(self, g, (samples, type_of, out))
         _abi.check(self._lib.mf_fit_batch(self._ctx, C.byref(b), C.byref(r), C.byref(opts),
                                           C.c_void_p(_stream_ptr(stream))), "mf_fit_batch")
         return out
+
+    def capture(self, samples: torch.Tensor, type_of: torch.Tensor | None = None, opts=None,
+                out: FitResult | None = None, stream=None, type_ranges=None) -> "FitGraph":
+        """mf_graph_capture: the fit of these buffers as a replayable graph
+        (runs it once eagerly first).  The graph keeps the tensors alive."""
+        return self.fit(samples, type_of, opts, out, stream, capture=True,
+                        type_ranges=type_ranges)
 
 
     # ---- kernel timing (HIP events on the launch stream) ----
@@ -183,7 +228,8 @@ class Engine:
         nch = samples.shape[1]
         out = torch.empty((20, nch), dtype=torch.float32, device=samples.device)
         sdt = _abi.MF_DTYPE_F64 if samples.dtype == torch.float64 else _abi.MF_DTYPE_F32
-        b = _abi.Batch(nch=nch, sample_dtype=sdt, pad=0, samples=samples.data_ptr(), type_of=None)
+        b = _abi.Batch(nch=nch, sample_dtype=sdt, nranges=0, samples=samples.data_ptr(),
+                       type_of=None, ranges=None)
         _abi.check(self._lib.mf_mlp_predict(self._ctx, C.byref(b), out.data_ptr(),
                                             C.c_void_p(_stream_ptr(stream))), "mf_mlp_predict")
         return out

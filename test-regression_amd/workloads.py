@@ -64,6 +64,7 @@ class Workload:
     mix: float = 0.0          # fraction of type-1 channels (0: all type 0)
     description: str = ""
     types: list = field(default_factory=list)
+    event: tuple = ()         # channels per type in one event, in order (event streams)
 
     def chtypes(self):
         out = (_abi.ChType * len(self.specs))()
@@ -71,12 +72,25 @@ class Workload:
             _abi.check(_abi.lib().mf_gen_type(C.byref(s), C.byref(out[i])), "mf_gen_type")
         return out
 
-    def generate(self, nch: int | None = None, nthreads: int = 0, with_truth: bool = False):
-        """Returns (samples [10, n] float64, type_of [n] uint8 or None, truth or None)."""
+    def event_types(self) -> np.ndarray:
+        """Type of each channel of one event: event[0] of type 0, then
+        event[1] of type 1, ..."""
+        return np.concatenate([np.full(k, t, dtype=np.uint8) for t, k in enumerate(self.event)])
+
+    def generate(self, nch: int | None = None, nthreads: int = 0, with_truth: bool = False,
+                 events: int = 0):
+        """Returns (samples [10, n] float64, type_of [n] uint8 or None, truth or None).
+        events > 0 (workloads with an event layout): n = events x the event
+        size, channels event after event, each event laid out by event_types()."""
         n = self.nch if nch is None else int(nch)
         L = _abi.lib()
         type_of = None
-        if self.mix > 0:
+        if events > 0:
+            if not self.event:
+                raise ValueError(f"{self.name} has no event layout")
+            type_of = np.tile(self.event_types(), events)
+            n = type_of.size
+        elif self.mix > 0:
             type_of = np.empty(n, dtype=np.uint8)
             _abi.check(L.mf_gen_mix(n, self.mix, self.seed + 7919,
                                     type_of.ctypes.data_as(C.c_void_p)), "mf_gen_mix")
@@ -115,5 +129,6 @@ def config(name: str, pulse_cov: str = "zero") -> Workload:
         return Workload("c5", [spec(ECAL, a0=(1, 500), pu=(40, 40), jitter=1.0, sigma_t=st),
                                spec(ENDCAP, a0=(1, 500), pu=(80, 80), jitter=1.0, sigma_t=st)],
                         75_848, 5, mix=14_648 / 75_848,
-                        description="event: 61,200 barrel PU40 + 14,648 endcap PU80")
+                        description="event: 61,200 barrel PU40 + 14,648 endcap PU80",
+                        event=(61_200, 14_648))
     raise KeyError(name)

@@ -35,10 +35,10 @@ def test_struct_layouts(pkg):
     assert a.ChType.pederr.offset == 56
     assert C.sizeof(a.Opts) == 56
     assert a.Opts.eps.offset == 16
-    assert C.sizeof(a.Batch) == 32 and C.sizeof(a.Result) == 40
+    assert C.sizeof(a.Batch) == 40 and C.sizeof(a.Result) == 40
     assert C.sizeof(a.GenSpec) == 8 + 11 * 8
     assert C.sizeof(a.Tune) == 32
-    assert pkg._abi.lib().mf_abi_version() == 1
+    assert pkg._abi.lib().mf_abi_version() == 2
 
 
 def test_default_opts_are_the_reference_constants(pkg):

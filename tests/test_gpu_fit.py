@@ -139,9 +139,18 @@ def test_fit_rejects_bad_input(pkg, gpu):
     w = workloads.config("c3")
     eng = engine.Engine(list(w.chtypes()))
     s = torch.zeros((10, 4), dtype=torch.float64, device="cuda")
+    s[5] = 20.0
     bad_t = torch.tensor([0, 1, 2, 0], dtype=torch.uint8, device="cuda")
-    with pytest.raises(_abi.MFError, match="type_of holds type 2"):
-        eng.fit(s, bad_t, _abi.default_opts())
+    # a type id that is not set: that channel is not fitted and is flagged
+    # (the partition runs on the device; nothing is read back to the host)
+    for prec in (_abi.MF_PREC_FP64, _abi.MF_PREC_FP32):
+        r = eng.fit(s if prec == _abi.MF_PREC_FP64 else s.float(), bad_t,
+                    _abi.default_opts(precision=prec))
+        torch.cuda.synchronize()
+        fl = r.flags.cpu().numpy()
+        assert fl[2] == _abi.FLAGS["bad_type"] and torch.all(r.x[:, 2] == 0)
+        assert not np.any(fl[[0, 1, 3]] & _abi.FLAGS["bad_type"])
+        assert torch.all(r.x[5, [0, 3]] > 0)
     with pytest.raises(_abi.MFError):
         eng.fit(s, None, _abi.default_opts(precision=7))
     with pytest.raises(_abi.MFError):
