@@ -224,6 +224,8 @@ def main():
     ap.add_argument("--mlp-weights", default="", help="parameters to load (default: the "
                     "committed weights, else train on 4M c2-distribution channels)")
     ap.add_argument("--save-mlp", default="", help="write the trained parameters here")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="skip mf_autotune (default launch shapes)")
     ap.add_argument("--events", type=int, default=2000, help="event stream (c5): events")
     ap.add_argument("--event-pool", type=int, default=64,
                     help="event stream (c5): distinct events resident in HBM, cycled")
@@ -281,14 +283,23 @@ def main():
             torch.cuda.synchronize()
             log(f"traffic probe: {args.steps} steps of {name}")
         return
+    auto = dict(block=0, persistent=_abi.MF_PERSISTENT_AUTO)
     paths = {
-        "plain": (s64, O(precision=_abi.MF_PREC_FP64), _abi.MF_PREC_FP64),
-        "fast": (s32, O(precision=_abi.MF_PREC_FP32), _abi.MF_PREC_FP32),
-        "warm": (s32, O(precision=_abi.MF_PREC_FP32, warm_start=1), _abi.MF_PREC_FP32),
+        "plain": (s64, O(precision=_abi.MF_PREC_FP64, block=0), _abi.MF_PREC_FP64),
+        "fast": (s32, O(precision=_abi.MF_PREC_FP32, **auto), _abi.MF_PREC_FP32),
+        "warm": (s32, O(precision=_abi.MF_PREC_FP32, warm_start=1, **auto), _abi.MF_PREC_FP32),
     }
-    res, outs = {}, {}
+    res, outs, tuned = {}, {}, {}
+    default_tune = eng.get_tune()
     for name, (x_in, o, prec) in paths.items():
         out = eng.alloc_result(nch, prec)
+        eng.set_tune(**default_tune)
+        if args.autotune:
+            # launch shape chosen for this path on this batch (mf_autotune)
+            t0 = time.perf_counter()
+            tune, tms = eng.autotune(x_in, tt, o, out)
+            tuned[name] = {**tune, "ms": tms, "seconds": time.perf_counter() - t0}
+            log(f"autotune {name}: {tuned[name]}")
         ms, wall, kern = time_steps(eng, lambda: eng.fit(x_in, tt, o, out), args.steps,
                                     args.warmup, stream)
         res[name] = {"ms": ms, "wall_ms": wall,
@@ -353,6 +364,7 @@ def main():
         "speedup_warm_vs_plain_fp64": res["plain"]["ms"] / res["warm"]["ms"],
         "speedup_cold_vs_plain_fp64": res["plain"]["ms"] / res["fast"]["ms"],
         "mlp_ms_per_step": res["warm"]["mlp_ms"],
+        "autotune": tuned or None,
         "accuracy": acc,
         "solves_per_channel": {"warm": float(((it_w >> 16) & 0xFFFF).mean()),
                                "cold": float(((it_f >> 16) & 0xFFFF).mean())},

@@ -86,7 +86,8 @@ typedef struct mf_opts {
     int32_t use_pulse_cov;  /* 1: add x_j^2 * fullpulsecov blocks (updateCov) */
     int32_t warm_start;     /* FP32 only: 1 = start from the MLP's prediction
                                (mf_mlp_*; every result is still KKT-verified) */
-    int32_t persistent;     /* 1: persistent kernel + device work queue       */
+    int32_t persistent;     /* 1: persistent kernel + device work queue;
+                               MF_PERSISTENT_AUTO: as mf_autotune chose       */
     int32_t block;          /* threads per block (0 = tuned default)          */
     int32_t compute_errors; /* FP64 only: MINOS-like in-time uncertainty
                                (PulseChiSqSNNLS.cpp:73-146) into mf_result.err */
@@ -197,10 +198,24 @@ typedef struct mf_tune {
     int32_t p2_wpc;     /* resident wavefronts per CU, FP64 phase (1..32)     */
     int32_t p1_refill;  /* idle lanes that trigger a refill, FP32 phase (1..64) */
     int32_t p2_refill;  /* idle lanes that trigger a refill, FP64 phase (1..64) */
-    int32_t pad[3];
+    int32_t fit_block;  /* threads per block of the per-channel kernels when
+                           opts.block == 0 (64, 128, 192 or 256)            */
+    int32_t use_persistent; /* 1: opts.persistent == MF_PERSISTENT_AUTO runs
+                               the persistent kernels                      */
+    int32_t pad;
 } mf_tune;
+#define MF_PERSISTENT_AUTO 2   /* opts.persistent: as the context's tune says */
 mf_status mf_get_tune(mf_context *ctx, mf_tune *out);
 mf_status mf_set_tune(mf_context *ctx, const mf_tune *tune);
+
+/* Autotuner: fits `in` (into `out`) under each candidate launch shape --
+ * per-channel kernels at 64, 128 and 256 threads per block, persistent
+ * kernels at several resident-wave counts and refill thresholds -- timing
+ * each with HIP events on `stream` (best of 3 after a warm-up), and keeps the
+ * fastest in the context (mf_get_tune).  *best and *ms (optional) receive
+ * the choice and its time per call.  Synchronizes the stream. */
+mf_status mf_autotune(mf_context *ctx, const mf_batch *in, const mf_result *out,
+                      const mf_opts *opts, void *stream, mf_tune *best, float *ms);
 
 /* Batched NNLS solvers, one problem per channel, FP64.
  * A: n matrices of m x m, column-major as the reference's FixedMatrix

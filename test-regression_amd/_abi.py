@@ -78,7 +78,11 @@ class Timings(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [("block", C.c_int32), ("p1_wpc", C.c_int32), ("p2_wpc", C.c_int32),
-                ("p1_refill", C.c_int32), ("p2_refill", C.c_int32), ("pad", C.c_int32 * 3)]
+                ("p1_refill", C.c_int32), ("p2_refill", C.c_int32), ("fit_block", C.c_int32),
+                ("use_persistent", C.c_int32), ("pad", C.c_int32)]
+
+
+MF_PERSISTENT_AUTO = 2
 
 
 class MFError(RuntimeError):
@@ -120,6 +124,8 @@ SIGNATURES = [
     ("mf_graph_launch", C.c_int, [C.c_void_p, C.c_void_p]),
     ("mf_graph_destroy", C.c_int, [C.c_void_p]),
     ("mf_set_tune", C.c_int, [C.c_void_p, C.POINTER(Tune)]),
+    ("mf_autotune", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                              C.POINTER(Tune), C.POINTER(C.c_float)]),
     ("mf_gen_type", C.c_int, [C.POINTER(GenSpec), C.POINTER(ChType)]),
     ("mf_gen_samples", C.c_int, [C.POINTER(GenSpec), C.c_int32, C.c_void_p, C.c_int64,
                                  C.c_uint64, C.c_void_p, C.c_void_p, C.c_int32]),

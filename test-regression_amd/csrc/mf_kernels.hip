@@ -716,12 +716,15 @@ struct FastTune {
     int block = 256;
     int p1_wpc = 16, p2_wpc = 12;
     int p1_need = 16, p2_need = 16;
+    int fit_block = 64;        // per-channel kernels, opts.block == 0
+    bool use_persistent = false;
 };
 static bool tune_valid(const mf_tune &t)
 {
-    return t.block >= 64 && t.block <= 256 && t.block % 64 == 0 && t.p1_wpc >= 1 &&
-           t.p1_wpc <= 32 && t.p2_wpc >= 1 && t.p2_wpc <= 32 && t.p1_refill >= 1 &&
-           t.p1_refill <= 64 && t.p2_refill >= 1 && t.p2_refill <= 64;
+    auto blk = [](int b) { return b >= 64 && b <= 256 && b % 64 == 0; };
+    return blk(t.block) && blk(t.fit_block) && t.p1_wpc >= 1 && t.p1_wpc <= 32 &&
+           t.p2_wpc >= 1 && t.p2_wpc <= 32 && t.p1_refill >= 1 && t.p1_refill <= 64 &&
+           t.p2_refill >= 1 && t.p2_refill <= 64 && (t.use_persistent == 0 || t.use_persistent == 1);
 }
 constexpr int QUEUE_SLOTS = 64;
 
@@ -806,6 +809,8 @@ extern "C" mf_status mf_get_tune(mf_context *ctx, mf_tune *out)
     out->p2_wpc = ctx->tune.p2_wpc;
     out->p1_refill = ctx->tune.p1_need;
     out->p2_refill = ctx->tune.p2_need;
+    out->fit_block = ctx->tune.fit_block;
+    out->use_persistent = ctx->tune.use_persistent ? 1 : 0;
     return MF_OK;
 }
 
@@ -820,6 +825,79 @@ extern "C" mf_status mf_set_tune(mf_context *ctx, const mf_tune *t)
     ctx->tune.p2_wpc = t->p2_wpc;
     ctx->tune.p1_need = t->p1_refill;
     ctx->tune.p2_need = t->p2_refill;
+    ctx->tune.fit_block = t->fit_block;
+    ctx->tune.use_persistent = t->use_persistent != 0;
+    return MF_OK;
+}
+
+extern "C" mf_status mf_autotune(mf_context *ctx, const mf_batch *in, const mf_result *out,
+                                 const mf_opts *opts, void *stream, mf_tune *best, float *ms)
+{
+    if (!ctx || !in || !out || !opts) return fail(MF_ERR_INVALID, "NULL argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    struct Cand {
+        FastTune t;
+        bool pers;
+    };
+    std::vector<Cand> cands;
+    const FastTune base = ctx->tune;
+    for (int b : {64, 128, 256}) {
+        FastTune t = base;
+        t.fit_block = b;
+        t.use_persistent = false;
+        cands.push_back({t, false});
+    }
+    if (opts->precision == MF_PREC_FP32)   // the persistent kernels exist for the fast path
+        for (int wpc : {8, 16})
+            for (int need : {16, 64}) {
+                FastTune t = base;
+                t.block = 256;
+                t.p1_wpc = wpc;
+                t.p2_wpc = wpc;
+                t.p1_need = need;
+                t.p2_need = need;
+                t.use_persistent = true;
+                cands.push_back({t, true});
+            }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    float best_ms = 3.0e38f;
+    FastTune chosen = base;
+    mf_status status = MF_OK;
+    for (const Cand &c : cands) {
+        ctx->tune = c.t;
+        mf_opts o = *opts;
+        o.block = 0;
+        o.persistent = c.pers ? 1 : 0;
+        float t_min = 3.0e38f;
+        for (int rep = 0; rep < 4 && status == MF_OK; ++rep) {
+            if (hipEventRecord(e0, st) != hipSuccess) status = fail(MF_ERR_HIP, "hipEventRecord");
+            if (status == MF_OK) status = mf_fit_batch(ctx, in, out, &o, st);
+            if (status == MF_OK && hipEventRecord(e1, st) != hipSuccess)
+                status = fail(MF_ERR_HIP, "hipEventRecord");
+            if (status == MF_OK && hipEventSynchronize(e1) != hipSuccess)
+                status = fail(MF_ERR_HIP, "hipEventSynchronize");
+            float t = 0.0f;
+            if (status == MF_OK && rep > 0 && hipEventElapsedTime(&t, e0, e1) == hipSuccess)
+                t_min = std::min(t_min, t);
+        }
+        if (status != MF_OK) break;
+        if (t_min < best_ms) {
+            best_ms = t_min;
+            chosen = c.t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->timing = timing;
+    ctx->tune = status == MF_OK ? chosen : base;
+    if (status != MF_OK) return status;
+    if (best) (void)mf_get_tune(ctx, best);
+    if (ms) *ms = best_ms;
     return MF_OK;
 }
 
@@ -1031,7 +1109,7 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
         return fail(MF_ERR_UNSUPPORTED, "warm_start needs MF_PREC_FP32");
     if (opts->warm_start && !ctx->mlp_ready)
         return fail(MF_ERR_INVALID, "warm_start needs MLP parameters (mf_mlp_init / mf_mlp_set_params)");
-    int block = opts->block > 0 ? opts->block : 64;
+    int block = opts->block > 0 ? opts->block : ctx->tune.fit_block;
     if (block % 64 || block > 256) return fail(MF_ERR_INVALID, "block must be 64, 128, 192 or 256");
     HIP_TRY(hipSetDevice(ctx->device));
     hipStream_t st = static_cast<hipStream_t>(stream);
@@ -1084,7 +1162,8 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
         if (opts->precision == MF_PREC_FP64)
             return dispatch_fit<double>(ctx, seg, in, out, prm, block, opts->compute_errors != 0,
                                         st);
-        const bool pers = opts->persistent != 0;
+        const bool pers = opts->persistent == 1 ||
+                          (opts->persistent == MF_PERSISTENT_AUTO && ctx->tune.use_persistent);
         return in->sample_dtype == MF_DTYPE_F64
                    ? dispatch_fast<double>(ctx, seg, in, out, prm, block, warm, pers, st)
                    : dispatch_fast<float>(ctx, seg, in, out, prm, block, warm, pers, st);

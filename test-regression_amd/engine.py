@@ -103,12 +103,8 @@ class Engine:
             err=torch.zeros((10, nch), dtype=dt, device=dev),
         )
 
-    def fit(self, samples: torch.Tensor, type_of: torch.Tensor | None = None, opts=None,
-            out: FitResult | None = None, stream=None, capture: bool = False,
-            type_ranges=None):
-        """samples: [>=nsamples, nch] float64/float32 on the device (row i = sample i)."""
-        if opts is None:
-            opts = _abi.default_opts()
+    def _batch_result(self, samples, type_of, opts, out, type_ranges):
+        """ctypes mf_batch / mf_result for a call (and the objects to keep alive)."""
         if samples.dim() != 2 or not samples.is_cuda:
             raise ValueError("samples must be a 2-D device tensor [samples, nch]")
         samples = samples.contiguous()
@@ -139,15 +135,24 @@ class Engine:
         r = _abi.Result(x=out.x.data_ptr(), chi2=out.chi2.data_ptr(),
                         flags=out.flags.data_ptr(), iters=out.iters.data_ptr(),
                         err=out.err.data_ptr() if out.err is not None else None)
+        return b, r, (samples, type_of, out, rng)
+
+    def fit(self, samples: torch.Tensor, type_of: torch.Tensor | None = None, opts=None,
+            out: FitResult | None = None, stream=None, capture: bool = False,
+            type_ranges=None):
+        """samples: [>=nsamples, nch] float64/float32 on the device (row i = sample i)."""
+        if opts is None:
+            opts = _abi.default_opts()
+        b, r, keep = self._batch_result(samples, type_of, opts, out, type_ranges)
         if capture:
             g = C.c_void_p()
             _abi.check(self._lib.mf_graph_capture(self._ctx, C.byref(b), C.byref(r),
                                                   C.byref(opts), C.c_void_p(_stream_ptr(stream)),
                                                   C.byref(g)), "mf_graph_capture")
-            return FitGraph(self, g, (samples, type_of, out))
+            return FitGraph(self, g, keep)
         _abi.check(self._lib.mf_fit_batch(self._ctx, C.byref(b), C.byref(r), C.byref(opts),
                                           C.c_void_p(_stream_ptr(stream))), "mf_fit_batch")
-        return out
+        return keep[2]
 
     def capture(self, samples: torch.Tensor, type_of: torch.Tensor | None = None, opts=None,
                 out: FitResult | None = None, stream=None, type_ranges=None) -> "FitGraph":
@@ -166,11 +171,26 @@ class Engine:
         _abi.check(self._lib.mf_get_timings(self._ctx, C.byref(t)), "mf_get_timings")
         return {"mlp_ms": t.mlp_ms, "fit_ms": t.fit_ms, "fit_launches": t.fit_launches}
 
+    TUNE_FIELDS = ("block", "p1_wpc", "p2_wpc", "p1_refill", "p2_refill", "fit_block",
+                   "use_persistent")
+
     def get_tune(self) -> dict:
-        """Launch shape of the persistent kernels (mf_tune)."""
+        """Launch shapes (mf_tune)."""
         t = _abi.Tune()
         _abi.check(self._lib.mf_get_tune(self._ctx, C.byref(t)), "mf_get_tune")
-        return {k: getattr(t, k) for k in ("block", "p1_wpc", "p2_wpc", "p1_refill", "p2_refill")}
+        return {k: getattr(t, k) for k in self.TUNE_FIELDS}
+
+    def autotune(self, samples, type_of=None, opts=None, out=None, stream=None,
+                 type_ranges=None):
+        """mf_autotune on this batch: (chosen tune, ms per call)."""
+        if opts is None:
+            opts = _abi.default_opts()
+        b, r, keep = self._batch_result(samples, type_of, opts, out, type_ranges)
+        t, ms = _abi.Tune(), C.c_float()
+        _abi.check(self._lib.mf_autotune(self._ctx, C.byref(b), C.byref(r), C.byref(opts),
+                                         C.c_void_p(_stream_ptr(stream)), C.byref(t),
+                                         C.byref(ms)), "mf_autotune")
+        return {k: getattr(t, k) for k in self.TUNE_FIELDS}, ms.value
 
     def set_tune(self, **kw):
         t = _abi.Tune(**{**self.get_tune(), **kw})
