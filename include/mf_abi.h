@@ -73,12 +73,20 @@ enum {
                           Minimize iterations of a live pulse covariance
                           in FP64                                           */
 };
-enum { MF_SOLVER_INPLACE_FNNLS = 0, MF_SOLVER_FNNLS = 1 };
+enum {
+    MF_SOLVER_INPLACE_FNNLS = 0,  /* nnls/interface/inplace_fnnls.h:9-14 (default)  */
+    MF_SOLVER_FNNLS = 1,          /* nnls/interface/fnnls.h:9-14                    */
+    MF_SOLVER_LEGACY = 2          /* legacy_multifit_cpu/src/PulseChiSqSNNLS.cpp:
+                                     268-363: the passive set and its order carry
+                                     over from one Minimize iteration to the next
+                                     (and into the MINOS refits); FP64 only; use
+                                     max_iter = 1000 as the legacy code does      */
+};
 enum { MF_DTYPE_F64 = 0, MF_DTYPE_F32 = 1 };
 
 typedef struct mf_opts {
     int32_t precision;      /* MF_PREC_*                                      */
-    int32_t solver;         /* MF_SOLVER_* (FP64 only; FP32 is inplace)       */
+    int32_t solver;         /* MF_SOLVER_* (FP32 path: inplace_fnnls only)    */
     int32_t max_iter;       /* FNNLS outer cap (reference GPU: 1000, CPU: 10) */
     int32_t max_minimize;   /* Minimize cap (reference: 50)                   */
     double  eps;            /* FNNLS max-w threshold (reference: 1e-11)       */

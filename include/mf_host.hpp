@@ -129,6 +129,9 @@ class PulseChiSqSNNLS {
     // per-channel MF_F_* flags of the last fit (no reference counterpart)
     int flags() const { return _flags; }
 
+  protected:
+    explicit PulseChiSqSNNLS(int solver);  // MF_SOLVER_*
+
   private:
     SamplePulseMatrix _pulsemat;
     PulseVector _ampvecmin, _errvec;
@@ -136,7 +139,22 @@ class PulseChiSqSNNLS {
     double _chisq = 0.0;
     bool _computeErrors = true;
     int _flags = 0;
+    int _solver = MF_SOLVER_INPLACE_FNNLS;
 };
+
+namespace legacy {
+// legacy_multifit_cpu's PulseChiSqSNNLS (interface/PulseChiSqSNNLS.h:8-33):
+// the same DoFit, whose NNLS keeps its passive set and column order from one
+// Minimize iteration to the next (src/PulseChiSqSNNLS.cpp:268-363), run on
+// the GPU with MF_SOLVER_LEGACY.  The legacy class returns X(), BXs() and
+// Errors() jointly permuted in its final column order; here they are in the
+// order of the bxs given to DoFit.  Its driver reads them as (BX, X) pairs
+// (src/multifit.cpp:204-224), which are the same.  invcov() is not provided.
+class PulseChiSqSNNLS : public mf::PulseChiSqSNNLS {
+  public:
+    PulseChiSqSNNLS() : mf::PulseChiSqSNNLS(MF_SOLVER_LEGACY) {}
+};
+}  // namespace legacy
 
 struct DoFitArgs {
     SampleVector samples;

@@ -461,6 +461,10 @@ typedef struct {
     int flags;
     int has_pi;          /* fullpulsecov has a nonzero entry (flop count only) */
     or_nnls_stats st;
+    /* legacy solver: the column order of _pulsemat/_ampvec/_bxs (position ->
+     * pulse index) and _nP, both kept from one NNLS call to the next */
+    int lperm[LD];
+    int lnp;
 } fitctx;
 
 /* updateCov, PulseChiSqSNNLS.cpp:169-194 */
@@ -472,7 +476,8 @@ static void update_cov(fitctx *c)
     memset(c->C, 0, sizeof(c->C));
     for (int i = 0; i < ns; ++i)
         for (int j = 0; j <= i; ++j) c->C[IDX(i, j)] = p2 * t->noisecor[IDX(i, j)];
-    for (int p = 0; p < t->npulses; ++p) {
+    for (int a = 0; a < t->npulses; ++a) {    /* column order of _pulsemat */
+        const int p = c->lperm[a];
         if (c->x[p] == 0.0) continue;
         int bx = t->bx[p];
         int f = imax(0, bx + t->first_off);
@@ -498,6 +503,82 @@ static void update_cov(fitctx *c)
     }
 }
 
+/* Legacy NNLS (legacy_multifit_cpu/src/PulseChiSqSNNLS.cpp:268-363) on the
+ * whitened problem (A, b) of the current Minimize iteration.  The legacy code
+ * keeps its pulse columns permuted between calls, so aTamat and aTbvec are
+ * formed in the persisted order; here AtA and Atb are formed in pulse order
+ * (the same dot products) and permuted.  Eigen's pivoted LDLT of the leading
+ * block (:6-72) is restated as the Cholesky solve of the same block: the
+ * same solution, different rounding. */
+static void legacy_nnls(fitctx *c, const double *A, const double *b, int m, int n,
+                        or_nnls_stats *st)
+{
+    double AtA0[LD * LD], Atb0[LD], M[LD * LD], v[LD], xp[LD], w[LD], sv[LD];
+    memset(AtA0, 0, sizeof(AtA0));
+    tmul_mn(A, AtA0, m, n);
+    atb_mn(A, b, Atb0, m, n);
+    int *perm = c->lperm;
+    memset(M, 0, sizeof(M));
+    for (int a = 0; a < n; ++a) {
+        for (int q = 0; q < n; ++q) M[IDX(a, q)] = AtA0[IDX(perm[a], perm[q])];
+        v[a] = Atb0[perm[a]];
+        xp[a] = c->x[perm[a]];
+    }
+    unsigned iter;
+    for (iter = 0; iter < c->o->max_iter; ++iter) {              /* :285 */
+        if (iter > 0 || c->lnp == 0) {                           /* :287 */
+            if (c->lnp == n) break;                              /* :288 */
+            int np = c->lnp;
+            for (int i = np; i < n; ++i) {                       /* :293-295 */
+                double t = 0.0;
+                for (int j = 0; j < n; ++j) t = fma(M[IDX(i, j)], xp[j], t);
+                w[i] = v[i] - t;
+            }
+            if (st) st->aflops += 2.0 * n * n + (n - np);
+            int q = np;                                          /* :297 maxCoeff */
+            double wmax = w[np];
+            for (int i = np + 1; i < n; ++i)
+                if (w[i] > wmax) { wmax = w[i]; q = i; }
+            if (wmax < c->o->eps) break;                         /* :300 */
+            sym_swap(M, np, q);                                  /* :306-311 */
+            swap_d(v, np, q);
+            swap_d(xp, np, q);
+            swap_i(perm, np, q);
+            ++c->lnp;
+            if (st) st->outer++;
+        }
+        while (c->lnp > 0) {                                     /* :318 */
+            const int np = c->lnp;
+            int f = llt_solve(M, v, sv, np);                     /* :324 */
+            if (f >= 0 && st) st->llt_fail = 1;
+            stats_solve(st, np);
+            double mins = sv[0];                                 /* :326 */
+            for (int i = 1; i < np; ++i) if (sv[i] < mins) mins = sv[i];
+            if (mins > 0.0) {
+                for (int i = 0; i < np; ++i) xp[i] = sv[i];
+                break;
+            }
+            int idx = 0;                                         /* :331-341 */
+            double minratio = DBL_MAX;
+            for (int i = 0; i < np; ++i)
+                if (sv[i] <= 0.0) {
+                    double ratio = xp[i] / (xp[i] - sv[i]);
+                    if (ratio < minratio) { minratio = ratio; idx = i; }
+                }
+            for (int i = 0; i < np; ++i)                         /* :343-344 */
+                xp[i] = fma(minratio, sv[i] - xp[i], xp[i]);
+            xp[idx] = 0.0;                                       /* :347 */
+            sym_swap(M, np - 1, idx);                            /* :353-358 */
+            swap_d(v, np - 1, idx);
+            swap_d(xp, np - 1, idx);
+            swap_i(perm, np - 1, idx);
+            --c->lnp;
+        }
+    }
+    if (iter == c->o->max_iter && c->o->max_iter > 0 && st) st->hit_max = 1;
+    for (int a = 0; a < n; ++a) c->x[perm[a]] = xp[a];
+}
+
 /* NNLS, PulseChiSqSNNLS.cpp:219-274 */
 static void nnls_step(fitctx *c)
 {
@@ -515,6 +596,10 @@ static void nnls_step(fitctx *c)
     for (int i = 0; i < np; ++i) xx[i] = c->x[i];         /* :258 */
     or_nnls_stats st;
     memset(&st, 0, sizeof(st));
+    if (c->o->solver == OR_SOLVER_LEGACY) {
+        legacy_nnls(c, A, b, ns, np, &st);
+        for (int i = 0; i < np; ++i) xx[i] = c->x[i];
+    } else
     switch (c->o->solver) {
     case OR_SOLVER_INPLACE_CHOL:
         inplace_fnnls_mn(A, b, xx, ns, np, c->o->eps, c->o->max_iter, &st, 1);
@@ -544,7 +629,10 @@ static double compute_chisq(const fitctx *c)
     double r[LD] = {0}, y[LD];
     for (int i = 0; i < ns; ++i) {
         double t = 0.0;
-        for (int j = 0; j < np; ++j) t = fma(c->P[IDX(i, j)], c->x[j], t);
+        for (int a = 0; a < np; ++a) {
+            const int j = c->lperm[a];
+            t = fma(c->P[IDX(i, j)], c->x[j], t);
+        }
         r[i] = t - c->s[i];
     }
     or_forward_subst(c->L, r, y, LD, ns);
@@ -603,6 +691,8 @@ int or_dofit(const or_chtype *t, const double *samples, const or_opts *o,
         if (!isfinite(samples[i])) c.flags |= OR_F_NONFINITE;
     }
     or_build_pulsemat(t, c.P);
+    for (int i = 0; i < LD; ++i) c.lperm[i] = i;          /* legacy: _nP = 0, pulse order */
+    c.lnp = 0;
     int nmin = 0;
     int status = minimize(&c, &nmin);                    /* :65 */
     r->n_minimize = nmin;
@@ -624,7 +714,18 @@ int or_dofit(const or_chtype *t, const double *samples, const or_opts *o,
             double chisq0 = c.chisq;
             double x0 = r->x[ipit];
             /* :96-102 compares against _nP, which the new NNLS never moves
-             * off zero, so the column swap never happens */
+             * off zero, so the column swap never happens; the legacy NNLS
+             * keeps _nP, and a passive in-time pulse is swapped to the last
+             * passive position and leaves the passive set
+             * (legacy_multifit_cpu/src/PulseChiSqSNNLS.cpp:151-157) */
+            if (o->solver == OR_SOLVER_LEGACY) {
+                int pos = 0;
+                while (c.lperm[pos] != ipit) ++pos;
+                if (pos < c.lnp) {
+                    swap_i(c.lperm, pos, c.lnp - 1);
+                    --c.lnp;
+                }
+            }
             double pulse[LD];
             for (int i = 0; i < ns; ++i) {
                 pulse[i] = c.P[IDX(i, ipit)];

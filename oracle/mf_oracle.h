@@ -79,7 +79,9 @@ typedef struct {
     double fullpulsecov[OR_FULL * OR_FULL];  /* ld 19                       */
 } or_chtype;
 
-enum { OR_SOLVER_INPLACE = 0, OR_SOLVER_INPLACE_CHOL = 1, OR_SOLVER_FNNLS = 2 };
+enum { OR_SOLVER_INPLACE = 0, OR_SOLVER_INPLACE_CHOL = 1, OR_SOLVER_FNNLS = 2,
+       OR_SOLVER_LEGACY = 3 /* legacy_multifit_cpu's NNLS: passive set kept across
+                               Minimize iterations */ };
 
 typedef struct {
     double eps;          /* FNNLS convergence on max w (reference 1e-11)      */

@@ -180,7 +180,7 @@ struct FitParams {
     int32_t max_iter;    // FNNLS outer cap
     int32_t max_minimize;
     int32_t use_pi;      // amplitude-weighted pulse covariance live
-    int32_t solver;      // 0 inplace_fnnls, 1 fnnls
+    int32_t solver;      // 0 inplace_fnnls, 1 fnnls, 2 legacy (persisted passive set)
     float eps_rel32;     // FP32 path: rounding allowance of w (see fnnls_inplace)
     float tau32;         // FP32 path: pivot ratio below which a solve goes to FP64
     double eps_rel64;    // FP32 path: rounding allowance of w in the FP64 phase
@@ -580,6 +580,119 @@ MF_HD void fnnls_inplace(const T (&AtA)[10][10], const T (&Atb)[10], T (&x)[10],
     if (iter == max_iter && max_iter > 0) st.hit_max = true;
 }
 
+// State of the legacy NNLS that lives across the Minimize iterations of one
+// fit: the passive set, the permutation (position -> index and its inverse)
+// and the number of passive positions (PulseChiSqSNNLS members _nP, and the
+// column order of _pulsemat / _ampvec / _bxs, which the legacy code permutes
+// in place: legacy_multifit_cpu/src/PulseChiSqSNNLS.cpp:304-311, 353-360).
+struct LegacyState {
+    uint32_t pm = 0;
+    uint64_t perm = 0, ipos = 0;
+    int np = 0;
+    MF_HD void reset()
+    {
+        pm = 0;
+        perm = ident_nibbles();
+        ipos = perm;
+        np = 0;
+    }
+    // takes index i out of the passive set: its position is swapped with the
+    // last passive one (PulseChiSqSNNLS.cpp:151-157, the MINOS set-up)
+    MF_HD void drop(int i)
+    {
+        const int pos = nib(ipos, i);
+        if (pos >= np) return;
+        --np;
+        const int il = nib(perm, np);
+        perm = setnib(setnib(perm, pos, il), np, i);
+        ipos = setnib(setnib(ipos, il, pos), i, np);
+        pm &= ~(1u << i);
+    }
+};
+
+// Legacy NNLS (legacy_multifit_cpu/src/PulseChiSqSNNLS.cpp:268-363): FNNLS
+// whose passive set and order persist from one Minimize iteration to the
+// next.  The first pass of a call with passive positions already set skips
+// the KKT test and solves first (:287).  w is taken on the free tail with the
+// first maximum in permuted order (:296-297), the passive system is solved
+// (the reference uses a fixed-size LDLT of the permuted leading block,
+// :6-72; here the masked Cholesky of the same block), and the step back
+// takes the first minimum ratio in permuted order, position 0 when no ratio
+// is below the maximum (:331-341).  Amplitudes stay in original index order;
+// the permutation only decides ties and which position a removal swaps with.
+template <typename T, int N>
+MF_HD void fnnls_legacy(const T (&AtA)[10][10], const T (&Atb)[10], T (&x)[10], T eps,
+                        int max_iter, FnnlsStats &st, LegacyState &L)
+{
+    int iter = 0;
+    for (; iter < max_iter; ++iter) {                           // :285
+        if (iter > 0 || L.np == 0) {                            // :287
+            if (L.np == N) break;                               // :288
+            T wmax = T(0);
+            int q = -1, qpos = 16;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                T t = T(0);
+#pragma unroll
+                for (int j = 0; j < N; ++j) t = mfma(sym(AtA, i, j), x[j], t);
+                const T w = Atb[i] - t;
+                const int pos = nib(L.ipos, i);
+                const bool isfree = !((L.pm >> i) & 1u);
+                const bool better = (q < 0) || (w > wmax) || (w == wmax && pos < qpos);
+                if (isfree && better) {
+                    wmax = w;
+                    q = i;
+                    qpos = pos;
+                }
+            }
+            if (wmax < eps) break;                              // :300
+            {   // swap positions np <-> qpos                      :304-311
+                const int inp = nib(L.perm, L.np);
+                L.perm = setnib(setnib(L.perm, L.np, q), qpos, inp);
+                L.ipos = setnib(setnib(L.ipos, q, L.np), inp, qpos);
+            }
+            L.pm |= 1u << q;
+            ++L.np;
+            st.outer++;
+        }
+        while (L.np > 0) {                                      // :318
+            T sv[10];
+            if (!solve_passive<T, N>(AtA, Atb, L.pm, sv)) st.llt_fail = true;
+            st.inner++;
+            bool allpos = true;
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                if ((L.pm >> i) & 1u) allpos = allpos && (sv[i] > T(0));
+            if (allpos) {                                       // :326-329
+#pragma unroll
+                for (int i = 0; i < N; ++i)
+                    if ((L.pm >> i) & 1u) x[i] = sv[i];
+                break;
+            }
+            T alpha = Lim<T>::max();                            // :331-341
+            int apos = 0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                if (((L.pm >> i) & 1u) && sv[i] <= T(0)) {
+                    const T ratio = mdiv(x[i], x[i] - sv[i]);
+                    const int pos = nib(L.ipos, i);
+                    if (ratio < alpha || (ratio == alpha && alpha != Lim<T>::max() && pos < apos)) {
+                        alpha = ratio;
+                        apos = pos;
+                    }
+                }
+            }
+            const int a = nib(L.perm, apos);
+#pragma unroll
+            for (int i = 0; i < N; ++i)                         // :343-349
+                if ((L.pm >> i) & 1u) x[i] = mfma(alpha, sv[i] - x[i], x[i]);
+            x[a] = T(0);
+            L.drop(a);                                          // :353-360
+        }
+    }
+    if (iter == max_iter && max_iter > 0) st.hit_max = true;
+}
+
 // fnnls (nnls/src/fnnls.cu:28-173): boolean-mask FNNLS on the full x.
 // Its outer test `!n_active || w < eps` (:113) counts the active indices
 // minus one, so it stops as soon as a single index is still free, without
@@ -665,7 +778,8 @@ template <typename T> struct FitOut {
 // Park is any type with store(k, v) / load(k) for k < 55.  Returns chi2.
 template <typename T, int NS, int NP, typename Park>
 MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams &prm,
-                      Park &park, T (&x)[10], FnnlsStats &st, int &flags, int zc = -1)
+                      Park &park, T (&x)[10], FnnlsStats &st, int &flags, int zc = -1,
+                      LegacyState *ls = nullptr)
 {
     const TypeMats<T> &tm = *opaque(&tm0);
     T Li[10][10], ri[10];
@@ -680,6 +794,8 @@ MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams 
         for (int j = 0; j <= i; ++j) park.store(i * (i + 1) / 2 + j, Li[i][j]);
     if (prm.solver == 1)
         fnnls_mask<T, NP>(AtA, Atb, x, T(prm.eps), prm.max_iter, st);
+    else if (prm.solver == 2 && ls)
+        fnnls_legacy<T, NP>(AtA, Atb, x, T(prm.eps), prm.max_iter, st, *ls);
     else
         fnnls_inplace<T, NP>(AtA, Atb, x, T(prm.eps), T(0), prm.max_iter, st);
 #pragma unroll
@@ -694,11 +810,11 @@ MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams 
 template <typename T, int NS, int NP, typename Park>
 MF_HD int minimize_from(const TypeMats<T> &tm, const T (&s)[10], const FitParams &prm,
                         Park &park, T (&x)[10], T &chi2, int it0, FnnlsStats &st, int &flags,
-                        int zc = -1)
+                        int zc = -1, LegacyState *ls = nullptr)
 {
     int it = it0;
     for (; it < prm.max_minimize; ++it) {                       // :154
-        const T now = minimize_step<T, NS, NP>(tm, s, prm, park, x, st, flags, zc);
+        const T now = minimize_step<T, NS, NP>(tm, s, prm, park, x, st, flags, zc, ls);
         const T d = now - chi2;
         chi2 = now;
         if (mabs(d) < T(prm.chi2_tol)) return it + 1;           // :162
@@ -724,10 +840,17 @@ MF_HD bool all_finite(const T (&s)[10])
 // samples reduced by it, Minimize continues from the current state, and
 // sigma+ = approxerr / sqrt(chi2+ - chi2_0); if x0 / sigma+ > 0.5 the same is
 // done at max(0, x0 - approxerr) and the two are averaged (:120-142).
+//
+// ls (legacy solver): the fit's persisted passive state.  When the in-time
+// pulse is passive it leaves the passive set first (:151-157 -- a branch the
+// current reference never takes, its _nP staying 0), and the two Minimize
+// calls continue from that state.
 template <typename T, int NS, int NP, typename Park>
 MF_HD T minos_error(const TypeMats<T> &tm, const T (&s)[10], const FitParams &prm, Park &park,
-                    const T (&xfit)[10], T chi2_0, int ip, FnnlsStats &st, int &flags)
+                    const T (&xfit)[10], T chi2_0, int ip, FnnlsStats &st, int &flags,
+                    LegacyState *ls = nullptr)
 {
+    if (ls) ls->drop(ip);
     T y = T(0), q = T(0);
     {
         T col[10];
@@ -753,7 +876,7 @@ MF_HD T minos_error(const TypeMats<T> &tm, const T (&s)[10], const FitParams &pr
     x[ip] = xplus;
 #pragma unroll
     for (int i = 0; i < 10; ++i) sr[i] = i < NS ? s[i] - xplus * tm.P[i][ip] : T(0);
-    minimize_from<T, NS, NP>(tm, sr, prm, park, x, chi2, 0, st, flags, ip);
+    minimize_from<T, NS, NP>(tm, sr, prm, park, x, chi2, 0, st, flags, ip, ls);
     T Li[10][10];
 #pragma unroll
     for (int i = 0; i < NS; ++i)
@@ -767,7 +890,7 @@ MF_HD T minos_error(const TypeMats<T> &tm, const T (&s)[10], const FitParams &pr
     x[ip] = xminus;
 #pragma unroll
     for (int i = 0; i < 10; ++i) sr[i] = i < NS ? s[i] - xminus * tm.P[i][ip] : T(0);
-    minimize_from<T, NS, NP>(tm, sr, prm, park, x, chi2, 0, st, flags, ip);
+    minimize_from<T, NS, NP>(tm, sr, prm, park, x, chi2, 0, st, flags, ip, ls);
 #pragma unroll
     for (int i = 0; i < NS; ++i)
 #pragma unroll
@@ -779,8 +902,11 @@ MF_HD T minos_error(const TypeMats<T> &tm, const T (&s)[10], const FitParams &pr
 
 // Plain fit: the reference's algorithm in precision T, cold start, and with
 // ERRORS the in-time uncertainty (ip >= 0: index of the in-time pulse).
+// LEGACY: the legacy NNLS (prm.solver must be 2), whose passive state lives
+// through the whole fit; a separate instantiation, so that the other solvers
+// do not carry its state in registers.
 // Non-finite samples are not fitted (x = 0, F_NONFINITE).
-template <typename T, int NS, int NP, bool ERRORS = false, typename Park>
+template <typename T, int NS, int NP, bool ERRORS = false, bool LEGACY = false, typename Park>
 MF_HD void fit_plain(const TypeMats<T> &tm, const T (&s)[10], const FitParams &prm,
                      Park &park, FitOut<T> &out, int ip = -1)
 {
@@ -793,14 +919,17 @@ MF_HD void fit_plain(const TypeMats<T> &tm, const T (&s)[10], const FitParams &p
     T chi2 = T(0);
     int flags = 0, nmin = 0;
     FnnlsStats st;
+    LegacyState ls;   // legacy solver: passive state across Minimize iterations
+    ls.reset();
+    LegacyState *lsp = LEGACY ? &ls : nullptr;
     if (all_finite<T, NS>(s))
-        nmin = minimize_from<T, NS, NP>(tm, s, prm, park, x, chi2, 0, st, flags);
+        nmin = minimize_from<T, NS, NP>(tm, s, prm, park, x, chi2, 0, st, flags, -1, lsp);
     else
         flags |= F_NONFINITE;
     if (ERRORS && ip >= 0 && !(flags & F_NONFINITE) && prm.max_minimize > 0) {
         FnnlsStats st2;
         int fl2 = 0;
-        const T e = minos_error<T, NS, NP>(tm, s, prm, park, x, chi2, ip, st2, fl2);
+        const T e = minos_error<T, NS, NP>(tm, s, prm, park, x, chi2, ip, st2, fl2, lsp);
 #pragma unroll
         for (int i = 0; i < 10; ++i)
             if (i == ip) out.err[i] = e;

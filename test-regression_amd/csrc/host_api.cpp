@@ -100,7 +100,8 @@ struct Fitted {
 };
 
 Fitted run_fits(const std::vector<mf_chtype> &types, const std::vector<uint8_t> &type_of,
-                const std::vector<double> &samples_soa, int64_t nch, bool errors)
+                const std::vector<double> &samples_soa, int64_t nch, bool errors,
+                int solver = MF_SOLVER_INPLACE_FNNLS)
 {
     Device &dev = Device::get();
     dev.set_types(types);
@@ -128,6 +129,7 @@ Fitted run_fits(const std::vector<mf_chtype> &types, const std::vector<uint8_t> 
     mf_opts o;
     mf_default_opts(&o);
     o.compute_errors = errors ? 1 : 0;
+    o.solver = solver;
     throw_status(mf_fit_batch(dev.ctx(), &in, &out, &o, st), "mf_fit_batch");
     Fitted f;
     f.x.resize(10 * (size_t)nch);
@@ -150,6 +152,7 @@ Fitted run_fits(const std::vector<mf_chtype> &types, const std::vector<uint8_t> 
 }  // namespace
 
 PulseChiSqSNNLS::PulseChiSqSNNLS() {}
+PulseChiSqSNNLS::PulseChiSqSNNLS(int solver) : _solver(solver) {}
 PulseChiSqSNNLS::~PulseChiSqSNNLS() {}
 
 bool PulseChiSqSNNLS::DoFit(const SampleVector &samples, const SampleMatrix &samplecor,
@@ -161,7 +164,7 @@ bool PulseChiSqSNNLS::DoFit(const SampleVector &samples, const SampleMatrix &sam
     std::vector<mf_chtype> types{make_chtype(samplecor, pederr, bxs, fullpulse, fullpulsecov)};
     std::vector<double> s(10);
     for (int i = 0; i < 10; ++i) s[(size_t)i] = samples(i);
-    Fitted f = run_fits(types, {}, s, 1, _computeErrors);
+    Fitted f = run_fits(types, {}, s, 1, _computeErrors, _solver);
     // pulse matrix as built by DoFit (PulseChiSqSNNLS.cpp:50-62)
     _pulsemat = SamplePulseMatrix::Zero();
     for (int p = 0; p < np; ++p) {

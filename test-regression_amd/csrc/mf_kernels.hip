@@ -111,7 +111,7 @@ __device__ __forceinline__ bool seg_item(const SegArg &a, int &t, int64_t &c)
     return true;
 }
 
-template <typename T, typename TS, int N, bool ERRORS>
+template <typename T, typename TS, int N, bool ERRORS, bool LEGACY>
 __global__ __launch_bounds__(256)
 void k_fit_plain(const DevType *__restrict__ types, SegArg seg,
                  const TS *__restrict__ samples, int64_t nch, T *__restrict__ xo,
@@ -128,7 +128,7 @@ void k_fit_plain(const DevType *__restrict__ types, SegArg seg,
     for (int i = 0; i < 10; ++i) s[i] = i < N ? (T)samples[(int64_t)i * nch + c] : T(0);
     LdsPark<T> park{reinterpret_cast<T *>(lds_raw) + threadIdx.x, (int)blockDim.x};
     FitOut<T> out;
-    fit_plain<T, N, N, ERRORS>(tm, s, prm, park, out, types[type_id].h.intime);
+    fit_plain<T, N, N, ERRORS, LEGACY>(tm, s, prm, park, out, types[type_id].h.intime);
 #pragma unroll
     for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = out.x[p];
     if (ERRORS) {
@@ -965,16 +965,13 @@ static mf_status launch_fit(mf_context *ctx, const SegArg &seg, const void *samp
     if (nt <= 0) return MF_OK;
     const int64_t grid = (nt + block - 1) / block;
     const size_t lds = sizeof(T) * 55 * (size_t)block;
-    if (errors)
-        hipLaunchKernelGGL((k_fit_plain<T, TS, N, true>), dim3((unsigned)grid), dim3(block), lds,
-                           st, ctx->d_types, seg, static_cast<const TS *>(samples), nch,
-                           static_cast<T *>(out->x), static_cast<T *>(out->chi2), out->flags,
-                           out->iters, static_cast<T *>(out->err), prm);
-    else
-        hipLaunchKernelGGL((k_fit_plain<T, TS, N, false>), dim3((unsigned)grid), dim3(block),
-                           lds, st, ctx->d_types, seg, static_cast<const TS *>(samples), nch,
-                           static_cast<T *>(out->x), static_cast<T *>(out->chi2), out->flags,
-                           out->iters, static_cast<T *>(out->err), prm);
+    const bool legacy = prm.solver == MF_SOLVER_LEGACY;
+    auto kern = errors ? (legacy ? k_fit_plain<T, TS, N, true, true> : k_fit_plain<T, TS, N, true, false>)
+                       : (legacy ? k_fit_plain<T, TS, N, false, true> : k_fit_plain<T, TS, N, false, false>);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), lds, st, ctx->d_types, seg,
+                       static_cast<const TS *>(samples), nch, static_cast<T *>(out->x),
+                       static_cast<T *>(out->chi2), out->flags, out->iters,
+                       static_cast<T *>(out->err), prm);
     HIP_TRY(hipGetLastError());
     return MF_OK;
 }
@@ -1103,7 +1100,8 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
         return fail(MF_ERR_INVALID, "precision must be MF_PREC_FP64 or MF_PREC_FP32");
     if (opts->max_minimize < 0 || opts->max_iter < 0)
         return fail(MF_ERR_INVALID, "max_minimize and max_iter must be >= 0");
-    if (opts->solver != MF_SOLVER_INPLACE_FNNLS && opts->solver != MF_SOLVER_FNNLS)
+    if (opts->solver != MF_SOLVER_INPLACE_FNNLS && opts->solver != MF_SOLVER_FNNLS &&
+        opts->solver != MF_SOLVER_LEGACY)
         return fail(MF_ERR_INVALID, "unknown solver");
     if (opts->warm_start && opts->precision != MF_PREC_FP32)
         return fail(MF_ERR_UNSUPPORTED, "warm_start needs MF_PREC_FP32");

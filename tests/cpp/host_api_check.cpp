@@ -10,7 +10,8 @@
 //         pederr, 19 fullpulse, 361 fullpulsecov (row-major), 10 bx (as double)
 // output: per channel: DoFit X[10], Errors[10], chi2; doFitWrapper X[10], chi2,
 //         ampl; inplace_fnnls x[10] of (A = I + pulsemat, b = samples); then
-//         the 3x3 L (9), y (3), x (3) in double and in float.
+//         the 3x3 L (9), y (3), x (3) in double and in float; then per
+//         channel legacy::PulseChiSqSNNLS::DoFit X[10], Errors[10], chi2.
 #include <stdio.h>
 #include <stdlib.h>
 
@@ -94,6 +95,13 @@ int main(int argc, char **argv)
         for (float v : Lf) out.push_back(v);
         for (float v : yf) out.push_back(v);
         for (float v : xf) out.push_back(v);
+        for (auto &a : args) {
+            mf::legacy::PulseChiSqSNNLS fit;
+            fit.DoFit(a.samples, a.samplecor, a.pederr, a.bxs, a.fullpulse, a.fullpulsecov);
+            for (int p = 0; p < 10; ++p) out.push_back(fit.X()[p]);
+            for (int p = 0; p < 10; ++p) out.push_back(fit.Errors()[p]);
+            out.push_back(fit.ChiSq());
+        }
         fwrite(out.data(), 8, out.size(), o);
     } catch (const mf::Error &e) {
         fprintf(stderr, "mf::Error %d: %s\n", e.status, e.what());

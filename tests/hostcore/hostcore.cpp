@@ -48,8 +48,13 @@ static void run(const DevType &dt, const double *samples, int64_t nch, int64_t c
     for (int i = 0; i < 10; ++i) s[i] = i < N ? (T)samples[i * nch + c] : T(0);
     RegPark<T> park;
     FitOut<T> out;
-    if (g_err)
+    const bool legacy = prm.solver == 2;
+    if (g_err && legacy)
+        fit_plain<T, N, N, true, true>(mats<T>(dt), s, prm, park, out, dt.h.intime);
+    else if (g_err)
         fit_plain<T, N, N, true>(mats<T>(dt), s, prm, park, out, dt.h.intime);
+    else if (legacy)
+        fit_plain<T, N, N, false, true>(mats<T>(dt), s, prm, park, out);
     else
         fit_plain<T, N, N>(mats<T>(dt), s, prm, park, out);
     for (int p = 0; p < 10; ++p) x[p * nch + c] = p < N ? (double)out.x[p] : 0.0;

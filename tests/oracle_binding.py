@@ -53,7 +53,7 @@ class OrStats(C.Structure):
                 ("k3sum", C.c_longlong), ("aflops", C.c_double)]
 
 
-OR_SOLVER_INPLACE, OR_SOLVER_INPLACE_CHOL, OR_SOLVER_FNNLS = 0, 1, 2
+OR_SOLVER_INPLACE, OR_SOLVER_INPLACE_CHOL, OR_SOLVER_FNNLS, OR_SOLVER_LEGACY = 0, 1, 2, 3
 
 _cache: dict = {}
 

@@ -8,7 +8,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from oracle_binding import OrStats, dptr, oracle, oracle_fit, oracle_opts
+from oracle_binding import OR_SOLVER_LEGACY, OrStats, dptr, oracle, oracle_fit, oracle_opts
 
 pytestmark = pytest.mark.gpu
 PROG = Path(__file__).resolve().parent / "cpp" / "host_api_check"
@@ -43,7 +43,9 @@ def test_cpp_host_library(pkg, gpu, tmp_path):
         k += n * 12
         nn = o[k:k + n * 20].reshape(n, 20)
         k += n * 20
-        la = o[k:]
+        la = o[k:k + 30]
+        k += 30
+        leg = o[k:].reshape(n, 21)
         ref = oracle_fit(ct, s, None, oracle_opts(compute_errors=1), nthreads=4)
         ref0 = oracle_fit(ct, s, None, oracle_opts(), nthreads=4)
         scale = np.maximum(np.abs(ref["x"]).max(axis=1), 1.0)[:, None]
@@ -78,5 +80,15 @@ def test_cpp_host_library(pkg, gpu, tmp_path):
         x3 = np.linalg.solve(M3, [3, 3, 4])
         assert np.allclose(la[:9].reshape(3, 3), Lx, atol=1e-14)
         assert np.allclose(la[12:15], x3, atol=1e-14)
+        # legacy_multifit_cpu's class: the legacy solver, errors on
+        refl = oracle_fit(ct, s, None, oracle_opts(compute_errors=1, solver=OR_SOLVER_LEGACY),
+                          nthreads=4)
+        scl = np.maximum(np.abs(refl["x"]).max(axis=1), 1.0)[:, None]
+        assert np.all(np.abs(leg[:, :10] - refl["x"]) <= 1e-10 * scl)
+        e, er = leg[:, 15], refl["err"][:, 5]
+        assert np.array_equal(np.isnan(e), np.isnan(er))
+        ok = ~np.isnan(er)
+        assert np.all(np.abs(e[ok] - er[ok]) <= 1e-9 * np.abs(er[ok]))
+        assert np.allclose(leg[:, 20], refl["chi2"], rtol=1e-10, atol=1e-10)
         assert np.allclose(la[15:24].reshape(3, 3), Lx, atol=1e-6)
         assert np.allclose(la[27:30], x3, atol=1e-5)
