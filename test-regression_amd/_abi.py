@@ -83,6 +83,7 @@ class Tune(C.Structure):
 
 
 MF_PERSISTENT_AUTO = 2
+MF_IO_F32, MF_IO_F64, MF_IO_I32, MF_IO_VEC_F64, MF_IO_VEC_I32 = 0, 1, 2, 3, 4
 
 
 class MFError(RuntimeError):
@@ -130,6 +131,14 @@ SIGNATURES = [
     ("mf_gen_samples", C.c_int, [C.POINTER(GenSpec), C.c_int32, C.c_void_p, C.c_int64,
                                  C.c_uint64, C.c_void_p, C.c_void_p, C.c_int32]),
     ("mf_gen_mix", C.c_int, [C.c_int64, C.c_double, C.c_uint64, C.c_void_p]),
+    ("mf_io_open", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("mf_io_close", C.c_int, [C.c_void_p]),
+    ("mf_io_tree_entries", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64)]),
+    ("mf_io_read_branch", C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int32, C.c_void_p,
+                                    C.c_int64, C.POINTER(C.c_int64), C.c_void_p, C.c_int64,
+                                    C.POINTER(C.c_int64)]),
+    ("mf_io_read_graph", C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                   C.POINTER(C.c_int64)]),
 ]
 
 

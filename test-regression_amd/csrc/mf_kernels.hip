@@ -45,6 +45,18 @@ static mf_status fail(mf_status st, const char *fmt, ...)
             return fail(MF_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));     \
     } while (0)
 
+// the same message slot for the library's other translation units
+namespace mf {
+mf_status set_error(mf_status st, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return st;
+}
+}  // namespace mf
+
 extern "C" const char *mf_last_error(void) { return g_err; }
 extern "C" int mf_abi_version(void) { return MF_ABI_VERSION; }
 

@@ -1,0 +1,66 @@
+"""The reference's own generated events (tests/golden/reference_samples.npz, read
+from its data files by the ROOT-free reader) fitted on the GPU with the
+reference CPU driver's set-up (multifit_cpu/src/multifit.cpp:44-83, 196-227),
+against the oracle on the same inputs.  Bars as the other parity tests:
+FP64 amplitudes within 1e-10 of the oracle (relative to the channel's
+largest amplitude, floor 1), FP32 fast path within 1e-4.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle_binding import OR_SOLVER_LEGACY, oracle_fit, oracle_opts
+
+pytestmark = pytest.mark.gpu
+FIXTURE = Path(__file__).resolve().parent / "golden" / "reference_samples.npz"
+
+
+def _setup(idstart=None):
+    from test_regression_amd import _abi, refdata
+    z = np.load(FIXTURE)
+    t = z["tail"]
+    ps = refdata.PulseShape(z["graph_x"], z["graph_y"], t[0], t[1], t[2])
+    ct = refdata.driver_chtype(ps, idstart=idstart if idstart is not None else refdata.IDSTART)
+    return z["samples"], z["amplitudeTruth"], ps, ct, (_abi.ChType * 1)(ct)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "legacy", "fp32"])
+def test_reference_events_match_oracle(pkg, gpu, mode):
+    from test_regression_amd import _abi, refdata
+    s, a, ps, ct, cts = _setup()
+    kw, oo, tol = {}, oracle_opts(), 1e-10
+    if mode == "legacy":
+        kw, oo = {"solver": _abi.MF_SOLVER_LEGACY}, oracle_opts(solver=OR_SOLVER_LEGACY)
+    if mode == "fp32":
+        kw, tol = {"precision": _abi.MF_PREC_FP32}, 1e-4
+        s = s.astype(np.float32).astype(np.float64)
+    print(f"[gpu-refdata] mode={mode} events={s.shape[1]}", flush=True)
+    fit = refdata.fit_samples(s if mode != "fp32" else s.astype(np.float32), ct,
+                              _abi.default_opts(**kw))
+    ref = oracle_fit(cts, s, None, oo, nthreads=4)
+    scale = np.maximum(np.abs(ref["x"]).max(axis=1), 1.0)
+    err = np.abs(fit["samplesReco"] - ref["x"]).max(axis=1) / scale
+    assert err.max() <= tol, err.max()
+    assert np.allclose(fit["chi2"], ref["chi2"], rtol=max(tol, 1e-10), atol=1e-10)
+    assert np.array_equal(fit["ampl"], fit["samplesReco"][:, 5])
+    # the fast path's own status bits (FP64 pivot, warm reject, refined) have
+    # no oracle counterpart
+    assert np.all((fit["flags"] & 15) == ref["flags"])
+
+
+def test_reference_events_recover_truth(pkg, gpu, tmp_path):
+    """With the generator's timing (IDSTART 150) the fitted in-time amplitude
+    matches amplitudeTruth within the noise; the .npz output keeps the
+    RecoAndSim branch names."""
+    from test_regression_amd import _abi, refdata
+    s, a, ps, ct, _ = _setup(idstart=150.0)
+    fit = refdata.fit_samples(s, ct, _abi.default_opts())
+    dA = fit["ampl"] - a
+    print(f"dA mean {dA.mean():.4f} std {dA.std():.4f}")
+    assert abs(dA.mean()) < 0.02 and dA.std() < 0.05
+    out = tmp_path / "reco.npz"
+    refdata.write_reco(str(out), s, a, fit, ps)
+    z = np.load(out)
+    assert np.array_equal(z["samplesReco"], fit["samplesReco"])
+    assert np.array_equal(z["samples"], s.T) and z["pulseShapeTemplate"].shape == (17,)
