@@ -18,6 +18,7 @@
  *                              (nnls/interface/inplace_fnnls.h:9-14, nnls/interface/kernel_wrapper.h:7-12)
  *   mf_fnnls_batch ........... fnnls / fnnls_kernel / fnnls_wrapper
  *                              (nnls/interface/fnnls.h:9-14, nnls/interface/kernel_wrapper.h:14-24)
+ *   mf_lh_nnls_batch ......... Eigen::NNLS<FixedMatrix>::solve (nnls/interface/eigen_nnls.h:51-414)
  *   mf_linalg_batch .......... cholesky_decomp, solve_forward_substitution,
  *                              solve_backward_substitution, swap_row_column,
  *                              fused_cholesky_forward_substitution_solver_{rcaddition,inbetween_removal}
@@ -76,11 +77,15 @@ enum {
 enum {
     MF_SOLVER_INPLACE_FNNLS = 0,  /* nnls/interface/inplace_fnnls.h:9-14 (default)  */
     MF_SOLVER_FNNLS = 1,          /* nnls/interface/fnnls.h:9-14                    */
-    MF_SOLVER_LEGACY = 2          /* legacy_multifit_cpu/src/PulseChiSqSNNLS.cpp:
+    MF_SOLVER_LEGACY = 2,         /* legacy_multifit_cpu/src/PulseChiSqSNNLS.cpp:
                                      268-363: the passive set and its order carry
                                      over from one Minimize iteration to the next
                                      (and into the MINOS refits); FP64 only; use
                                      max_iter = 1000 as the legacy code does      */
+    MF_SOLVER_LAWSON_HANSON = 3   /* Eigen::NNLS started at the current amplitudes
+                                     (setX), the alternative kept in
+                                     multifit_cpu/src/PulseChiSqSNNLS.cpp:253-257;
+                                     max_iter bounds least-squares solves; FP64 */
 };
 enum { MF_DTYPE_F64 = 0, MF_DTYPE_F32 = 1 };
 
@@ -236,6 +241,15 @@ mf_status mf_inplace_fnnls_batch(const double *A, const double *b, double *x,
 mf_status mf_fnnls_batch(const double *A, const double *b, double *x,
                          int64_t n, int32_t m, double eps, uint32_t max_iter,
                          void *stream);
+/* Lawson-Hanson NNLS of the reference's Eigen::NNLS class
+ * (nnls/interface/eigen_nnls.h:84-98 constructor, :124 setX, :336-414
+ * solve, as tests/main.cpp:53-58 calls it).  Same layout as above; x in is
+ * the setX starting point.  max_iter bounds the least-squares solves; <= 0:
+ * none (the solver then stops after 1000 solves, reported as not solved).
+ * solved (optional, n entries): 1 where solve() returned true. */
+mf_status mf_lh_nnls_batch(const double *A, const double *b, double *x,
+                           int64_t n, int32_t m, double eps, int32_t max_iter,
+                           int32_t *solved, void *stream);
 
 /* Batched linalgebra primitives on n row-major matrices of full x full
  * (full <= 10), operating on the leading view x view block, FP64 or FP32

@@ -431,6 +431,134 @@ void or_fnnls(const double *A, const double *b, double *x, int n, double eps,
     fnnls_mn(A, b, x, n, n, eps, max_iter, st);
 }
 
+/* ---- Lawson-Hanson (eigen_nnls.h) ---------------------------------------
+ * Householder reflector of a column segment v[0..len) (Golub & Van Loan,
+ * 5.1.2, the convention of LAPACK's xLARFG): beta = -sign(v0) |v|,
+ * essential part v[1..) / (v0 - beta), tau = (beta - v0) / beta; a segment
+ * whose tail is zero gets tau = 0 and beta = v0 (eigen_nnls.h:276-279 builds
+ * it through Eigen's makeHouseholderInPlace).  col(r) reads row r of the
+ * column. */
+static void lh_make_householder(double *QR, int col, int k, int m, double *tau)
+{
+    double tail = 0.0;
+    for (int r = k + 1; r < m; ++r) tail += QR[IDX(r, col)] * QR[IDX(r, col)];
+    const double c0 = QR[IDX(k, col)];
+    if (tail <= DBL_MIN) {
+        *tau = 0.0;
+        for (int r = k + 1; r < m; ++r) QR[IDX(r, col)] = 0.0;
+        return;                          /* beta = c0 stays on the diagonal */
+    }
+    double beta = sqrt(c0 * c0 + tail);
+    if (c0 >= 0.0) beta = -beta;
+    for (int r = k + 1; r < m; ++r) QR[IDX(r, col)] /= (c0 - beta);
+    *tau = (beta - c0) / beta;
+    QR[IDX(k, col)] = beta;
+}
+
+/* applies reflector i (essential part below the diagonal of QR's column i,
+ * coefficient tau) to rows i..m-1 of the vector v (stride vs) */
+static void lh_apply(const double *QR, int i, double tau, int m, double *v, int vs)
+{
+    if (m - i == 1) {
+        v[i * vs] *= 1.0 - tau;
+        return;
+    }
+    if (tau == 0.0) return;
+    double t = 0.0;
+    for (int r = i + 1; r < m; ++r) t += QR[IDX(r, i)] * v[r * vs];
+    t += v[i * vs];
+    v[i * vs] -= tau * t;
+    for (int r = i + 1; r < m; ++r) v[r * vs] -= tau * QR[IDX(r, i)] * t;
+}
+
+/* nnls_householder_qr_inplace_update (eigen_nnls.h:251-281): column k of QR
+ * becomes A's column `acol`, the k reflectors before it are applied, and
+ * reflector k is formed in place */
+static void lh_qr_update(double *QR, double *tau, const double *A, int acol, int k, int m)
+{
+    for (int r = 0; r < m; ++r) QR[IDX(r, k)] = A[IDX(r, acol)];
+    for (int i = 0; i < k; ++i) lh_apply(QR, i, tau[i], m, QR + k, LD);
+    lh_make_householder(QR, k, k, m, &tau[k]);
+}
+
+/* eigen_nnls.h:336-414 */
+static int lh_nnls_mn(const double *A, const double *b, double *x, int m, int n, double eps,
+                      int max_iter, or_nnls_stats *st)
+{
+    double AtA[LD * LD], Atb[LD], w[LD], y[LD], QR[LD * LD], tau[LD], c[LD];
+    int perm[LD];
+    memset(AtA, 0, sizeof(AtA));
+    memset(QR, 0, sizeof(QR));
+    memset(tau, 0, sizeof(tau));
+    tmul_mn(A, AtA, m, n);                                       /* :98 */
+    atb_mn(A, b, Atb, m, n);                                     /* :352 */
+    for (int i = 0; i < n; ++i) perm[i] = i;                     /* :348-349 */
+    int np = 0, nls = 0;
+    for (;;) {
+        for (int i = 0; i < n; ++i) {                            /* :214-216 */
+            double t = 0.0;
+            for (int j = 0; j < n; ++j) t += AtA[IDX(i, j)] * x[j];
+            w[i] = Atb[i] - t;
+        }
+        if (st) st->aflops += 2.0 * n * n;
+        if (n == np) return 1;                                   /* :361 */
+        int q = np;                                              /* :170-180 */
+        for (int i = np + 1; i < n; ++i)
+            if (w[perm[q]] < w[perm[i]]) q = i;
+        if (w[perm[q]] - eps < 0) return 1;
+        swap_i(perm, q, np);                                     /* :431-440 */
+        ++np;
+        lh_qr_update(QR, tau, A, perm[np - 1], np - 1, m);
+        if (st) st->outer++;
+        for (;;) {
+            if (max_iter > 0 && nls >= max_iter) {               /* :375-377 */
+                if (st) st->hit_max = 1;
+                return 0;
+            }
+            for (int r = 0; r < m; ++r) c[r] = b[r];             /* :462-472 */
+            for (int i = 0; i < np; ++i) lh_apply(QR, i, tau[i], m, c, 1);
+            for (int i = np - 1; i >= 0; --i) {                  /* R c = Q^T b */
+                c[i] /= QR[IDX(i, i)];
+                for (int r = 0; r < i; ++r) c[r] -= c[i] * QR[IDX(r, i)];
+            }
+            for (int i = 0; i < n; ++i) y[i] = 0.0;
+            for (int i = 0; i < np; ++i) y[perm[i]] = c[i];
+            ++nls;
+            stats_solve(st, np);
+            int feasible = 1, rem = 0;                           /* :384-396 */
+            double alpha = DBL_MAX;
+            for (int i = 0; i < np; ++i) {
+                const int idx = perm[i];
+                if (y[idx] <= 0) {
+                    const double t = -x[idx] / (y[idx] - x[idx]);
+                    if (alpha > t) {
+                        alpha = t;
+                        rem = i;
+                    }
+                    feasible = 0;
+                }
+            }
+            if (feasible) {                                      /* :398-401 */
+                for (int i = 0; i < n; ++i) x[i] = y[i];
+                break;
+            }
+            for (int i = 0; i < np; ++i) {                       /* :403-407 */
+                const int idx = perm[i];
+                x[idx] += alpha * (y[idx] - x[idx]);
+            }
+            swap_i(perm, rem, np - 1);                           /* :443-456 */
+            --np;
+            for (int i = rem; i < np; ++i) lh_qr_update(QR, tau, A, perm[i], i, m);
+        }
+    }
+}
+
+int or_lh_nnls(const double *A, const double *b, double *x, int n, double eps, int max_iter,
+               or_nnls_stats *st)
+{
+    return lh_nnls_mn(A, b, x, n, n, eps, max_iter, st);
+}
+
 /* ------------------------------------------------------------------------ */
 /* per-channel multifit                                                      */
 /* ------------------------------------------------------------------------ */
@@ -601,6 +729,9 @@ static void nnls_step(fitctx *c)
         for (int i = 0; i < np; ++i) xx[i] = c->x[i];
     } else
     switch (c->o->solver) {
+    case OR_SOLVER_LH:                                    /* :253-257 */
+        if (!lh_nnls_mn(A, b, xx, ns, np, c->o->eps, (int)c->o->max_iter, &st)) st.hit_max = 1;
+        break;
     case OR_SOLVER_INPLACE_CHOL:
         inplace_fnnls_mn(A, b, xx, ns, np, c->o->eps, c->o->max_iter, &st, 1);
         break;

@@ -64,6 +64,13 @@ void or_inplace_fnnls_chol(const double *A, const double *b, double *x, int n,
 /* reference fnnls (nnls/src/fnnls.cu:28-173), bool-mask FNNLS. */
 void or_fnnls(const double *A, const double *b, double *x, int n, double eps,
               unsigned max_iter, or_nnls_stats *st);
+/* Lawson-Hanson NNLS of the reference's Eigen::NNLS class
+ * (nnls/interface/eigen_nnls.h:336-414): Householder QR of the passive
+ * columns, updated column by column.  x is in/out (setX warm start; the
+ * passive set starts empty); max_iter counts least-squares solves, <= 0 for
+ * no limit.  Returns 1 when solved, 0 when max_iter stopped it. */
+int or_lh_nnls(const double *A, const double *b, double *x, int n, double eps,
+               int max_iter, or_nnls_stats *st);
 
 /* ---- per-channel multifit (multifit_cpu/src/PulseChiSqSNNLS.cpp) ---- */
 
@@ -80,8 +87,10 @@ typedef struct {
 } or_chtype;
 
 enum { OR_SOLVER_INPLACE = 0, OR_SOLVER_INPLACE_CHOL = 1, OR_SOLVER_FNNLS = 2,
-       OR_SOLVER_LEGACY = 3 /* legacy_multifit_cpu's NNLS: passive set kept across
-                               Minimize iterations */ };
+       OR_SOLVER_LEGACY = 3, /* legacy_multifit_cpu's NNLS: passive set kept across
+                                Minimize iterations */
+       OR_SOLVER_LH = 4      /* Eigen::NNLS with setX(x): the alternative of
+                                PulseChiSqSNNLS.cpp:253-257 */ };
 
 typedef struct {
     double eps;          /* FNNLS convergence on max w (reference 1e-11)      */

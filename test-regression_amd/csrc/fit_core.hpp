@@ -180,7 +180,8 @@ struct FitParams {
     int32_t max_iter;    // FNNLS outer cap
     int32_t max_minimize;
     int32_t use_pi;      // amplitude-weighted pulse covariance live
-    int32_t solver;      // 0 inplace_fnnls, 1 fnnls, 2 legacy (persisted passive set)
+    int32_t solver;      // 0 inplace_fnnls, 1 fnnls, 2 legacy (persisted passive set),
+                         // 3 Lawson-Hanson (Eigen::NNLS)
     float eps_rel32;     // FP32 path: rounding allowance of w (see fnnls_inplace)
     float tau32;         // FP32 path: pivot ratio below which a solve goes to FP64
     double eps_rel64;    // FP32 path: rounding allowance of w in the FP64 phase
@@ -693,6 +694,95 @@ MF_HD void fnnls_legacy(const T (&AtA)[10][10], const T (&Atb)[10], T (&x)[10], 
     if (iter == max_iter && max_iter > 0) st.hit_max = true;
 }
 
+// Bound on the least-squares solves of lh_nnls when the caller sets none
+// (max_iter <= 0): the reference then loops until convergence; a GPU lane
+// needs an end.  Reaching it is reported like max_iter.
+constexpr int LH_SOLVE_CAP = 1000;
+
+// Lawson-Hanson NNLS with the semantics of the reference's Eigen::NNLS
+// class (nnls/interface/eigen_nnls.h:336-414).  The passive set starts
+// empty whatever x holds (x is the setX starting point, :124); the gradient
+// is taken on the whole x; the argmax over the free set takes the first
+// maximum in permuted order (:170-180) and stops when it is below eps
+// (:361); a feasible solve replaces all of x, free entries becoming 0
+// (:398-401); an infeasible one interpolates the passive entries and removes
+// one index, the first minimum ratio in permuted order (:384-396, :443-456),
+// without zeroing it.  max_iter bounds the least-squares solves (:375-377).
+// The reference solves the passive system by a Householder QR of A's passive
+// columns; here the masked Cholesky of AtA's passive block: the same
+// solution, different rounding.  Returns false when max_iter stopped it.
+template <typename T, int N>
+MF_HD bool lh_nnls(const T (&AtA)[10][10], const T (&Atb)[10], T (&x)[10], T eps, int max_iter,
+                   FnnlsStats &st)
+{
+    LegacyState L;   // passive set and permutation
+    L.reset();
+    const int cap = max_iter > 0 ? max_iter : LH_SOLVE_CAP;
+    int nls = 0;
+    for (;;) {
+        if (L.np == N) return true;                             // :361
+        T wmax = T(0);
+        int q = -1, qpos = 16;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            T t = T(0);
+#pragma unroll
+            for (int j = 0; j < N; ++j) t = mfma(sym(AtA, i, j), x[j], t);
+            const T w = Atb[i] - t;
+            const int pos = nib(L.ipos, i);
+            const bool better = (q < 0) || (w > wmax) || (w == wmax && pos < qpos);
+            if (!((L.pm >> i) & 1u) && better) {
+                wmax = w;
+                q = i;
+                qpos = pos;
+            }
+        }
+        if (wmax - eps < T(0)) return true;
+        {   // swap positions np <-> qpos                         :431-440
+            const int inp = nib(L.perm, L.np);
+            L.perm = setnib(setnib(L.perm, L.np, q), qpos, inp);
+            L.ipos = setnib(setnib(L.ipos, q, L.np), inp, qpos);
+        }
+        L.pm |= 1u << q;
+        ++L.np;
+        st.outer++;
+        for (;;) {
+            if (nls >= cap) {                                   // :375-377
+                st.hit_max = true;
+                return false;
+            }
+            T y[10];
+            if (!solve_passive<T, N>(AtA, Atb, L.pm, y)) st.llt_fail = true;
+            ++nls;
+            st.inner++;
+            bool feasible = true;
+            T alpha = Lim<T>::max();
+            int rpos = 16;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                if (((L.pm >> i) & 1u) && y[i] <= T(0)) {
+                    feasible = false;
+                    const T t = mdiv(-x[i], y[i] - x[i]);
+                    const int pos = nib(L.ipos, i);
+                    if (t < alpha || (t == alpha && pos < rpos)) {
+                        alpha = t;
+                        rpos = pos;
+                    }
+                }
+            }
+            if (feasible) {
+#pragma unroll
+                for (int i = 0; i < N; ++i) x[i] = y[i];
+                break;
+            }
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                if ((L.pm >> i) & 1u) x[i] = x[i] + alpha * (y[i] - x[i]);
+            L.drop(nib(L.perm, rpos));
+        }
+    }
+}
+
 // fnnls (nnls/src/fnnls.cu:28-173): boolean-mask FNNLS on the full x.
 // Its outer test `!n_active || w < eps` (:113) counts the active indices
 // minus one, so it stops as soon as a single index is still free, without
@@ -776,7 +866,11 @@ template <typename T> struct FitOut {
 // updateCov + LLT, NNLS on the whitened problem (warm x), chi2.  Li (L^{-1} of
 // the covariance) is parked in `park` between the normal equations and chi2;
 // Park is any type with store(k, v) / load(k) for k < 55.  Returns chi2.
-template <typename T, int NS, int NP, typename Park>
+// XS: solver compiled in besides inplace_fnnls / fnnls (chosen at run time
+// by prm.solver): XS_LEGACY (with ls) or XS_LH.
+enum { XS_NONE = 0, XS_LEGACY = 2, XS_LH = 3 };
+
+template <typename T, int NS, int NP, int XS = XS_NONE, typename Park>
 MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams &prm,
                       Park &park, T (&x)[10], FnnlsStats &st, int &flags, int zc = -1,
                       LegacyState *ls = nullptr)
@@ -792,10 +886,12 @@ MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams 
     for (int i = 0; i < NS; ++i)
 #pragma unroll
         for (int j = 0; j <= i; ++j) park.store(i * (i + 1) / 2 + j, Li[i][j]);
-    if (prm.solver == 1)
-        fnnls_mask<T, NP>(AtA, Atb, x, T(prm.eps), prm.max_iter, st);
-    else if (prm.solver == 2 && ls)
+    if (XS == XS_LEGACY && ls)
         fnnls_legacy<T, NP>(AtA, Atb, x, T(prm.eps), prm.max_iter, st, *ls);
+    else if (XS == XS_LH)   // PulseChiSqSNNLS.cpp:253-257 (setX, solve, x)
+        lh_nnls<T, NP>(AtA, Atb, x, T(prm.eps), prm.max_iter, st);
+    else if (prm.solver == 1)
+        fnnls_mask<T, NP>(AtA, Atb, x, T(prm.eps), prm.max_iter, st);
     else
         fnnls_inplace<T, NP>(AtA, Atb, x, T(prm.eps), T(0), prm.max_iter, st);
 #pragma unroll
@@ -807,14 +903,14 @@ MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams 
 
 // Minimize (PulseChiSqSNNLS.cpp:149-166) from iteration `it0` on, with the
 // previous chi2 `chi2` and amplitudes x.  Returns the iteration count.
-template <typename T, int NS, int NP, typename Park>
+template <typename T, int NS, int NP, int XS = XS_NONE, typename Park>
 MF_HD int minimize_from(const TypeMats<T> &tm, const T (&s)[10], const FitParams &prm,
                         Park &park, T (&x)[10], T &chi2, int it0, FnnlsStats &st, int &flags,
                         int zc = -1, LegacyState *ls = nullptr)
 {
     int it = it0;
     for (; it < prm.max_minimize; ++it) {                       // :154
-        const T now = minimize_step<T, NS, NP>(tm, s, prm, park, x, st, flags, zc, ls);
+        const T now = minimize_step<T, NS, NP, XS>(tm, s, prm, park, x, st, flags, zc, ls);
         const T d = now - chi2;
         chi2 = now;
         if (mabs(d) < T(prm.chi2_tol)) return it + 1;           // :162
@@ -845,7 +941,7 @@ MF_HD bool all_finite(const T (&s)[10])
 // pulse is passive it leaves the passive set first (:151-157 -- a branch the
 // current reference never takes, its _nP staying 0), and the two Minimize
 // calls continue from that state.
-template <typename T, int NS, int NP, typename Park>
+template <typename T, int NS, int NP, int XS = XS_NONE, typename Park>
 MF_HD T minos_error(const TypeMats<T> &tm, const T (&s)[10], const FitParams &prm, Park &park,
                     const T (&xfit)[10], T chi2_0, int ip, FnnlsStats &st, int &flags,
                     LegacyState *ls = nullptr)
@@ -876,7 +972,7 @@ MF_HD T minos_error(const TypeMats<T> &tm, const T (&s)[10], const FitParams &pr
     x[ip] = xplus;
 #pragma unroll
     for (int i = 0; i < 10; ++i) sr[i] = i < NS ? s[i] - xplus * tm.P[i][ip] : T(0);
-    minimize_from<T, NS, NP>(tm, sr, prm, park, x, chi2, 0, st, flags, ip, ls);
+    minimize_from<T, NS, NP, XS>(tm, sr, prm, park, x, chi2, 0, st, flags, ip, ls);
     T Li[10][10];
 #pragma unroll
     for (int i = 0; i < NS; ++i)
@@ -890,7 +986,7 @@ MF_HD T minos_error(const TypeMats<T> &tm, const T (&s)[10], const FitParams &pr
     x[ip] = xminus;
 #pragma unroll
     for (int i = 0; i < 10; ++i) sr[i] = i < NS ? s[i] - xminus * tm.P[i][ip] : T(0);
-    minimize_from<T, NS, NP>(tm, sr, prm, park, x, chi2, 0, st, flags, ip, ls);
+    minimize_from<T, NS, NP, XS>(tm, sr, prm, park, x, chi2, 0, st, flags, ip, ls);
 #pragma unroll
     for (int i = 0; i < NS; ++i)
 #pragma unroll
@@ -902,11 +998,12 @@ MF_HD T minos_error(const TypeMats<T> &tm, const T (&s)[10], const FitParams &pr
 
 // Plain fit: the reference's algorithm in precision T, cold start, and with
 // ERRORS the in-time uncertainty (ip >= 0: index of the in-time pulse).
-// LEGACY: the legacy NNLS (prm.solver must be 2), whose passive state lives
-// through the whole fit; a separate instantiation, so that the other solvers
-// do not carry its state in registers.
+// XS: XS_LEGACY for the legacy NNLS (prm.solver 2), whose passive state
+// lives through the whole fit, XS_LH for Lawson-Hanson (prm.solver 3);
+// separate instantiations, so that the default solvers do not carry their
+// state and code.
 // Non-finite samples are not fitted (x = 0, F_NONFINITE).
-template <typename T, int NS, int NP, bool ERRORS = false, bool LEGACY = false, typename Park>
+template <typename T, int NS, int NP, bool ERRORS = false, int XS = XS_NONE, typename Park>
 MF_HD void fit_plain(const TypeMats<T> &tm, const T (&s)[10], const FitParams &prm,
                      Park &park, FitOut<T> &out, int ip = -1)
 {
@@ -921,15 +1018,15 @@ MF_HD void fit_plain(const TypeMats<T> &tm, const T (&s)[10], const FitParams &p
     FnnlsStats st;
     LegacyState ls;   // legacy solver: passive state across Minimize iterations
     ls.reset();
-    LegacyState *lsp = LEGACY ? &ls : nullptr;
+    LegacyState *lsp = XS == XS_LEGACY ? &ls : nullptr;
     if (all_finite<T, NS>(s))
-        nmin = minimize_from<T, NS, NP>(tm, s, prm, park, x, chi2, 0, st, flags, -1, lsp);
+        nmin = minimize_from<T, NS, NP, XS>(tm, s, prm, park, x, chi2, 0, st, flags, -1, lsp);
     else
         flags |= F_NONFINITE;
     if (ERRORS && ip >= 0 && !(flags & F_NONFINITE) && prm.max_minimize > 0) {
         FnnlsStats st2;
         int fl2 = 0;
-        const T e = minos_error<T, NS, NP>(tm, s, prm, park, x, chi2, ip, st2, fl2, lsp);
+        const T e = minos_error<T, NS, NP, XS>(tm, s, prm, park, x, chi2, ip, st2, fl2, lsp);
 #pragma unroll
         for (int i = 0; i < 10; ++i)
             if (i == ip) out.err[i] = e;

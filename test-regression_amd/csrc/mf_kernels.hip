@@ -123,7 +123,7 @@ __device__ __forceinline__ bool seg_item(const SegArg &a, int &t, int64_t &c)
     return true;
 }
 
-template <typename T, typename TS, int N, bool ERRORS, bool LEGACY>
+template <typename T, typename TS, int N, bool ERRORS, int XS>
 __global__ __launch_bounds__(256)
 void k_fit_plain(const DevType *__restrict__ types, SegArg seg,
                  const TS *__restrict__ samples, int64_t nch, T *__restrict__ xo,
@@ -140,7 +140,7 @@ void k_fit_plain(const DevType *__restrict__ types, SegArg seg,
     for (int i = 0; i < 10; ++i) s[i] = i < N ? (T)samples[(int64_t)i * nch + c] : T(0);
     LdsPark<T> park{reinterpret_cast<T *>(lds_raw) + threadIdx.x, (int)blockDim.x};
     FitOut<T> out;
-    fit_plain<T, N, N, ERRORS, LEGACY>(tm, s, prm, park, out, types[type_id].h.intime);
+    fit_plain<T, N, N, ERRORS, XS>(tm, s, prm, park, out, types[type_id].h.intime);
 #pragma unroll
     for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = out.x[p];
     if (ERRORS) {
@@ -568,7 +568,8 @@ __global__ void k_type_scatter(const uint8_t *__restrict__ type_of, int64_t nch,
 // ---------------------------------------------------------------------------
 template <int M>
 __device__ void solve_one(const double *__restrict__ A, const double *__restrict__ b,
-                          double *__restrict__ x, int solver, double eps, unsigned max_iter)
+                          double *__restrict__ x, int solver, double eps, unsigned max_iter,
+                          int32_t *__restrict__ solved)
 {
     // A column-major, leading dimension 10: A[col*10 + row]
     double AtA[10][10], Atb[10], xx[10];
@@ -588,10 +589,14 @@ __device__ void solve_one(const double *__restrict__ A, const double *__restrict
         xx[i] = x[i];
     }
     FnnlsStats st;
-    if (solver == 1)
+    if (solver == 2) {
+        const bool ok = lh_nnls<double, M>(AtA, Atb, xx, eps, (int)max_iter, st);
+        if (solved) *solved = ok ? 1 : 0;
+    } else if (solver == 1) {
         fnnls_mask<double, M>(AtA, Atb, xx, eps, (int)max_iter, st);
-    else
+    } else {
         fnnls_inplace<double, M>(AtA, Atb, xx, eps, 0.0, (int)max_iter, st);
+    }
 #pragma unroll
     for (int i = 0; i < M; ++i) x[i] = xx[i];
 }
@@ -599,24 +604,25 @@ __device__ void solve_one(const double *__restrict__ A, const double *__restrict
 __global__ __launch_bounds__(256) void k_nnls(const double *__restrict__ A,
                                               const double *__restrict__ b, double *__restrict__ x,
                                               int64_t n, int m, int solver, double eps,
-                                              unsigned max_iter)
+                                              unsigned max_iter, int32_t *__restrict__ solved)
 {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     const double *Ac = A + c * 100;
     const double *bc = b + c * 10;
     double *xc = x + c * 10;
+    int32_t *ok = solved ? solved + c : nullptr;
     switch (m) {
-    case 1: solve_one<1>(Ac, bc, xc, solver, eps, max_iter); break;
-    case 2: solve_one<2>(Ac, bc, xc, solver, eps, max_iter); break;
-    case 3: solve_one<3>(Ac, bc, xc, solver, eps, max_iter); break;
-    case 4: solve_one<4>(Ac, bc, xc, solver, eps, max_iter); break;
-    case 5: solve_one<5>(Ac, bc, xc, solver, eps, max_iter); break;
-    case 6: solve_one<6>(Ac, bc, xc, solver, eps, max_iter); break;
-    case 7: solve_one<7>(Ac, bc, xc, solver, eps, max_iter); break;
-    case 8: solve_one<8>(Ac, bc, xc, solver, eps, max_iter); break;
-    case 9: solve_one<9>(Ac, bc, xc, solver, eps, max_iter); break;
-    default: solve_one<10>(Ac, bc, xc, solver, eps, max_iter); break;
+    case 1: solve_one<1>(Ac, bc, xc, solver, eps, max_iter, ok); break;
+    case 2: solve_one<2>(Ac, bc, xc, solver, eps, max_iter, ok); break;
+    case 3: solve_one<3>(Ac, bc, xc, solver, eps, max_iter, ok); break;
+    case 4: solve_one<4>(Ac, bc, xc, solver, eps, max_iter, ok); break;
+    case 5: solve_one<5>(Ac, bc, xc, solver, eps, max_iter, ok); break;
+    case 6: solve_one<6>(Ac, bc, xc, solver, eps, max_iter, ok); break;
+    case 7: solve_one<7>(Ac, bc, xc, solver, eps, max_iter, ok); break;
+    case 8: solve_one<8>(Ac, bc, xc, solver, eps, max_iter, ok); break;
+    case 9: solve_one<9>(Ac, bc, xc, solver, eps, max_iter, ok); break;
+    default: solve_one<10>(Ac, bc, xc, solver, eps, max_iter, ok); break;
     }
 }
 
@@ -977,9 +983,13 @@ static mf_status launch_fit(mf_context *ctx, const SegArg &seg, const void *samp
     if (nt <= 0) return MF_OK;
     const int64_t grid = (nt + block - 1) / block;
     const size_t lds = sizeof(T) * 55 * (size_t)block;
-    const bool legacy = prm.solver == MF_SOLVER_LEGACY;
-    auto kern = errors ? (legacy ? k_fit_plain<T, TS, N, true, true> : k_fit_plain<T, TS, N, true, false>)
-                       : (legacy ? k_fit_plain<T, TS, N, false, true> : k_fit_plain<T, TS, N, false, false>);
+    auto pick = [&](auto e) {
+        constexpr bool E = decltype(e)::value;
+        return prm.solver == MF_SOLVER_LEGACY ? k_fit_plain<T, TS, N, E, XS_LEGACY>
+               : prm.solver == MF_SOLVER_LAWSON_HANSON ? k_fit_plain<T, TS, N, E, XS_LH>
+                                                       : k_fit_plain<T, TS, N, E, XS_NONE>;
+    };
+    auto kern = errors ? pick(std::true_type{}) : pick(std::false_type{});
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), lds, st, ctx->d_types, seg,
                        static_cast<const TS *>(samples), nch, static_cast<T *>(out->x),
                        static_cast<T *>(out->chi2), out->flags, out->iters,
@@ -1113,7 +1123,7 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
     if (opts->max_minimize < 0 || opts->max_iter < 0)
         return fail(MF_ERR_INVALID, "max_minimize and max_iter must be >= 0");
     if (opts->solver != MF_SOLVER_INPLACE_FNNLS && opts->solver != MF_SOLVER_FNNLS &&
-        opts->solver != MF_SOLVER_LEGACY)
+        opts->solver != MF_SOLVER_LEGACY && opts->solver != MF_SOLVER_LAWSON_HANSON)
         return fail(MF_ERR_INVALID, "unknown solver");
     if (opts->warm_start && opts->precision != MF_PREC_FP32)
         return fail(MF_ERR_UNSUPPORTED, "warm_start needs MF_PREC_FP32");
@@ -1516,7 +1526,8 @@ extern "C" mf_status mf_get_timings(mf_context *ctx, mf_timings *out)
 }
 
 static mf_status nnls_batch(const double *A, const double *b, double *x, int64_t n, int32_t m,
-                            double eps, uint32_t max_iter, int solver, void *stream)
+                            double eps, uint32_t max_iter, int solver, void *stream,
+                            int32_t *solved = nullptr)
 {
     if (n < 0) return fail(MF_ERR_INVALID, "n < 0");
     if (n == 0) return MF_OK;
@@ -1524,7 +1535,8 @@ static mf_status nnls_batch(const double *A, const double *b, double *x, int64_t
     if (m < 1 || m > 10) return fail(MF_ERR_INVALID, "m must be in 1..10");
     const int block = 64;
     hipLaunchKernelGGL(k_nnls, dim3((unsigned)((n + block - 1) / block)), dim3(block), 0,
-                       static_cast<hipStream_t>(stream), A, b, x, n, m, solver, eps, max_iter);
+                       static_cast<hipStream_t>(stream), A, b, x, n, m, solver, eps, max_iter,
+                       solved);
     HIP_TRY(hipGetLastError());
     return MF_OK;
 }
@@ -1540,6 +1552,14 @@ extern "C" mf_status mf_fnnls_batch(const double *A, const double *b, double *x,
                                     int32_t m, double eps, uint32_t max_iter, void *stream)
 {
     return nnls_batch(A, b, x, n, m, eps, max_iter, 1, stream);
+}
+
+extern "C" mf_status mf_lh_nnls_batch(const double *A, const double *b, double *x, int64_t n,
+                                      int32_t m, double eps, int32_t max_iter, int32_t *solved,
+                                      void *stream)
+{
+    return nnls_batch(A, b, x, n, m, eps, (uint32_t)(max_iter > 0 ? max_iter : 0), 2, stream,
+                      solved);
 }
 
 extern "C" mf_status mf_linalg_batch(int32_t op, int32_t dtype, void *M, void *L, void *b, void *y,

@@ -270,6 +270,23 @@ def nnls_batch(A: torch.Tensor, b: torch.Tensor, x: torch.Tensor, m: int = 10,
     return x
 
 
+def lh_nnls_batch(A: torch.Tensor, b: torch.Tensor, x: torch.Tensor, m: int = 10,
+                  eps: float = 1e-10, max_iter: int = -1, stream=None):
+    """Batched Lawson-Hanson NNLS (Eigen::NNLS semantics, mf_lh_nnls_batch);
+    layout as nnls_batch, x in/out (the setX starting point; eps default as
+    Eigen::NNLS's constructor).  Returns (x, solved) with solved [n] int32,
+    1 where the solve converged before max_iter least-squares solves."""
+    L = _abi.lib()
+    for t in (A, b, x):
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+            raise ValueError("A, b, x must be contiguous float64 device tensors")
+    solved = torch.zeros(A.shape[0], dtype=torch.int32, device=A.device)
+    _abi.check(L.mf_lh_nnls_batch(A.data_ptr(), b.data_ptr(), x.data_ptr(), A.shape[0], m, eps,
+                                  max_iter, C.c_void_p(solved.data_ptr()),
+                                  C.c_void_p(_stream_ptr(stream))), "mf_lh_nnls_batch")
+    return x, solved
+
+
 def linalg_batch(op: int, M=None, L=None, b=None, y=None, full: int = 10, view: int = 10,
                  i: int = 0, j: int = 0, stream=None):
     """Batched linalgebra primitive (see mf_linalg_batch)."""

@@ -48,13 +48,17 @@ static void run(const DevType &dt, const double *samples, int64_t nch, int64_t c
     for (int i = 0; i < 10; ++i) s[i] = i < N ? (T)samples[i * nch + c] : T(0);
     RegPark<T> park;
     FitOut<T> out;
-    const bool legacy = prm.solver == 2;
-    if (g_err && legacy)
-        fit_plain<T, N, N, true, true>(mats<T>(dt), s, prm, park, out, dt.h.intime);
+    const int ip = g_err ? dt.h.intime : -1;
+    if (g_err && prm.solver == 2)
+        fit_plain<T, N, N, true, XS_LEGACY>(mats<T>(dt), s, prm, park, out, ip);
+    else if (g_err && prm.solver == 3)
+        fit_plain<T, N, N, true, XS_LH>(mats<T>(dt), s, prm, park, out, ip);
     else if (g_err)
-        fit_plain<T, N, N, true>(mats<T>(dt), s, prm, park, out, dt.h.intime);
-    else if (legacy)
-        fit_plain<T, N, N, false, true>(mats<T>(dt), s, prm, park, out);
+        fit_plain<T, N, N, true>(mats<T>(dt), s, prm, park, out, ip);
+    else if (prm.solver == 2)
+        fit_plain<T, N, N, false, XS_LEGACY>(mats<T>(dt), s, prm, park, out);
+    else if (prm.solver == 3)
+        fit_plain<T, N, N, false, XS_LH>(mats<T>(dt), s, prm, park, out);
     else
         fit_plain<T, N, N>(mats<T>(dt), s, prm, park, out);
     for (int p = 0; p < 10; ++p) x[p * nch + c] = p < N ? (double)out.x[p] : 0.0;
@@ -135,8 +139,10 @@ extern "C" int hc_fit_batch(const mf_chtype *types, int ntypes, const uint8_t *t
 }
 
 // single NNLS problems (A column-major ld 10, m x m) through fit_core's solvers
-extern "C" void hc_nnls(const double *A, const double *b, double *x, int m, int solver,
-                        double eps, unsigned max_iter)
+// (0 inplace_fnnls, 1 fnnls, 2 Lawson-Hanson with max_iter as a signed
+// solve count).  Returns 0 when Lawson-Hanson stopped at max_iter, else 1.
+extern "C" int hc_nnls(const double *A, const double *b, double *x, int m, int solver,
+                       double eps, unsigned max_iter)
 {
     double AtA[10][10] = {{0}}, Atb[10] = {0}, xx[10] = {0};
     for (int i = 0; i < m; ++i) {
@@ -151,12 +157,15 @@ extern "C" void hc_nnls(const double *A, const double *b, double *x, int m, int 
         xx[i] = x[i];
     }
     FnnlsStats st;
+    bool ok = true;
 #define CASE(M)                                                                   \
     case M:                                                                       \
-        if (solver == 1) fnnls_mask<double, M>(AtA, Atb, xx, eps, (int)max_iter, st); \
+        if (solver == 2) ok = lh_nnls<double, M>(AtA, Atb, xx, eps, (int)max_iter, st); \
+        else if (solver == 1) fnnls_mask<double, M>(AtA, Atb, xx, eps, (int)max_iter, st); \
         else fnnls_inplace<double, M>(AtA, Atb, xx, eps, 0.0, (int)max_iter, st); \
         break;
     switch (m) { CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) }
 #undef CASE
     for (int i = 0; i < m; ++i) x[i] = xx[i];
+    return ok ? 1 : 0;
 }

@@ -53,7 +53,8 @@ class OrStats(C.Structure):
                 ("k3sum", C.c_longlong), ("aflops", C.c_double)]
 
 
-OR_SOLVER_INPLACE, OR_SOLVER_INPLACE_CHOL, OR_SOLVER_FNNLS, OR_SOLVER_LEGACY = 0, 1, 2, 3
+OR_SOLVER_INPLACE, OR_SOLVER_INPLACE_CHOL, OR_SOLVER_FNNLS, OR_SOLVER_LEGACY, OR_SOLVER_LH = \
+    0, 1, 2, 3, 4
 
 _cache: dict = {}
 
@@ -83,6 +84,8 @@ def oracle():
     for f in ("or_inplace_fnnls", "or_inplace_fnnls_chol", "or_fnnls"):
         getattr(L, f).argtypes = [dp, dp, dp, C.c_int, C.c_double, C.c_uint,
                                   C.POINTER(OrStats)]
+    L.or_lh_nnls.argtypes = [dp, dp, dp, C.c_int, C.c_double, C.c_int, C.POINTER(OrStats)]
+    L.or_lh_nnls.restype = C.c_int
     L.or_build_pulsemat.argtypes = [C.c_void_p, dp]
     L.or_dofit.argtypes = [C.c_void_p, dp, C.POINTER(OrOpts), C.POINTER(OrResult)]
     L.or_dofit.restype = C.c_int
@@ -111,6 +114,7 @@ def hostcore():
                                dp, dp, C.c_void_p, C.c_void_p, dp]
     L.hc_fit_batch.restype = C.c_int
     L.hc_nnls.argtypes = [dp, dp, dp, C.c_int, C.c_int, C.c_double, C.c_uint]
+    L.hc_nnls.restype = C.c_int
     L.hc_mask_inverse.argtypes = [C.c_void_p, C.c_int, dp, dp]
     L.hc_mask_inverse.restype = C.c_int
     L.hc_set_use_tables.argtypes = [C.c_int]
