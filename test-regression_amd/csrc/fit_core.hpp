@@ -416,9 +416,10 @@ constexpr TriIndex TRI{};
 
 // s = inv b for the passive set pm, inv read from a MaskTab row (see there):
 // s_i = sum_j inv_ij b_j over the packed lower triangle, each entry used for
-// both (i, j) and (j, i).  The row is streamed in 16-byte chunks, two stages
-// in flight, so a solve costs about one memory round trip.  Returns the
-// block's pivot ratio (the row's last slot).
+// both (i, j) and (j, i).  A row is zero outside its passive block, so b
+// needs no masking and s is zero outside P.  The row is streamed in 16-byte
+// chunks, two stages of S chunks in flight.  Returns the block's pivot ratio
+// (the row's last slot).
 template <int N>
 MF_HD double solve_inverse(const double *row, const double (&b)[10], uint32_t pm,
                            double (&s)[10])
@@ -429,13 +430,10 @@ MF_HD double solve_inverse(const double *row, const double (&b)[10], uint32_t pm
     constexpr int S = 4;                 // chunks per stage
     constexpr int NST = (NCU + S - 1) / S;
     static_assert(NE <= MASK_PACK - 1 && NCU <= NC, "packing");
+    (void)pm;
     const GlobalPtr<Dbl2> p = (GlobalPtr<Dbl2>)row;
-    double bm[10];
 #pragma unroll
-    for (int k = 0; k < 10; ++k) {
-        bm[k] = (k < N && ((pm >> k) & 1u)) ? b[k] : 0.0;
-        s[k] = 0.0;
-    }
+    for (int k = 0; k < 10; ++k) s[k] = 0.0;
     Dbl2 buf[2][S];
     double ratio = 0.0;
 #pragma unroll
@@ -455,20 +453,20 @@ MF_HD double solve_inverse(const double *row, const double (&b)[10], uint32_t pm
                 const double v = buf[st & 1][c][h];
                 if (e < NE) {
                     const int i = TRI.i[e], j = TRI.j[e];
-                    s[i] = fma(v, bm[j], s[i]);
-                    if (i != j) s[j] = fma(v, bm[i], s[j]);
+                    s[i] = fma(v, b[j], s[i]);
+                    if (i != j) s[j] = fma(v, b[i], s[j]);
                 } else if (e == MASK_PACK - 1) {
                     ratio = v;
                 }
             }
         }
         if (st + 2 < NST) {
-            // the address of stage st + 2 depends on this stage's result, so
-            // its loads issue after this stage's use: at most two stages
-            // (8 chunks) are in flight per lane
+            // the address of stage st + 2 depends on the last chunk of this
+            // stage, so its loads issue once this stage has arrived: at most
+            // two stages (2 S chunks) are in flight per lane
             GlobalPtr<Dbl2> q = p;
 #if defined(__HIP_DEVICE_COMPILE__)
-            asm volatile("" : "+v"(q) : "v"(s[0]));
+            asm volatile("" : "+v"(q) : "v"(buf[st & 1][S - 1]));
 #endif
 #pragma unroll
             for (int c = 0; c < S; ++c)
@@ -1327,8 +1325,9 @@ MF_HD void fast_phase2(const DevType &ty, const float (&s32)[10], const FitParam
     double chi2 = 0.0;
     if (fast_p2_begin<NS, NP>(ty, s32, prm, x, Atb64, S, flags)) {
         const int outer0 = st.outer;
+        const MaskTab *tab = *opaque(&ty.tab);   // wave-uniform: one scalar load
         while (!fnnls_step<double, NP, true, TAB>(ty.d.AtA0, Atb64, S, prm.eps, prm.eps_rel64,
-                                                  prm.max_iter, st, 0.0, nullptr, ty.tab)) {
+                                                  prm.max_iter, st, 0.0, nullptr, tab)) {
         }
         flags |= F_REFINED;
         if (warm && st.outer != outer0) flags |= F_WARM_REJECT;

@@ -454,7 +454,7 @@ void k_fast_p2q(const DevType *__restrict__ types, int type_id, Seg seg,
         }
         if (c >= 0 &&
             fnnls_step<double, N, true, true>(ty.d.AtA0, Atb, S, prm.eps, prm.eps_rel64,
-                                              prm.max_iter, st, 0.0, nullptr, ty.tab)) {
+                                              prm.max_iter, st, 0.0, nullptr, *opaque(&ty.tab))) {
             flags |= F_REFINED;
             if (warm && st.outer != outer0) flags |= F_WARM_REJECT;
             // the samples are read again here rather than held through the steps

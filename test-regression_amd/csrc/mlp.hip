@@ -324,17 +324,21 @@ __device__ __forceinline__ float mlp_features(const float (&v)[10], int ns, __bf
     return sc;
 }
 
+struct NoTileEnd {
+    __device__ void operator()(int) const {}
+};
+
 // The network on the 64 channels of a wavefront, as 4 tiles of 16 channels.
 // xs: the wavefront's features (mlp_features), L: the staged network.
 // emit(t, o, z) is called for the lane's channel t (0..63) and outputs
 // o = 16 mt + 4 g + j < 20 with z = the output pre-activation (logits for
-// o < 10; the amplitude head's softplus is left to the caller).
-template <typename Emit>
+// o < 10; the amplitude head's softplus is left to the caller); tile_end(t)
+// after the last output of the lane's channel t of a tile.
+template <typename Emit, typename TileEnd = NoTileEnd>
 __device__ __forceinline__ void mlp_tiles(const MlpLds &L, const __bf16 (*xs)[16], int lane,
-                                          Emit &&emit)
+                                          Emit &&emit, TileEnd &&tile_end = NoTileEnd{})
 {
     const int g = lane >> 4, n = lane & 15;
-    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll 1
     for (int nt = 0; nt < 4; ++nt) {
         // an opaque zero offset per tile: the fragments are read from LDS
@@ -342,28 +346,30 @@ __device__ __forceinline__ void mlp_tiles(const MlpLds &L, const __bf16 (*xs)[16
         int w0 = 0;
         asm volatile("" : "+s"(w0));
         const bf16x4 bx = *reinterpret_cast<const bf16x4 *>(&xs[nt * 16 + n][4 * g]);
+        // The bias enters as the accumulator operand: in the transposed form
+        // the lane's accumulator j is output unit 16 mt + 4 g + j.
         f32x4 d[4];
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
             const bf16x4 a = *reinterpret_cast<const bf16x4 *>(&L.w[w0 + FRAG_W1 + mt * 256 + lane * 4]);
+            const f32x4 bb = *reinterpret_cast<const f32x4 *>(&L.b[mt * 16 + 4 * g]);
             d[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
                 __builtin_bit_cast(short __attribute__((ext_vector_type(4))), a),
-                __builtin_bit_cast(short __attribute__((ext_vector_type(4))), bx), zero, 0, 0, 0);
+                __builtin_bit_cast(short __attribute__((ext_vector_type(4))), bx), bb, 0, 0, 0);
         }
         bf16x8 h[2];
 #pragma unroll
         for (int layer = 0; layer < 2; ++layer) {
-            // bias + GELU of the 4 tiles, regrouped as the 2 K blocks of the next layer
+            // GELU of the 4 tiles, regrouped as the 2 K blocks of the next layer
 #pragma unroll
             for (int kb = 0; kb < 2; ++kb) {
                 bf16x4 lo, hi;
 #pragma unroll
                 for (int half = 0; half < 2; ++half) {
                     const int mt = 2 * kb + half;
-                    const f32x4 bb = *reinterpret_cast<const f32x4 *>(&L.b[64 * layer + mt * 16 + 4 * g]);
                     f32x4 v;
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) v[j] = gelu(d[mt][j] + bb[j]);
+                    for (int j = 0; j < 4; ++j) v[j] = gelu(d[mt][j]);
                     (half ? hi : lo) = cvt4(v);
                 }
                 h[kb] = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
@@ -373,7 +379,7 @@ __device__ __forceinline__ void mlp_tiles(const MlpLds &L, const __bf16 (*xs)[16
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) {
                 if (mt >= nmt) break;
-                f32x4 acc = zero;
+                f32x4 acc = *reinterpret_cast<const f32x4 *>(&L.b[64 * (layer + 1) + mt * 16 + 4 * g]);
 #pragma unroll
                 for (int kb = 0; kb < 2; ++kb) {
                     const bf16x8 a = *reinterpret_cast<const bf16x8 *>(
@@ -385,14 +391,22 @@ __device__ __forceinline__ void mlp_tiles(const MlpLds &L, const __bf16 (*xs)[16
         }
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
-            const f32x4 bb = *reinterpret_cast<const f32x4 *>(&L.b[128 + mt * 16 + 4 * g]);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int o = mt * 16 + 4 * g + j;
-                if (o < 20) emit(nt * 16 + n, o, d[mt][j] + bb[j]);
+                if (o < 20) emit(nt * 16 + n, o, d[mt][j]);
             }
         }
+        tile_end(nt * 16 + n);
     }
+}
+
+// softplus(z) = max(z, 0) + ln 2 log2(1 + 2^(-|z| log2 e)) with the hardware
+// exp2 and log2 (the log's argument lies in (1, 2])
+__device__ __forceinline__ float softplus_hw(float z)
+{
+    const float e = __builtin_amdgcn_exp2f(-fabsf(z) * 1.4426950408889634f);
+    return fmaf(0.6931471805599453f, __builtin_amdgcn_logf(1.0f + e), fmaxf(z, 0.0f));
 }
 
 // out: [20][ldo] float: logits 0..9, then amplitudes (softplus * m) 10..19,
@@ -438,6 +452,10 @@ __global__ __launch_bounds__(INFER_BLOCK) void k_mlp_infer(
 // Warm-start form of the prediction, for the fit kernels: pm[c] = the
 // passive-set bits {i: logit_i > 0}, px[i][c] = the predicted amplitude
 // (softplus * m) of pulse i, for channels c = 0..n-1.
+// The tiles leave each output with the lane that computed it (4 lanes per
+// channel); the outputs go through LDS (pz: softplus by pulse and channel,
+// pmw: logit bits by channel) so that each lane stores its own channel's 10
+// amplitudes as 10 coalesced rows.  The per-output work is branch-free.
 template <typename TS>
 __global__ __launch_bounds__(INFER_BLOCK) void k_mlp_warm(
     const TS *__restrict__ s, int64_t ld, int ns, int64_t n, const uint16_t *__restrict__ frag,
@@ -445,11 +463,12 @@ __global__ __launch_bounds__(INFER_BLOCK) void k_mlp_warm(
 {
     __shared__ MlpLds L;
     __shared__ __bf16 xs[INFER_BLOCK / 64][64][16];
-    __shared__ float ms[INFER_BLOCK / 64][64];
+    __shared__ float pz[INFER_BLOCK / 64][10][64];
     __shared__ uint32_t pmw[INFER_BLOCK / 64][64];
     mlp_stage(L, frag, pq, threadIdx.x, INFER_BLOCK);
     __syncthreads();
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t valid = (1u << (ns < 10 ? ns : 10)) - 1u;
     for (int64_t base = ((int64_t)blockIdx.x * (INFER_BLOCK / 64) + wv) * 64; base < n;
          base += (int64_t)gridDim.x * INFER_BLOCK) {
         const int64_t c = base + lane;
@@ -457,22 +476,34 @@ __global__ __launch_bounds__(INFER_BLOCK) void k_mlp_warm(
         float v[10];
 #pragma unroll
         for (int i = 0; i < 10; ++i) v[i] = (live && i < ns) ? (float)s[(int64_t)i * ld + c] : 0.0f;
-        ms[wv][lane] = mlp_features(v, ns, xs[wv], lane);
+        const float sc = mlp_features(v, ns, xs[wv], lane);
         pmw[wv][lane] = 0u;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        mlp_tiles(L, xs[wv], lane, [&](int t, int o, float z) {
-            if (o < 10) {
-                if (z > 0.0f && o < ns) atomicOr(&pmw[wv][t], 1u << o);
-            } else if (base + t < n) {
-                px[(int64_t)(o - 10) * ld + base + t] = softplusf(z) * ms[wv][t];
-            }
-        });
+        uint32_t bits = 0;
+        mlp_tiles(
+            L, xs[wv], lane,
+            [&](int t, int o, float z) {
+                bits |= (o < 10 && z > 0.0f) ? 1u << o : 0u;
+                const float sp = softplus_hw(z);
+                if (o >= 10) pz[wv][o - 10][t] = sp;
+            },
+            [&](int t) {
+                if (bits) atomicOr(&pmw[wv][t], bits & valid);
+                bits = 0;
+            });
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (live) ppm[c] = pmw[wv][lane];
+        if (live) {
+            ppm[c] = pmw[wv][lane];
+#pragma unroll
+            for (int i = 0; i < 10; ++i) px[(int64_t)i * ld + c] = pz[wv][i][lane] * sc;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
