@@ -39,7 +39,7 @@ sys.path.insert(0, str(ROOT / "tests"))
 
 PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
 
-# HBM bytes per launch of a warm step's fit kernels (k_fast_p1 + k_fast_p2)
+# HBM bytes per launch of a warm step's fit kernel (k_fast_fused)
 # on the default workload, from two rocprofv3 --pmc runs of
 # `bench.py --traffic-probe`: FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, the
 # factors that make the counters read a 256 MiB device copy as 256 MiB each
@@ -47,6 +47,13 @@ PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
 MEASURED_TRAFFIC = {("c2", "zero", 1_000_000): {
     "bytes": 273.3e6, "how": "rocprofv3 --pmc FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00), "
                              "bench.py --traffic-probe, copy-calibrated"}}
+
+
+def fit_kernel_name(pulse_cov: str) -> str:
+    """The fast path's fit kernels for a pulse-covariance mode."""
+    if pulse_cov == "live":
+        return "k_fast_p1 + k_fast_p2 (the step's fit kernels)"
+    return "k_fast_fused (both phases in one launch)"
 
 
 def log(msg):
@@ -195,7 +202,7 @@ def event_stream(args, eng, w, log):
                   "latency_ms_p99": float(np.quantile(lat_e, 0.99))},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["f32"],
                      "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f32"], "traffic": None,
-                     "kernel": "k_fast_p1 + k_fast_p2, both types", "kernel_ms": fit_ms,
+                     "kernel": "k_fast_fused, both types", "kernel_ms": fit_ms,
                      "aflops_per_channel": af},
         "accuracy_vs_plain_fp64": acc,
     }
@@ -354,7 +361,7 @@ def main():
                                  {}).get("bytes"),
                      "traffic_how": (MEASURED_TRAFFIC.get((args.config, args.pulse_cov, nch)) or
                                      {}).get("how"),
-                     "kernel": "k_fast_p1 + k_fast_p2 (the step's fit kernels)",
+                     "kernel": fit_kernel_name(args.pulse_cov),
                      "kernel_ms": kern_ms,
                      "aflops_per_channel": af},
         "plain_fp64": {"channels_per_s": nch / (res["plain"]["ms"] / 1e3),

@@ -152,12 +152,14 @@ void k_fit_plain(const DevType *__restrict__ types, SegArg seg,
     if (ito) ito[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
 }
 
-// Fast path (MF_PREC_FP32), two launches (see fast_phase1 / fast_phase2):
-// k_fast_p1 runs the FP32 active-set exploration and leaves its amplitudes in
-// the output x (FP32), its flags and counters in flags / iters; k_fast_p2
-// reads them back, finishes the fit in FP64 and writes the final outputs.
-// Phase 1 needs about half the registers of phase 2, so splitting them lets
-// the long FP32 phase run at twice the occupancy.
+// Fast path (MF_PREC_FP32), phases fast_phase1 (FP32 active-set
+// exploration) and fast_phase2 (FP64 finish).  Types without a live pulse
+// covariance run both in one launch (k_fast_fused).  With a live covariance
+// phase 2 runs the reference's later Minimize iterations in FP64 and needs
+// about twice the registers of phase 1, so it is a launch of its own:
+// k_fast_p1 leaves its amplitudes in the output x (FP32), its flags and
+// counters in flags / iters; k_fast_p2 reads them back, finishes the fit and
+// writes the final outputs.
 // Warm start (WarmIn): the MLP's predicted passive set pm[c] and amplitudes
 // x[i][c]; a predicted passive value starts at max(x, 1e-6) (phase 1 needs a
 // feasible point).  Null pointers: cold start.
@@ -205,6 +207,48 @@ void k_fast_p1(const DevType *__restrict__ types, SegArg seg,
     for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = ((pm >> p) & 1u) ? x[p] : 0.0f;
     flo[c] = flags | (st.hit_max ? F_FNNLS_CAP : 0) | (st.llt_fail ? F_LLT_FAIL : 0);
     ito[c] = (st.outer << 8) | (st.inner << 16);
+}
+
+// Both phases in one launch, for types whose pulse covariance is not live:
+// the FP64 phase of one wavefront overlaps the FP32 phase of others, and the
+// hand-over through x / flags / iters stays in registers.  Results are
+// identical to k_fast_p1 + k_fast_p2.
+struct NoPark {
+    __device__ void store(int, double) {}
+    __device__ double load(int) const { return 0.0; }
+};
+// Four waves per SIMD: at 128 VGPRs a few values spill (about 56 bytes per
+// lane, outside the solve loops); measured faster than 137 VGPRs at three.
+template <typename TS, int N>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void k_fast_fused(const DevType *__restrict__ types, SegArg seg,
+                  const TS *__restrict__ samples, int64_t nch, float *__restrict__ xo,
+                  float *__restrict__ chi2o, int32_t *__restrict__ flo,
+                  int32_t *__restrict__ ito, WarmIn warm, FitParams prm)
+{
+    int type_id;
+    int64_t c;
+    if (!seg_item(seg, type_id, c)) return;
+    float s[10], x[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) s[i] = i < N ? (float)samples[(int64_t)i * nch + c] : 0.0f;
+    uint32_t pm = warm_load<N>(warm, c, nch, x);
+    int flags = 0;
+    FnnlsStats st;
+    const DevType &ty = types[type_id];
+    if (all_finite<float, N>(s) && prm.max_minimize > 0)
+        fast_phase1<N, N>(ty, s, prm, x, pm, st, flags);
+#pragma unroll
+    for (int p = 0; p < 10; ++p) x[p] = (p < N && ((pm >> p) & 1u)) ? x[p] : 0.0f;
+    NoPark park;
+    FitOut<float> out;
+    fast_phase2<N, N, false>(ty, s, prm, x, warm.pm != nullptr, st, flags & F_FP64_PIVOT, park,
+                             out);
+#pragma unroll
+    for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = out.x[p];
+    chi2o[c] = out.chi2;
+    flo[c] = out.flags;
+    ito[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
 }
 
 template <typename TS, int N, bool LIVE>
@@ -1027,6 +1071,14 @@ static mf_status launch_fast(mf_context *ctx, const SegArg &seg, const void *sam
         return MF_OK;
     }
     const int64_t grid = (nt + block - 1) / block;
+    if (!LIVE) {
+        hipLaunchKernelGGL((k_fast_fused<TS, N>), dim3((unsigned)grid), dim3(block), 0, st,
+                           ctx->d_types, seg, static_cast<const TS *>(samples), nch,
+                           static_cast<float *>(out->x), static_cast<float *>(out->chi2),
+                           out->flags, out->iters, warm, prm);
+        HIP_TRY(hipGetLastError());
+        return MF_OK;
+    }
     hipLaunchKernelGGL((k_fast_p1<TS, N>), dim3((unsigned)grid), dim3(block), 0, st,
                        ctx->d_types, seg, static_cast<const TS *>(samples), nch,
                        static_cast<float *>(out->x), out->flags, out->iters, warm, prm);
