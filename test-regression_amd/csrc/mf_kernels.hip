@@ -20,6 +20,7 @@
 #include "../../include/mf_abi.h"
 #include "fit_core.hpp"
 #include "mlp.hpp"
+#include "mlp_device.hpp"
 #include "types_host.hpp"
 
 using namespace mf;
@@ -244,6 +245,56 @@ void k_fast_fused(const DevType *__restrict__ types, SegArg seg,
     FitOut<float> out;
     fast_phase2<N, N, false>(ty, s, prm, x, warm.pm != nullptr, st, flags & F_FP64_PIVOT, park,
                              out);
+#pragma unroll
+    for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = out.x[p];
+    chi2o[c] = out.chi2;
+    flo[c] = out.flags;
+    ito[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
+}
+
+// The warm-started fast path in one launch: the MLP's prediction for the
+// block's channels (the network staged in LDS once per block, the forward
+// pass on MFMA, as k_mlp_warm), then both fit phases as k_fast_fused.  The
+// prediction never leaves the chip.  Blocks of FAST_WARM_BLOCK threads.
+constexpr int FAST_WARM_BLOCK = 256;
+template <typename TS, int N>
+__global__ __launch_bounds__(FAST_WARM_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+void k_fast_warm(const DevType *__restrict__ types, SegArg seg,
+                 const TS *__restrict__ samples, int64_t nch, float *__restrict__ xo,
+                 float *__restrict__ chi2o, int32_t *__restrict__ flo,
+                 int32_t *__restrict__ ito, const uint16_t *__restrict__ frag,
+                 const float *__restrict__ pq, FitParams prm)
+{
+    __shared__ mlp::MlpLds L;
+    __shared__ __bf16 xs[FAST_WARM_BLOCK / 64][64][16];
+    __shared__ float pz[FAST_WARM_BLOCK / 64][10][64];
+    __shared__ uint32_t pmw[FAST_WARM_BLOCK / 64][64];
+    mlp::mlp_stage(L, frag, pq, threadIdx.x, FAST_WARM_BLOCK);
+    __syncthreads();
+    int type_id = 0;
+    int64_t c = 0;
+    const bool live = seg_item(seg, type_id, c);
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float s[10], xp[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) s[i] = (live && i < N) ? (float)samples[(int64_t)i * nch + c] : 0.0f;
+    // every lane of the wavefront takes part in the forward pass
+    uint32_t pm = mlp::mlp_warm_wave(L, xs[wv], pz[wv], pmw[wv], lane, s, N, xp);
+    if (!live) return;
+    pm &= (1u << N) - 1u;
+    float x[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) x[i] = (i < N && ((pm >> i) & 1u)) ? fmaxf(xp[i], 1e-6f) : 0.0f;
+    int flags = 0;
+    FnnlsStats st;
+    const DevType &ty = types[type_id];
+    if (all_finite<float, N>(s) && prm.max_minimize > 0)
+        fast_phase1<N, N>(ty, s, prm, x, pm, st, flags);
+#pragma unroll
+    for (int p = 0; p < 10; ++p) x[p] = (p < N && ((pm >> p) & 1u)) ? x[p] : 0.0f;
+    NoPark park;
+    FitOut<float> out;
+    fast_phase2<N, N, false>(ty, s, prm, x, true, st, flags & F_FP64_PIVOT, park, out);
 #pragma unroll
     for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = out.x[p];
     chi2o[c] = out.chi2;
@@ -1042,14 +1093,26 @@ static mf_status launch_fit(mf_context *ctx, const SegArg &seg, const void *samp
     return MF_OK;
 }
 
+// mlp_here: a warm start whose prediction the fit kernel computes itself
+// (k_fast_warm); warm then holds no prediction.
 template <typename TS, int N, bool LIVE>
 static mf_status launch_fast(mf_context *ctx, const SegArg &seg, const void *samples,
                              int64_t nch, const mf_result *out, const FitParams &prm, int block,
-                             WarmIn warm, bool persistent, hipStream_t st)
+                             WarmIn warm, bool persistent, hipStream_t st, bool mlp_here = false)
 {
     const int64_t nt = seg_threads(seg);
     if (nt <= 0) return MF_OK;
     if (!out->iters) return fail(MF_ERR_INVALID, "the FP32 path needs mf_result.iters");
+    if (mlp_here) {
+        if (LIVE || persistent) return fail(MF_ERR_INVALID, "in-kernel warm start: not for this launch");
+        const int64_t g = (nt + FAST_WARM_BLOCK - 1) / FAST_WARM_BLOCK;
+        hipLaunchKernelGGL((k_fast_warm<TS, N>), dim3((unsigned)g), dim3(FAST_WARM_BLOCK), 0, st,
+                           ctx->d_types, seg, static_cast<const TS *>(samples), nch,
+                           static_cast<float *>(out->x), static_cast<float *>(out->chi2),
+                           out->flags, out->iters, ctx->mlp_frag, ctx->mlp_pq, prm);
+        HIP_TRY(hipGetLastError());
+        return MF_OK;
+    }
     if (persistent && !LIVE && seg.nrange == 0) {
         const FastTune &tu = ctx->tune;
         const int wpb = tu.block / 64;
@@ -1104,7 +1167,7 @@ static bool type_live(const mf_chtype &t, const FitParams &prm)
 template <typename TS>
 static mf_status dispatch_fast(mf_context *ctx, const SegArg &seg, const mf_batch *in,
                                const mf_result *out, const FitParams &prm, int block,
-                               WarmIn warm, bool persistent, hipStream_t st)
+                               WarmIn warm, bool persistent, hipStream_t st, bool mlp_here)
 {
     const mf_chtype &t = ctx->types[(size_t)seg_type0(seg)];
     const bool live = type_live(t, prm);
@@ -1112,11 +1175,11 @@ static mf_status dispatch_fast(mf_context *ctx, const SegArg &seg, const mf_batc
         return live ? launch_fast<TS, 10, true>(ctx, seg, in->samples, in->nch, out, prm, block,
                                                 warm, persistent, st)
                     : launch_fast<TS, 10, false>(ctx, seg, in->samples, in->nch, out, prm, block,
-                                                 warm, persistent, st);
+                                                 warm, persistent, st, mlp_here);
     return live ? launch_fast<TS, 8, true>(ctx, seg, in->samples, in->nch, out, prm, block, warm,
                                            persistent, st)
                 : launch_fast<TS, 8, false>(ctx, seg, in->samples, in->nch, out, prm, block, warm,
-                                            persistent, st);
+                                            persistent, st, mlp_here);
 }
 
 template <typename T>
@@ -1203,7 +1266,14 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
         return fail(MF_ERR_INVALID, "compute_errors needs mf_result.err");
     const int ntypes = (int)ctx->types.size();
     WarmIn warm{nullptr, nullptr};
-    if (opts->warm_start) {
+    const bool pers = opts->persistent == 1 ||
+                      (opts->persistent == MF_PERSISTENT_AUTO && ctx->tune.use_persistent);
+    // A warm start without a live pulse covariance in any type runs the MLP
+    // inside the fit kernel; otherwise the MLP kernel writes the prediction
+    // for all channels first.
+    bool mlp_here = opts->warm_start && !pers;
+    for (const mf_chtype &t : ctx->types) mlp_here = mlp_here && !type_live(t, prm);
+    if (opts->warm_start && !mlp_here) {
         if (ctx->pred_cap < in->nch) {
             if (ctx->graphs_pred)
                 return fail(MF_ERR_INVALID, "the scratch would grow under a captured graph; "
@@ -1234,11 +1304,9 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
         if (opts->precision == MF_PREC_FP64)
             return dispatch_fit<double>(ctx, seg, in, out, prm, block, opts->compute_errors != 0,
                                         st);
-        const bool pers = opts->persistent == 1 ||
-                          (opts->persistent == MF_PERSISTENT_AUTO && ctx->tune.use_persistent);
         return in->sample_dtype == MF_DTYPE_F64
-                   ? dispatch_fast<double>(ctx, seg, in, out, prm, block, warm, pers, st)
-                   : dispatch_fast<float>(ctx, seg, in, out, prm, block, warm, pers, st);
+                   ? dispatch_fast<double>(ctx, seg, in, out, prm, block, warm, pers, st, mlp_here)
+                   : dispatch_fast<float>(ctx, seg, in, out, prm, block, warm, pers, st, mlp_here);
     };
     ctx->n_fit_ev = 0;
     auto go = [&](const SegArg &seg) {
