@@ -1313,16 +1313,23 @@ MF_HD int fixed_cov_nmin(double chi2, const FitParams &prm, int &flags)
 }
 
 // Phase 2 and the rest of the fit (see fit_fast).  x: phase 1's amplitudes
-// (passive set = entries > 0).  Fills out.
-template <int NS, int NP, bool LIVE, bool TAB = true, typename Park>
-MF_HD void fast_phase2(const DevType &ty, const float (&s32)[10], const FitParams &prm,
+// (passive set = entries > 0).  Fills out.  reload(s32) is called before
+// each use of the samples (the start and chi2): a kernel can read them again
+// from memory instead of holding them in registers through the solves.
+struct KeepSamples {
+    MF_HD void operator()(float (&)[10]) const {}
+};
+template <int NS, int NP, bool LIVE, bool TAB = true, typename Park,
+          typename Reload = KeepSamples>
+MF_HD void fast_phase2(const DevType &ty, float (&s32)[10], const FitParams &prm,
                        const float (&x)[10], bool warm, FnnlsStats &st, int flags, Park &park,
-                       FitOut<float> &out)
+                       FitOut<float> &out, Reload &&reload = KeepSamples{})
 {
     double Atb64[10];
     ActiveState<double> S;
     int nmin = 0;
     double chi2 = 0.0;
+    reload(s32);
     if (fast_p2_begin<NS, NP>(ty, s32, prm, x, Atb64, S, flags)) {
         const int outer0 = st.outer;
         const MaskTab *tab = *opaque(&ty.tab);   // wave-uniform: one scalar load
@@ -1331,6 +1338,7 @@ MF_HD void fast_phase2(const DevType &ty, const float (&s32)[10], const FitParam
         }
         flags |= F_REFINED;
         if (warm && st.outer != outer0) flags |= F_WARM_REJECT;
+        reload(s32);
         chi2 = fast_p2_chi2<NS, NP>(ty, s32, S.x);
         // With a live pulse covariance the later iterations run the
         // reference's FP64 iteration body (per-channel covariance,
@@ -1366,11 +1374,14 @@ MF_HD void fast_phase2(const DevType &ty, const float (&s32)[10], const FitParam
 
 // The fast fit of one channel, both phases in one call.
 template <int NS, int NP, bool LIVE, bool TAB = true, typename Park>
-MF_HD void fit_fast(const DevType &ty, const float (&s32)[10], const FitParams &prm,
+MF_HD void fit_fast(const DevType &ty, const float (&s32_in)[10], const FitParams &prm,
                     float (&x)[10], uint32_t pm, bool warm, Park &park, FitOut<float> &out)
 {
     int flags = 0;
     FnnlsStats st;
+    float s32[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) s32[i] = s32_in[i];
     if (all_finite<float, NS>(s32) && prm.max_minimize > 0)
         fast_phase1<NS, NP>(ty, s32, prm, x, pm, st, flags);
     fast_phase2<NS, NP, LIVE, TAB>(ty, s32, prm, x, warm, st, flags, park, out);

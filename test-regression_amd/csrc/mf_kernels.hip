@@ -218,6 +218,20 @@ struct NoPark {
     __device__ void store(int, double) {}
     __device__ double load(int) const { return 0.0; }
 };
+// Reads a channel's samples again where phase 2 uses them, so that they need
+// not stay in registers (or spill) through the solves.  The pointer is
+// laundered so the compiler does not reuse the values it loaded before.
+template <typename TS, int N> struct SampleReload {
+    const TS *s;
+    int64_t nch, c;
+    __device__ void operator()(float (&v)[10]) const
+    {
+        const TS *q = s;
+        asm volatile("" : "+s"(q));
+#pragma unroll
+        for (int i = 0; i < 10; ++i) v[i] = i < N ? (float)q[(int64_t)i * nch + c] : 0.0f;
+    }
+};
 // Four waves per SIMD: at 128 VGPRs a few values spill (about 56 bytes per
 // lane, outside the solve loops); measured faster than 137 VGPRs at three.
 template <typename TS, int N>
@@ -244,7 +258,7 @@ void k_fast_fused(const DevType *__restrict__ types, SegArg seg,
     NoPark park;
     FitOut<float> out;
     fast_phase2<N, N, false>(ty, s, prm, x, warm.pm != nullptr, st, flags & F_FP64_PIVOT, park,
-                             out);
+                             out, SampleReload<TS, N>{samples, nch, c});
 #pragma unroll
     for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = out.x[p];
     chi2o[c] = out.chi2;
@@ -294,7 +308,8 @@ void k_fast_warm(const DevType *__restrict__ types, SegArg seg,
     for (int p = 0; p < 10; ++p) x[p] = (p < N && ((pm >> p) & 1u)) ? x[p] : 0.0f;
     NoPark park;
     FitOut<float> out;
-    fast_phase2<N, N, false>(ty, s, prm, x, true, st, flags & F_FP64_PIVOT, park, out);
+    fast_phase2<N, N, false>(ty, s, prm, x, true, st, flags & F_FP64_PIVOT, park, out,
+                             SampleReload<TS, N>{samples, nch, c});
 #pragma unroll
     for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = out.x[p];
     chi2o[c] = out.chi2;
