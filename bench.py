@@ -39,20 +39,28 @@ sys.path.insert(0, str(ROOT / "tests"))
 
 PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
 
-# HBM bytes per launch of a warm step's fit kernel (k_fast_fused)
+# HBM bytes per launch of a warm step's kernel (k_fast_warm)
 # on the default workload, from two rocprofv3 --pmc runs of
 # `bench.py --traffic-probe`: FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, the
 # factors that make the counters read a 256 MiB device copy as 256 MiB each
 # way.  Other workloads report null.
 MEASURED_TRAFFIC = {("c2", "zero", 1_000_000): {
-    "bytes": 273.3e6, "how": "rocprofv3 --pmc FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00), "
+    "bytes": 121.2e6, "how": "rocprofv3 --pmc FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00), "
                              "bench.py --traffic-probe, copy-calibrated"}}
 
 
-def fit_kernel_name(pulse_cov: str) -> str:
+# Forward pass of the warm-start MLP per channel (10 -> 64 -> 64 -> 20): MFMA
+# (bfloat16) work inside the warm fit kernel, not counted in the VALU
+# roofline's achieved figure.
+MLP_FLOPS_PER_CHANNEL = 2 * (10 * 64 + 64 * 64 + 64 * 20)
+
+
+def fit_kernel_name(pulse_cov: str, warm: bool = True) -> str:
     """The fast path's fit kernels for a pulse-covariance mode."""
     if pulse_cov == "live":
-        return "k_fast_p1 + k_fast_p2 (the step's fit kernels)"
+        return ("k_mlp_warm + " if warm else "") + "k_fast_p1 + k_fast_p2 (the step's kernels)"
+    if warm:
+        return "k_fast_warm (MLP prediction, FP32 phase and FP64 finish in one launch)"
     return "k_fast_fused (both phases in one launch)"
 
 
@@ -202,8 +210,9 @@ def event_stream(args, eng, w, log):
                   "latency_ms_p99": float(np.quantile(lat_e, 0.99))},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["f32"],
                      "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f32"], "traffic": None,
-                     "kernel": "k_fast_fused, both types", "kernel_ms": fit_ms,
-                     "aflops_per_channel": af},
+                     "kernel": fit_kernel_name("zero") + ", both types", "kernel_ms": fit_ms,
+                     "aflops_per_channel": af,
+                     "mlp_mfma_flops_per_channel_not_counted": MLP_FLOPS_PER_CHANNEL},
         "accuracy_vs_plain_fp64": acc,
     }
     if not args.no_cpu:
@@ -363,7 +372,8 @@ def main():
                                      {}).get("how"),
                      "kernel": fit_kernel_name(args.pulse_cov),
                      "kernel_ms": kern_ms,
-                     "aflops_per_channel": af},
+                     "aflops_per_channel": af,
+                     "mlp_mfma_flops_per_channel_not_counted": MLP_FLOPS_PER_CHANNEL},
         "plain_fp64": {"channels_per_s": nch / (res["plain"]["ms"] / 1e3),
                        "ms_per_step": res["plain"]["ms"]},
         "fast_cold_fp32": {"channels_per_s": nch / (res["fast"]["ms"] / 1e3),
