@@ -1217,20 +1217,19 @@ MF_HD double chisq64(const TypeMats<double> &td, const TL (&Li)[10][10], const d
 // First Minimize iteration of the fast fit.  The covariance is C0 for every
 // channel of the type at x = 0, so the normal equations are the type's AtA0
 // and Atb = G0 s (no per-channel covariance, Cholesky or whitening).
-//   phase 1 (FP32, fast_phase1): active-set iterations from (x, pm) -- x = 0,
-//     pm = 0 for a cold start, a predicted passive set for a warm start --
+//   phase 1 (FP32, fast_phase1, cold starts): active-set iterations from x = 0
 //     until the FP32 KKT test passes or a pivot of the passive block loses
 //     more than a fraction tau32 of its diagonal (F_FP64_PIVOT);
 //   phase 2 (FP64, fast_phase2): the same iterations continue from that state
-//     on the FP64 normal equations with the reference's threshold eps: the
-//     first solve refines the amplitudes to FP64, the KKT test verifies the
-//     active set (a warm start that needed more steps is flagged
-//     F_WARM_REJECT).
+//     (a warm start: from the predicted passive set and amplitudes) on the
+//     FP64 normal equations with the reference's threshold eps: the first
+//     solve refines the amplitudes to FP64, the KKT test verifies the active
+//     set (a warm start that needed more steps is flagged F_WARM_REJECT).
 // The result is the NNLS optimum of the x = 0 problem, which is what the
 // reference's first (cold) FNNLS computes.  The two phases are separate
-// functions so that a kernel can run them as separate launches: phase 1
-// needs about half the registers of phase 2.  Phase 2 takes the passive set
-// as the entries with x > 0.
+// functions so that a kernel can run them as separate launches (with a live
+// pulse covariance phase 2 needs about twice the registers of phase 1).
+// Phase 2 takes the passive set as the entries with x > 0.
 template <int NS, int NP>
 MF_HD void fast_phase1(const DevType &ty, const float (&s32)[10], const FitParams &prm,
                        float (&x)[10], uint32_t &pm, FnnlsStats &st, int &flags)
@@ -1382,7 +1381,7 @@ MF_HD void fit_fast(const DevType &ty, const float (&s32_in)[10], const FitParam
     float s32[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) s32[i] = s32_in[i];
-    if (all_finite<float, NS>(s32) && prm.max_minimize > 0)
+    if (!warm && all_finite<float, NS>(s32) && prm.max_minimize > 0)
         fast_phase1<NS, NP>(ty, s32, prm, x, pm, st, flags);
     fast_phase2<NS, NP, LIVE, TAB>(ty, s32, prm, x, warm, st, flags, park, out);
 }

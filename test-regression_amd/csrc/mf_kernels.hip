@@ -154,8 +154,13 @@ void k_fit_plain(const DevType *__restrict__ types, SegArg seg,
 }
 
 // Fast path (MF_PREC_FP32), phases fast_phase1 (FP32 active-set
-// exploration) and fast_phase2 (FP64 finish).  Types without a live pulse
-// covariance run both in one launch (k_fast_fused).  With a live covariance
+// exploration) and fast_phase2 (FP64 finish).  A cold start runs both; a warm
+// start goes straight to the FP64 phase from the prediction: the prediction
+// is exact for most channels, and an FP64 step with the passive set's
+// inverse table costs fewer instructions than an FP32 factorization (the
+// FP64 vector rate equals the unpacked FP32 rate), so the FP32 phase only
+// repeated work.  Types without a live pulse covariance run in one launch
+// (k_fast_fused, or k_fast_warm with the MLP inside).  With a live covariance
 // phase 2 runs the reference's later Minimize iterations in FP64 and needs
 // about twice the registers of phase 1, so it is a launch of its own:
 // k_fast_p1 leaves its amplitudes in the output x (FP32), its flags and
@@ -202,7 +207,7 @@ void k_fast_p1(const DevType *__restrict__ types, SegArg seg,
     uint32_t pm = warm_load<N>(warm, c, nch, x);
     int flags = 0;
     FnnlsStats st;
-    if (all_finite<float, N>(s) && prm.max_minimize > 0)
+    if (!warm.pm && all_finite<float, N>(s) && prm.max_minimize > 0)
         fast_phase1<N, N>(types[type_id], s, prm, x, pm, st, flags);
 #pragma unroll
     for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = ((pm >> p) & 1u) ? x[p] : 0.0f;
@@ -251,7 +256,7 @@ void k_fast_fused(const DevType *__restrict__ types, SegArg seg,
     int flags = 0;
     FnnlsStats st;
     const DevType &ty = types[type_id];
-    if (all_finite<float, N>(s) && prm.max_minimize > 0)
+    if (!warm.pm && all_finite<float, N>(s) && prm.max_minimize > 0)
         fast_phase1<N, N>(ty, s, prm, x, pm, st, flags);
 #pragma unroll
     for (int p = 0; p < 10; ++p) x[p] = (p < N && ((pm >> p) & 1u)) ? x[p] : 0.0f;
@@ -299,16 +304,12 @@ void k_fast_warm(const DevType *__restrict__ types, SegArg seg,
     float x[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) x[i] = (i < N && ((pm >> i) & 1u)) ? fmaxf(xp[i], 1e-6f) : 0.0f;
-    int flags = 0;
+    // a warm start goes straight to the FP64 phase (see k_fast_p1)
     FnnlsStats st;
     const DevType &ty = types[type_id];
-    if (all_finite<float, N>(s) && prm.max_minimize > 0)
-        fast_phase1<N, N>(ty, s, prm, x, pm, st, flags);
-#pragma unroll
-    for (int p = 0; p < 10; ++p) x[p] = (p < N && ((pm >> p) & 1u)) ? x[p] : 0.0f;
     NoPark park;
     FitOut<float> out;
-    fast_phase2<N, N, false>(ty, s, prm, x, true, st, flags & F_FP64_PIVOT, park, out,
+    fast_phase2<N, N, false>(ty, s, prm, x, true, st, 0, park, out,
                              SampleReload<TS, N>{samples, nch, c});
 #pragma unroll
     for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = out.x[p];
@@ -473,7 +474,15 @@ void k_fast_p1q(const DevType *__restrict__ types, int type_id, Seg seg,
             S.adds = 0;
             st = FnnlsStats();
             ill = false;
-            if (all_finite<float, N>(svn) && prm.max_minimize > 0) {
+            if (warm.pm) {
+                // a warm start goes straight to the FP64 phase (k_fast_p2q)
+#pragma unroll
+                for (int p = 0; p < N; ++p)
+                    xo[(int64_t)p * nch + c] = ((S.pm >> p) & 1u) ? S.x[p] : 0.0f;
+                flo[c] = 0;
+                ito[c] = 0;
+                c = -1;
+            } else if (all_finite<float, N>(svn) && prm.max_minimize > 0) {
                 const TypeMats<float> &tf = *opaque(&ty.f);
 #pragma unroll
                 for (int p = 0; p < 10; ++p) {
