@@ -45,7 +45,7 @@ PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
 # factors that make the counters read a 256 MiB device copy as 256 MiB each
 # way.  Other workloads report null.
 MEASURED_TRAFFIC = {("c2", "zero", 1_000_000): {
-    "bytes": 121.2e6, "how": "rocprofv3 --pmc FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00), "
+    "bytes": 104.7e6, "how": "rocprofv3 --pmc FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00), "
                              "bench.py --traffic-probe, copy-calibrated"}}
 
 
@@ -60,7 +60,7 @@ def fit_kernel_name(pulse_cov: str, warm: bool = True) -> str:
     if pulse_cov == "live":
         return ("k_mlp_warm + " if warm else "") + "k_fast_p1 + k_fast_p2 (the step's kernels)"
     if warm:
-        return "k_fast_warm (MLP prediction, FP32 phase and FP64 finish in one launch)"
+        return "k_fast_warm (MLP prediction and FP64 active-set phase in one launch)"
     return "k_fast_fused (both phases in one launch)"
 
 
@@ -363,7 +363,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.config}: {w.description}", "channels": nch,
                    "samples": 10, "pulses": 10, "pulse_cov": args.pulse_cov,
-                   "path": "warm-start fast path (MLP + FP32 active set + FP64 polish)"},
+                   "path": "warm-start fast path (MLP prediction on MFMA, FP64 active-set finish)"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak,
                      "traffic": (MEASURED_TRAFFIC.get((args.config, args.pulse_cov, nch)) or

@@ -114,7 +114,8 @@ enum {
     MF_F_FNNLS_CAP    = 4,   /* an FNNLS call hit max_iter                    */
     MF_F_NONFINITE    = 8,   /* non-finite input sample: not fitted, x = 0    */
     MF_F_FP64_PIVOT   = 16,  /* FP32 path: a pivot was recomputed in FP64     */
-    MF_F_WARM_REJECT  = 32,  /* warm start failed its KKT check: cold refit   */
+    MF_F_WARM_REJECT  = 32,  /* warm start failed its KKT check; the active-set
+                                iterations went on from it                    */
     MF_F_REFINED      = 64,  /* FP32 path: FP64 refinement step applied       */
     MF_F_BAD_TYPE     = 128  /* type_of[c] >= ntypes: not fitted, x = 0       */
 };
