@@ -78,9 +78,12 @@ __device__ __forceinline__ float mlp_features(const float (&v)[10], int ns, __bf
     for (int i = 0; i < 10; ++i)
         if (i < ns) mx = fmaxf(mx, v[i]);
     const float sc = fmaxf(mx, 1.0f);
+    // the hardware reciprocal (1 ulp): the features are rounded to bfloat16
+    // (2^-9 relative) next
+    const float rs = __builtin_amdgcn_rcpf(sc);
     f32x4 q[4];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) q[i / 4][i % 4] = (i < 10 && i < ns) ? v[i] / sc : 0.0f;
+    for (int i = 0; i < 16; ++i) q[i / 4][i % 4] = (i < 10 && i < ns) ? v[i] * rs : 0.0f;
 #pragma unroll
     for (int k = 0; k < 4; ++k) *reinterpret_cast<bf16x4 *>(&xs[lane][4 * k]) = cvt4(q[k]);
     return sc;
@@ -92,13 +95,12 @@ struct NoTileEnd {
 
 // The network on the 64 channels of a wavefront, as 4 tiles of 16 channels.
 // xs: the wavefront's features (mlp_features), L: the staged network.
-// emit(t, o, z) is called for the lane's channel t (0..63) and outputs
-// o = 16 mt + 4 g + j < 20 with z = the output pre-activation (logits for
-// o < 10; the amplitude head's softplus is left to the caller); tile_end(t)
-// after the last output of the lane's channel t of a tile.
-template <typename Emit, typename TileEnd = NoTileEnd>
-__device__ __forceinline__ void mlp_tiles(const MlpLds &L, const __bf16 (*xs)[16], int lane,
-                                          Emit &&emit, TileEnd &&tile_end = NoTileEnd{})
+// tile(t, g, d0, d1) is called once per tile with the output pre-activations
+// the lane holds for channel t (0..63), g = lane / 16: d0[j] is output
+// 4 g + j, d1[j] is output 16 + 4 g + j (outputs past 19 are padding).
+template <typename TileFn>
+__device__ __forceinline__ void mlp_tiles_d(const MlpLds &L, const __bf16 (*xs)[16], int lane,
+                                            TileFn &&tile)
 {
     const int g = lane >> 4, n = lane & 15;
 #pragma unroll 1
@@ -151,16 +153,29 @@ __device__ __forceinline__ void mlp_tiles(const MlpLds &L, const __bf16 (*xs)[16
                 d[mt] = acc;
             }
         }
+        tile(nt * 16 + n, g, d[0], d[1]);
+    }
+}
+
+// The same, one output at a time: emit(t, o, z) for the lane's channel t and
+// outputs o = 16 mt + 4 g + j < 20 with z = the output pre-activation (logits
+// for o < 10; the amplitude head's softplus is left to the caller);
+// tile_end(t) after the last output of the lane's channel t of a tile.
+template <typename Emit, typename TileEnd = NoTileEnd>
+__device__ __forceinline__ void mlp_tiles(const MlpLds &L, const __bf16 (*xs)[16], int lane,
+                                          Emit &&emit, TileEnd &&tile_end = NoTileEnd{})
+{
+    mlp_tiles_d(L, xs, lane, [&](int t, int g, const f32x4 &d0, const f32x4 &d1) {
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int o = mt * 16 + 4 * g + j;
-                if (o < 20) emit(nt * 16 + n, o, d[mt][j]);
+                if (o < 20) emit(t, o, mt ? d1[j] : d0[j]);
             }
         }
-        tile_end(nt * 16 + n);
-    }
+        tile_end(t);
+    });
 }
 
 // softplus(z) = max(z, 0) + ln 2 log2(1 + 2^(-|z| log2 e)) with the hardware
@@ -186,18 +201,25 @@ __device__ __forceinline__ uint32_t mlp_warm_wave(const MlpLds &L, __bf16 (*xs)[
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    uint32_t bits = 0;
-    mlp_tiles(
-        L, xs, lane,
-        [&](int t, int o, float z) {
-            bits |= (o < 10 && z > 0.0f) ? 1u << o : 0u;
-            const float sp = softplus_hw(z);
-            if (o >= 10) pz[o - 10][t] = sp;
-        },
-        [&](int t) {
-            if (bits) atomicOr(&pmw[t], bits & valid);
-            bits = 0;
-        });
+    mlp_tiles_d(L, xs, lane, [&](int t, int g, const f32x4 &d0, const f32x4 &d1) {
+        // logits, o = 4 g + j < 10: d0 of lanes g = 0, 1, 2
+        uint32_t bits = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int o = 4 * g + j;
+            bits |= (o < 10 && d0[j] > 0.0f) ? 1u << o : 0u;
+        }
+        if (bits) atomicOr(&pmw[t], bits & valid);
+        // amplitude head, o = 10..19: d0 of lanes g = 2 (o = 8 + j, j >= 2) and
+        // g = 3 (o = 12 + j), d1 of lanes g = 0 (o = 16 + j); one softplus per j
+        // on the value this lane holds
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float sp = softplus_hw(g == 0 ? d1[j] : d0[j]);
+            const int r = g == 0 ? 6 + j : g == 3 ? 2 + j : j - 2;   // o - 10
+            if (g == 0 || g == 3 || (g == 2 && j >= 2)) pz[r][t] = sp;
+        }
+    });
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

@@ -138,6 +138,41 @@ def test_persistent_equals_per_channel_launch(pkg, gpu, warm):
     assert torch.equal(a.flags, b.flags) and torch.equal(a.iters, b.iters)
 
 
+@pytest.mark.parametrize("cfg,n,layout", [("c2", 100_003, "type_of"), ("c3", 50_000, "type_of"),
+                                          ("c5", 75_848, "ranges")])
+def test_warm_single_launch_equals_persistent(pkg, gpu, cfg, n, layout):
+    """The warm single launch (the MLP's prediction and the FP64 phase in one
+    kernel; blocks of c5's type ranges hold channels of both types) against
+    the persistent kernels (the prediction through memory, lanes refilled
+    from a queue): each channel runs the same steps, so the outputs are
+    identical, bit for bit, including a last block that is only partly
+    filled."""
+    from test_regression_amd import engine, workloads, _abi, warmstart
+    w = workloads.config(cfg)
+    eng = engine.Engine(list(w.chtypes()))
+    warmstart.ensure_trained(eng)
+    kw = {}
+    if layout == "ranges":
+        s, t = w.generate(events=1)[:2]
+        kw["type_ranges"] = [0, w.event[0], sum(w.event)]
+        t = None
+    else:
+        s, t, _ = w.generate(n)
+        t = torch.from_numpy(t).cuda() if t is not None else None
+    x = torch.from_numpy(s.astype(np.float32)).cuda()
+    print(f"[gpu-fast] warm single launch vs persistent cfg={cfg} n={x.shape[1]} {layout}",
+          flush=True)
+    o = dict(precision=_abi.MF_PREC_FP32, warm_start=1)
+    a = eng.fit(x, t, _abi.default_opts(**o), **kw)
+    b = eng.fit(x, t, _abi.default_opts(persistent=1, **o), **kw)
+    torch.cuda.synchronize()
+    assert torch.equal(a.x, b.x) and torch.equal(a.chi2, b.chi2)
+    assert torch.equal(a.flags, b.flags) and torch.equal(a.iters, b.iters)
+    # the warm starts were used: most channels took one solve, some more
+    inner = a.iters.cpu().numpy() >> 16
+    assert (inner == 1).mean() > 0.2 and (inner > 1).mean() > 0.05
+
+
 def test_persistent_edge_cases(pkg, gpu):
     """Non-finite and empty channels, and batches smaller than one wavefront
     or not a multiple of it, through the queue."""
