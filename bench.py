@@ -45,7 +45,7 @@ PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
 # factors that make the counters read a 256 MiB device copy as 256 MiB each
 # way.  Other workloads report null.
 MEASURED_TRAFFIC = {("c2", "zero", 1_000_000): {
-    "bytes": 104.7e6, "how": "rocprofv3 --pmc FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00), "
+    "bytes": 105.3e6, "how": "rocprofv3 --pmc FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00), "
                              "bench.py --traffic-probe, copy-calibrated"}}
 
 
@@ -68,22 +68,36 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+# Cycles of the spin kernel (torch.cuda._sleep) that holds the launch stream
+# before a timed region: 0.8 ms or more at any shader clock, longer than the
+# host takes to queue the first step.
+GATE_CYCLES = 2_000_000
+
+
 def time_steps(eng, fn, steps, warmup, stream, kernel_steps=5):
     """K steps back to back between two events on the launch stream, with a
-    synchronize on both sides (ms per step); then, outside the timed region,
-    a few steps with the library's kernel events (fit and MLP kernel time)."""
+    synchronize on both sides (ms per step).  A spin kernel holds the stream
+    while the host queues the first step, so the events time the device's K
+    steps rather than the host's latency for its first call (about 0.1 ms).
+    Also: the host clock over K more steps, from the first call to the end of
+    the last (ms per step), and, outside both, a few steps with the library's
+    kernel events (fit and MLP kernel time)."""
     import torch
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda._sleep(GATE_CYCLES)
     a.record(stream)
     for _ in range(steps):
         fn()
     b.record(stream)
     torch.cuda.synchronize()
-    wall = (time.perf_counter() - t0) * 1e3 / steps
     eng.set_timing(True)
     kern = []
     for _ in range(kernel_steps):
@@ -356,6 +370,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms,
+        "ms_per_step_host_clock": res["warm"]["wall_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
