@@ -288,6 +288,9 @@ void k_fast_warm(const DevType *__restrict__ types, SegArg seg,
     __shared__ __bf16 xs[FAST_WARM_BLOCK / 64][64][16];
     __shared__ float pz[FAST_WARM_BLOCK / 64][10][64];
     __shared__ uint32_t pmw[FAST_WARM_BLOCK / 64][64];
+    // a block past the end of its type's list (the grid of a type's launch is
+    // sized for the whole batch) stages nothing
+    if (seg.nrange == 0 && (int64_t)blockIdx.x * FAST_WARM_BLOCK >= seg_count(seg.list)) return;
     mlp::mlp_stage(L, frag, pq, threadIdx.x, FAST_WARM_BLOCK);
     __syncthreads();
     int type_id = 0;
@@ -602,27 +605,88 @@ __global__ void k_zero_u32(unsigned *__restrict__ p, int n)
 }
 
 // Partition of a mixed batch by channel type, on the device only (no host
-// round trip, so a fit call can be captured in a graph): counts per type,
-// their exclusive scan, and the index list of each type.  part layout:
-// counts[256], cursors[256], offsets[256] (64-bit).
+// round trip, so a fit call can be captured in a graph), without atomics on
+// shared addresses and stable: each type's list holds its channels in
+// ascending order.  The batch is cut into at most PART_BLOCKS contiguous
+// chunks, one per block:
+//   k_part_count    the chunk's count of each type (wavefront ballots);
+//   k_part_scan     per type, the exclusive scan of the chunk counts (the
+//                   chunk's first slot within the type) and the type's total;
+//   k_type_scan     the exclusive scan of the totals (each type's first slot);
+//   k_part_scatter  each chunk again, 256 channels at a time: a channel's slot
+//                   is its type's next slot in the chunk plus its rank among
+//                   the earlier channels of its type in those 256.
+// part layout: counts[256], (unused)[256], offsets[256] (int64), then
+// bcount[256][PART_BLOCKS] and boff[256][PART_BLOCKS] (int32).
+constexpr int PART_BLOCKS = 512;
+constexpr size_t PART_BYTES = sizeof(int64_t) * 768 + 2 * sizeof(int32_t) * 256 * PART_BLOCKS;
+
 __global__ void k_zero_u64(unsigned long long *__restrict__ p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
 }
 
-__global__ void k_type_count(const uint8_t *__restrict__ type_of, int64_t nch,
-                             unsigned long long *__restrict__ counts)
+// The type ids of a wavefront's lanes (t < 0: no channel), one type at a
+// time: fn(type, lanes of that type, this lane's rank among them) on every
+// lane, with this lane's own rank only where the type is its own.
+template <typename Fn>
+__device__ __forceinline__ void wave_types(int t, Fn &&fn)
 {
-    __shared__ unsigned int loc[256];
-    for (int t = threadIdx.x; t < 256; t += blockDim.x) loc[t] = 0;
+    const int lane = __lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    bool pending = t >= 0;
+    for (uint64_t pm = __ballot(pending); pm; pm = __ballot(pending)) {
+        const int lead = __ffsll((unsigned long long)pm) - 1;
+        const int tt = __shfl(t, lead);
+        const bool me = pending && t == tt;
+        const uint64_t mine = __ballot(me);
+        fn(tt, lane == lead, mine, __popcll(mine & below));
+        if (me) pending = false;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_part_count(const uint8_t *__restrict__ type_of,
+                                                    int64_t nch, int ntypes, int64_t chunk,
+                                                    int32_t *__restrict__ bcount)
+{
+    __shared__ int cnt[256];
+    cnt[threadIdx.x] = 0;
     __syncthreads();
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nch;
-         c += (int64_t)gridDim.x * blockDim.x)
-        atomicAdd(&loc[type_of[c]], 1u);
+    const int64_t c0 = (int64_t)blockIdx.x * chunk, c1 = c0 + chunk < nch ? c0 + chunk : nch;
+    for (int64_t base = c0; base < c1; base += 256) {
+        const int64_t c = base + threadIdx.x;
+        int t = c < c1 ? (int)type_of[c] : -1;
+        if (t >= ntypes) t = -1;
+        wave_types(t, [&](int tt, bool lead, uint64_t mine, int) {
+            if (lead) atomicAdd(&cnt[tt], __popcll(mine));
+        });
+    }
     __syncthreads();
-    for (int t = threadIdx.x; t < 256; t += blockDim.x)
-        if (loc[t]) atomicAdd(&counts[t], (unsigned long long)loc[t]);
+    if ((int)threadIdx.x < ntypes) bcount[threadIdx.x * PART_BLOCKS + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// one block per type, nblk <= 2 * 256 chunks
+__global__ __launch_bounds__(256) void k_part_scan(int nblk, const int32_t *__restrict__ bcount,
+                                                   int32_t *__restrict__ boff,
+                                                   long long *__restrict__ counts)
+{
+    __shared__ int v[256];
+    const int t = blockIdx.x, i = threadIdx.x;
+    const int32_t *bc = bcount + (size_t)t * PART_BLOCKS;
+    const int a0 = 2 * i < nblk ? bc[2 * i] : 0, a1 = 2 * i + 1 < nblk ? bc[2 * i + 1] : 0;
+    v[i] = a0 + a1;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        const int x = i >= d ? v[i - d] : 0;
+        __syncthreads();
+        v[i] += x;
+        __syncthreads();
+    }
+    const int ex = v[i] - (a0 + a1);
+    if (2 * i < nblk) boff[(size_t)t * PART_BLOCKS + 2 * i] = ex;
+    if (2 * i + 1 < nblk) boff[(size_t)t * PART_BLOCKS + 2 * i + 1] = ex + a0;
+    if (i == 255) counts[t] = v[255];
 }
 
 // one block of 256 threads: offsets[t] = sum of counts[u], u < t
@@ -641,44 +705,52 @@ __global__ void k_type_scan(const long long *__restrict__ counts, long long *__r
     offsets[t] = v[t] - counts[t];
 }
 
-// Index lists of the types; a channel whose type id is not set (>= ntypes)
-// is not fitted: x = 0, chi2 = 0, flags = F_BAD_TYPE.  Slots are claimed per
-// wavefront: for each type present among its lanes, one atomic add on that
-// type's cursor, and the lanes of the type write at their rank (a cursor
-// shared by every channel would serialize the whole batch on one address).
+// The index lists of the types; a channel whose type id is not set
+// (>= ntypes) is not fitted: x = 0, chi2 = 0, flags = F_BAD_TYPE.
 template <typename T>
-__global__ void k_type_scatter(const uint8_t *__restrict__ type_of, int64_t nch, int ntypes,
-                               const long long *__restrict__ offsets,
-                               unsigned long long *__restrict__ cursors, int32_t *__restrict__ idx,
-                               T *__restrict__ xo, T *__restrict__ chi2o,
-                               int32_t *__restrict__ flo, int32_t *__restrict__ ito)
+__global__ __launch_bounds__(256) void k_part_scatter(
+    const uint8_t *__restrict__ type_of, int64_t nch, int ntypes, int64_t chunk,
+    const int32_t *__restrict__ boff, const long long *__restrict__ offsets,
+    int32_t *__restrict__ idx, T *__restrict__ xo, T *__restrict__ chi2o,
+    int32_t *__restrict__ flo, int32_t *__restrict__ ito)
 {
-    const int lane = __lane_id();
-    for (int64_t c0 = (int64_t)blockIdx.x * blockDim.x; c0 < nch;
-         c0 += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c = c0 + threadIdx.x;
-        int t = c < nch ? (int)type_of[c] : -1;
+    __shared__ int run[256];      // the type's next slot in this chunk
+    __shared__ int wc[4][256];    // the type's channels in each wavefront of the 256
+    const int tid = threadIdx.x, wv = tid >> 6;
+    if (tid < ntypes) run[tid] = (int)(offsets[tid] + boff[(size_t)tid * PART_BLOCKS + blockIdx.x]);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) wc[w][tid] = 0;
+    __syncthreads();
+    const int64_t c0 = (int64_t)blockIdx.x * chunk, c1 = c0 + chunk < nch ? c0 + chunk : nch;
+    for (int64_t base = c0; base < c1; base += 256) {
+        const int64_t c = base + tid;
+        int t = c < c1 ? (int)type_of[c] : -1;
         if (t >= ntypes) {
+#pragma unroll
             for (int i = 0; i < 10; ++i) xo[(int64_t)i * nch + c] = T(0);
             chi2o[c] = T(0);
             flo[c] = F_BAD_TYPE;
             if (ito) ito[c] = 0;
             t = -1;
         }
-        bool pending = t >= 0;
-        for (uint64_t pmask = __ballot(pending); pmask; pmask = __ballot(pending)) {
-            const int lead = __ffsll((unsigned long long)pmask) - 1;
-            const int tt = __shfl(t, lead);
-            const uint64_t mine = __ballot(pending && t == tt);
-            unsigned long long base = 0;
-            if (lane == lead) base = atomicAdd(&cursors[tt], (unsigned long long)__popcll(mine));
-            base = __shfl(base, lead);
-            if (pending && t == tt) {
-                const int r = __popcll(mine & ((1ull << lane) - 1ull));
-                idx[offsets[tt] + (long long)(base + (unsigned long long)r)] = (int32_t)c;
-                pending = false;
-            }
+        int rank = 0;
+        wave_types(t, [&](int tt, bool lead, uint64_t mine, int r) {
+            if (lead) wc[wv][tt] = __popcll(mine);
+            if (t == tt) rank = r;
+        });
+        __syncthreads();
+        if (t >= 0) {
+            int off = run[t] + rank;
+            for (int w = 0; w < wv; ++w) off += wc[w][t];
+            idx[off] = (int32_t)c;
         }
+        __syncthreads();
+        if (tid < ntypes) {
+            run[tid] += wc[0][tid] + wc[1][tid] + wc[2][tid] + wc[3][tid];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) wc[w][tid] = 0;
+        }
+        __syncthreads();
     }
 }
 
@@ -871,7 +943,7 @@ struct mf_context {
     DevType *d_types = nullptr;
     MaskTab *d_tabs = nullptr;     // one per type, pointed to by DevType::tab
     // partition scratch
-    void *d_part = nullptr;  // counts[256], cursors[256], offsets[256] (int64)
+    void *d_part = nullptr;  // PART_BYTES: type counts, offsets, per-chunk counts and slots
     int32_t *d_idx = nullptr;
     int64_t idx_cap = 0;
     // warm-start MLP: masters, bfloat16 copies (as floats and as bits),
@@ -1376,7 +1448,7 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
     if ((!ctx->d_part || ctx->idx_cap < in->nch) && ctx->graphs_part)
         return fail(MF_ERR_INVALID, "the scratch would grow under a captured graph; "
                                     "destroy the context's graphs first");
-    if (!ctx->d_part) HIP_TRY(hipMalloc(&ctx->d_part, sizeof(int64_t) * 768));
+    if (!ctx->d_part) HIP_TRY(hipMalloc(&ctx->d_part, PART_BYTES));
     if (ctx->idx_cap < in->nch) {
         if (ctx->d_idx) HIP_TRY(hipFree(ctx->d_idx));
         ctx->d_idx = nullptr;
@@ -1384,25 +1456,32 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
         ctx->idx_cap = in->nch;
     }
     int64_t *counts = static_cast<int64_t *>(ctx->d_part);
-    int64_t *cursors = counts + 256, *offsets = counts + 512;
-    hipLaunchKernelGGL(k_zero_u64, dim3(2), dim3(256), 0, st,
-                       reinterpret_cast<unsigned long long *>(counts), 512);
-    const int pgrid = (int)std::min<int64_t>(2048, (in->nch + 255) / 256);
-    hipLaunchKernelGGL(k_type_count, dim3(pgrid), dim3(256), 0, st, in->type_of, in->nch,
-                       reinterpret_cast<unsigned long long *>(counts));
+    int64_t *offsets = counts + 512;
+    int32_t *bcount = reinterpret_cast<int32_t *>(counts + 768);
+    int32_t *boff = bcount + 256 * PART_BLOCKS;
+    // chunks: multiples of 256 channels, at most PART_BLOCKS of them
+    const int64_t nblk0 = std::min<int64_t>(PART_BLOCKS, (in->nch + 255) / 256);
+    const int64_t chunk = ((in->nch + nblk0 - 1) / nblk0 + 255) / 256 * 256;
+    const int nblk = (int)((in->nch + chunk - 1) / chunk);
+    hipLaunchKernelGGL(k_zero_u64, dim3(3), dim3(256), 0, st,
+                       reinterpret_cast<unsigned long long *>(counts), 768);
+    hipLaunchKernelGGL(k_part_count, dim3(nblk), dim3(256), 0, st, in->type_of, in->nch, ntypes,
+                       chunk, bcount);
+    hipLaunchKernelGGL(k_part_scan, dim3(ntypes), dim3(256), 0, st, nblk, bcount, boff,
+                       reinterpret_cast<long long *>(counts));
     hipLaunchKernelGGL(k_type_scan, dim3(1), dim3(256), 0, st,
                        reinterpret_cast<const long long *>(counts),
                        reinterpret_cast<long long *>(offsets));
     if (opts->precision == MF_PREC_FP64)
-        hipLaunchKernelGGL(k_type_scatter<double>, dim3(pgrid), dim3(256), 0, st, in->type_of,
-                           in->nch, ntypes, reinterpret_cast<const long long *>(offsets),
-                           reinterpret_cast<unsigned long long *>(cursors), ctx->d_idx,
+        hipLaunchKernelGGL(k_part_scatter<double>, dim3(nblk), dim3(256), 0, st, in->type_of,
+                           in->nch, ntypes, chunk, boff,
+                           reinterpret_cast<const long long *>(offsets), ctx->d_idx,
                            static_cast<double *>(out->x), static_cast<double *>(out->chi2),
                            out->flags, out->iters);
     else
-        hipLaunchKernelGGL(k_type_scatter<float>, dim3(pgrid), dim3(256), 0, st, in->type_of,
-                           in->nch, ntypes, reinterpret_cast<const long long *>(offsets),
-                           reinterpret_cast<unsigned long long *>(cursors), ctx->d_idx,
+        hipLaunchKernelGGL(k_part_scatter<float>, dim3(nblk), dim3(256), 0, st, in->type_of,
+                           in->nch, ntypes, chunk, boff,
+                           reinterpret_cast<const long long *>(offsets), ctx->d_idx,
                            static_cast<float *>(out->x), static_cast<float *>(out->chi2),
                            out->flags, out->iters);
     HIP_TRY(hipGetLastError());

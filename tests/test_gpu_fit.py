@@ -157,6 +157,47 @@ def test_fit_rejects_bad_input(pkg, gpu):
         eng.fit(s, None, _abi.default_opts(block=100))
 
 
+@pytest.mark.parametrize("mode", ["plain", "fast", "warm"])
+def test_type_partition_equals_per_type_batches(pkg, gpu, mode):
+    """A mixed batch is split by type on the device (per-chunk counts, scans,
+    a stable scatter).  Batch sizes around the chunking (one wavefront, one
+    block, several 256-channel passes per chunk), with unset type ids mixed
+    in: every channel is fitted once with its own type, bit for bit as in a
+    batch of that type alone, and a channel with an unset type is flagged."""
+    from test_regression_amd import engine, workloads, warmstart, _abi
+    w = workloads.config("c3")
+    eng = engine.Engine(list(w.chtypes()))
+    if mode == "warm":
+        warmstart.ensure_trained(eng)
+    prec = _abi.MF_PREC_FP64 if mode == "plain" else _abi.MF_PREC_FP32
+    o = _abi.default_opts(precision=prec, warm_start=1 if mode == "warm" else 0)
+    dt = torch.float64 if mode == "plain" else torch.float32
+    rng = np.random.default_rng(31)
+    for n in (1, 63, 257, 131_073, 300_001):
+        s, _, _ = w.generate(n)
+        t = rng.choice(np.array([0, 1, 5], dtype=np.uint8), size=n, p=[0.6, 0.38, 0.02])
+        print(f"[gpu-fit] partition mode={mode} n={n}", flush=True)
+        x = torch.from_numpy(s).to("cuda", dt)
+        r = eng.fit(x, torch.from_numpy(t).cuda(), o)
+        torch.cuda.synchronize()
+        rx, rf, ri, rc = (r.x.cpu().numpy(), r.flags.cpu().numpy(), r.iters.cpu().numpy(),
+                          r.chi2.cpu().numpy())
+        bad = t == 5
+        assert np.all(rf[bad] == _abi.FLAGS["bad_type"]) and np.all(rx[:, bad] == 0)
+        assert not np.any(rf[~bad] & _abi.FLAGS["bad_type"])
+        for ty in (0, 1):
+            sel = np.flatnonzero(t == ty)
+            if sel.size == 0:
+                continue
+            q = eng.fit(x[:, torch.from_numpy(sel).cuda()].contiguous(),
+                        torch.full((sel.size,), ty, dtype=torch.uint8, device="cuda"), o)
+            torch.cuda.synchronize()
+            assert np.array_equal(q.x.cpu().numpy(), rx[:, sel]), (n, ty)
+            assert np.array_equal(q.flags.cpu().numpy(), rf[sel])
+            assert np.array_equal(q.iters.cpu().numpy(), ri[sel])
+            assert np.array_equal(q.chi2.cpu().numpy(), rc[sel])
+
+
 @pytest.mark.parametrize("pulse_cov", ["zero", "live"])
 def test_fp64_fit_minos_errors(pkg, gpu, pulse_cov):
     """compute_errors: MINOS-like in-time uncertainty (PulseChiSqSNNLS.cpp:73-146).
