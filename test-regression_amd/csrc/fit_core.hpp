@@ -1371,7 +1371,8 @@ MF_HD void fast_phase2(const DevType &ty, float (&s32)[10], const FitParams &prm
     out.n_inner = st.inner;
 }
 
-// The fast fit of one channel, both phases in one call.
+// The fast fit of one channel in one call: phase 1 from a cold start, then
+// phase 2 (a warm start begins at phase 2).
 template <int NS, int NP, bool LIVE, bool TAB = true, typename Park>
 MF_HD void fit_fast(const DevType &ty, const float (&s32_in)[10], const FitParams &prm,
                     float (&x)[10], uint32_t pm, bool warm, Park &park, FitOut<float> &out)

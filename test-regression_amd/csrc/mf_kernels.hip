@@ -273,8 +273,8 @@ void k_fast_fused(const DevType *__restrict__ types, SegArg seg,
 
 // The warm-started fast path in one launch: the MLP's prediction for the
 // block's channels (the network staged in LDS once per block, the forward
-// pass on MFMA, as k_mlp_warm), then both fit phases as k_fast_fused.  The
-// prediction never leaves the chip.  Blocks of FAST_WARM_BLOCK threads.
+// pass on MFMA, as k_mlp_warm), then the FP64 phase from that prediction.
+// The prediction never leaves the chip.  Blocks of FAST_WARM_BLOCK threads.
 constexpr int FAST_WARM_BLOCK = 256;
 template <typename TS, int N>
 __global__ __launch_bounds__(FAST_WARM_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
