@@ -107,19 +107,47 @@ def time_steps(eng, fn, steps, warmup, stream, kernel_steps=5):
     return a.elapsed_time(b) / steps, wall, kern
 
 
+def host_cpu():
+    """The host's CPU model (/proc/cpuinfo) and its logical CPU count."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu": model, "nproc": os.cpu_count()}
+
+
 def cpu_baseline(ct, s, t, seconds, threads):
-    """Oracle throughput on host cores over bounded slices of the batch,
-    repeated until about `seconds` of CPU work."""
+    """Oracle throughput on host cores over a bounded slice of the batch:
+    5 runs of about seconds/6 each with `threads` threads (their median is
+    the baseline), then one thread on a smaller slice for about seconds/6."""
     from oracle_binding import oracle_fit, oracle_opts
     n = min(s.shape[1], 200_000)
-    sl = np.ascontiguousarray(s[:, :n])
-    tl = t[:n] if t is not None else None
-    done, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
-        oracle_fit(ct, sl, tl, oracle_opts(), nthreads=threads)
-        done += n
-    dt = time.perf_counter() - t0
-    return done / dt, n, done, dt
+
+    def rate(nsl, th, budget):
+        a = np.ascontiguousarray(s[:, :nsl])
+        b = t[:nsl] if t is not None else None
+        done, t0 = 0, time.perf_counter()
+        while True:
+            oracle_fit(ct, a, b, oracle_opts(), nthreads=th)
+            done += nsl
+            dt = time.perf_counter() - t0
+            if dt >= budget:
+                return done / dt, done, dt
+
+    runs = [rate(n, threads, seconds / 6) for _ in range(5)]
+    n1 = min(n, 20_000)
+    single = rate(n1, 1, seconds / 6)
+    return {"value": float(np.median([r[0] for r in runs])), "unit": "channels/s",
+            "cores": threads, "kind": "port",
+            "runs": [r[0] for r in runs], "single_thread": single[0],
+            "fits": sum(r[1] for r in runs) + single[1],
+            "seconds": sum(r[2] for r in runs) + single[2], "slice": n, "slice_1t": n1,
+            "host": host_cpu()}
 
 
 def oracle_sample(ct, s, t, n):
@@ -231,11 +259,11 @@ def event_stream(args, eng, w, log):
     }
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        rate, n, done, dt = cpu_baseline(ct, sample, w.event_types(), args.cpu_seconds, threads)
-        line["cpu_baseline"] = {"value": rate, "unit": "channels/s", "cores": threads,
-                                "kind": "port",
-                                "sample": f"one event ({n} channels) fitted {done // n} times "
-                                          f"({dt:.1f} s) by the oracle, {threads} threads"}
+        cb = cpu_baseline(ct, sample, w.event_types(), args.cpu_seconds, threads)
+        cb["sample"] = (f"one event ({cb['slice']} channels) fitted by the oracle: median of "
+                        f"5 runs of {args.cpu_seconds / 6:.0f} s on {threads} threads; "
+                        f"single_thread: its first {cb['slice_1t']} channels on one thread")
+        line["cpu_baseline"] = cb
     print(json.dumps(line), flush=True)
 
 
@@ -407,12 +435,12 @@ def main():
         line["mlp"]["train"] = mlp_info["report"]
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        rate, n, done, dt = cpu_baseline(ct, s, t, args.cpu_seconds, threads)
-        line["cpu_baseline"] = {"value": rate, "unit": "channels/s", "cores": threads,
-                                "kind": "port",
-                                "sample": f"first {n} channels of the same batch fitted "
-                                          f"{done // n} times ({done} fits, {dt:.1f} s) by the "
-                                          f"oracle, FNNLS max_iter 1000, {threads} threads"}
+        cb = cpu_baseline(ct, s, t, args.cpu_seconds, threads)
+        cb["sample"] = (f"first {cb['slice']} channels of the same batch fitted by the oracle "
+                        f"(plain FP64 method, FNNLS max_iter 1000): median of 5 runs of "
+                        f"{args.cpu_seconds / 6:.0f} s on {threads} threads; single_thread: "
+                        f"the first {cb['slice_1t']} channels on one thread")
+        line["cpu_baseline"] = cb
     print(json.dumps(line), flush=True)
 
 
