@@ -123,6 +123,51 @@ def test_reader_errors(pkg, tmp_path):
         assert 0 < c.size < 100 and np.all(c == 2800)
 
 
+@needs_ref
+def test_reader_survives_corrupted_files(pkg, tmp_path):
+    """Corrupted copies of the reference's files (bytes flipped, cut short,
+    length fields made huge or negative): every call returns a value or an
+    MFError, none crashes or reads outside the file."""
+    from test_regression_amd import _abi, refdata
+    rng = np.random.default_rng(5)
+    branches = [("samples", _abi.MF_IO_VEC_F64), ("amplitudeTruth", _abi.MF_IO_F64),
+                ("energyPU", _abi.MF_IO_VEC_F64), ("nMinBias", _abi.MF_IO_VEC_I32),
+                ("BX0", _abi.MF_IO_I32), ("input_pedestal", _abi.MF_IO_F32)]
+    for src in (F100, SHAPE):
+        data = np.frombuffer(src.read_bytes(), dtype=np.uint8)
+        for trial in range(150):
+            m = data.copy()
+            mode = trial % 4
+            if mode == 0:
+                i = rng.integers(0, m.size, 8)
+                m[i] ^= rng.integers(1, 256, 8).astype(np.uint8)
+            elif mode == 1:
+                m = m[:rng.integers(64, m.size)]
+            elif mode == 2:
+                i = int(rng.integers(0, m.size - 4))
+                m[i:i + 4] = np.frombuffer(np.array([0xFFFFFFF0 if trial % 8 < 4 else 0x7FFFFF00],
+                                                    dtype=">u4").tobytes(), dtype=np.uint8)
+            else:
+                i = rng.integers(0, min(2048, m.size), 4)
+                m[i] ^= rng.integers(1, 256, 4).astype(np.uint8)
+            f = tmp_path / f"m{trial}.root"
+            f.write_bytes(m.tobytes())
+            try:
+                with refdata.RootFile(str(f)) as r:
+                    for b, kind in branches:
+                        try:
+                            r.read_branch("Samples", b, kind)
+                        except _abi.MFError:
+                            pass
+                    try:
+                        r.read_graph("PulseShape/grPulseShape")
+                    except _abi.MFError:
+                        pass
+            except _abi.MFError:
+                pass
+            f.unlink()
+
+
 def test_driver_constants(pkg):
     """Pulse::NoiseInit (Pulse.h:134-141) and the driver's template
     (multifit.cpp:48-68) from the pulse shape."""
