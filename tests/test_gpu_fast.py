@@ -77,17 +77,24 @@ def test_fast_fit_fp64_samples(pkg, gpu):
     _check(r, ref)
 
 
-def test_fast_fit_edge_cases(pkg, gpu):
-    from test_regression_amd import engine, workloads, _abi
+@pytest.mark.parametrize("warm", [0, 1])
+def test_fast_fit_edge_cases(pkg, gpu, warm):
+    """Zero, NaN, negative and infinite samples and a 1e5 ADC pulse, from a
+    cold start and from the MLP's prediction (which sees the same samples)."""
+    from test_regression_amd import engine, workloads, warmstart, _abi
     w = workloads.config("c1")
     ct = w.chtypes()
     eng = engine.Engine(list(ct))
+    if warm:
+        warmstart.ensure_trained(eng)
     s = np.zeros((10, 5), dtype=np.float32)
     s[:, 1] = np.nan
     s[:, 2] = 1e5 * np.exp(-0.5 * (np.arange(10) - 5) ** 2)
     s[:, 3] = -5.0
     s[4, 4] = np.inf
-    r = eng.fit(torch.from_numpy(s).cuda(), None, _abi.default_opts(precision=_abi.MF_PREC_FP32))
+    print(f"[gpu-fast] edge cases warm={warm}", flush=True)
+    r = eng.fit(torch.from_numpy(s).cuda(), None,
+                _abi.default_opts(precision=_abi.MF_PREC_FP32, warm_start=warm))
     torch.cuda.synchronize()
     x = r.x.cpu().numpy()
     fl = r.flags.cpu().numpy()
