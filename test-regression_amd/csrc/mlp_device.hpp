@@ -103,7 +103,7 @@ __device__ __forceinline__ void mlp_tiles_d(const MlpLds &L, const __bf16 (*xs)[
                                             TileFn &&tile)
 {
     const int g = lane >> 4, n = lane & 15;
-#pragma unroll 1
+#pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
         // an opaque zero offset per tile: the fragments are read from LDS
         // where they are used, not hoisted into registers for all tiles
