@@ -280,7 +280,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--oracle-sample", type=int, default=20_000)
     ap.add_argument("--mlp-weights", default="", help="parameters to load (default: the "
-                    "committed weights, else train on 4M c2-distribution channels)")
+                    "committed weights for the workload: c3's own, else those of the c2 "
+                    "distribution; missing ones are trained on 4M channels)")
     ap.add_argument("--save-mlp", default="", help="write the trained parameters here")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="skip mf_autotune (default launch shapes)")
@@ -319,9 +320,10 @@ def main():
     stream = torch.cuda.current_stream()
 
     # learned warm start: load or train (not timed)
-    wpath = Path(args.mlp_weights) if args.mlp_weights else warmstart.WEIGHTS
+    wpath = Path(args.mlp_weights) if args.mlp_weights else warmstart.weights_for(args.config)
     t0 = time.perf_counter()
-    mlp_info = warmstart.ensure_trained(eng, path=wpath if wpath.exists() else None, log=log)
+    mlp_info = warmstart.ensure_trained(eng, warmstart.training_workload(args.config),
+                                        path=wpath if wpath.exists() else None, log=log)
     mlp_info["seconds"] = time.perf_counter() - t0
     if args.save_mlp:
         np.save(args.save_mlp, eng.mlp_get_params())
