@@ -185,8 +185,11 @@ def event_stream(args, eng, w, log):
     pool = torch.from_numpy(np.ascontiguousarray(
         s.reshape(10, P, nev).transpose(1, 0, 2)).astype(np.float32)).cuda()
     ranges = np.concatenate([[0], np.cumsum(w.event)]).tolist()   # barrel, then endcap
-    wpath = Path(args.mlp_weights) if args.mlp_weights else warmstart.WEIGHTS
-    warmstart.ensure_trained(eng, path=wpath if wpath.exists() else None, log=log)
+    wpath = Path(args.mlp_weights) if args.mlp_weights else warmstart.weights_for(args.config)
+    mlp_info = warmstart.ensure_trained(eng, warmstart.training_workload(args.config),
+                                        path=wpath if wpath.exists() else None, log=log)
+    if args.save_mlp:
+        np.save(args.save_mlp, eng.mlp_get_params())
     o = _abi.default_opts(precision=_abi.MF_PREC_FP32, warm_start=1)
     outs = [eng.alloc_result(nev, _abi.MF_PREC_FP32) for _ in range(P)]
     graphs = [eng.capture(pool[e], None, o, outs[e], type_ranges=ranges) for e in range(P)]
@@ -256,6 +259,7 @@ def event_stream(args, eng, w, log):
                      "aflops_per_channel": af,
                      "mlp_mfma_flops_per_channel_not_counted": MLP_FLOPS_PER_CHANNEL},
         "accuracy_vs_plain_fp64": acc,
+        "mlp": {k: v for k, v in mlp_info.items() if k != "report"},
     }
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

@@ -33,9 +33,10 @@ def weights_for(workload: str) -> Path:
 
 
 def training_workload(workload: str) -> str:
-    """The distribution a workload's parameters are trained on: c3's own
-    mixture for c3, the c2 distribution (c4) otherwise."""
-    return "c3" if workload == "c3" else "c4"
+    """The distribution a workload's parameters are trained on: its own
+    mixture where its own parameters are committed (c3), the c2 distribution
+    (c4) otherwise."""
+    return workload if workload in WEIGHTS_BY_WORKLOAD else "c4"
 
 
 @dataclass
@@ -95,8 +96,9 @@ def evaluate(eng, samples32: torch.Tensor, xref: torch.Tensor) -> dict:
 def ensure_trained(eng, workload_cfg: str = "c4", n_train: int = 4_000_000, epochs: int = 400,
                    path: Path | None = WEIGHTS, log=None) -> dict:
     """Loads saved parameters, or trains on n_train channels of a workload's
-    distribution (c4: the c2 distribution; c3: its ECAL/HCAL mixture with its
-    type ids), labelled by the plain FP64 fit, and saves them.  The committed
+    distribution (c4: the c2 distribution; a mixed workload such as c3: its
+    mixture of channel types, with their type ids), labelled by the plain
+    FP64 fit, and saves them.  The committed
     parameters were made this way: 400 epochs of batch 8192, Adam with a
     cosine learning rate from 3e-3, seed 0 (about two minutes on one
     MI355X)."""
