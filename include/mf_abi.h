@@ -4,9 +4,12 @@
  * Plain C, plain pointers and sizes.  Device buffers are HIP device pointers,
  * streams are hipStream_t passed as void* (NULL = the null stream).  Every
  * entry point returns an mf_status; mf_last_error() gives the message of the
- * last failure on the calling thread.  Nothing here allocates on a caller's
- * behalf except mf_context_create / mf_set_types (type tables) and the
- * scratch of mf_fit_batch (kept in the context, grown on demand).
+ * last failure on the calling thread.  Device memory is allocated only into
+ * the context, kept there and freed by mf_context_destroy: the queue heads
+ * (mf_context_create), the type tables (mf_set_types), the scratch of
+ * mf_fit_batch (type partition, warm-start prediction; grown on demand, never
+ * while a graph that uses it exists) and the MLP's parameters and training
+ * activations (mf_mlp_*).
  *
  * Reference interfaces each entry point replaces (paths relative to the
  * reference repository root):
