@@ -131,3 +131,34 @@ def test_mlp_training_and_warm_start(pkg, gpu):
     inner_w = warm.n_inner().float().mean().item()
     print(f"[gpu-mlp] solves per channel: cold {inner_c:.2f} warm {inner_w:.2f}", flush=True)
     assert inner_w < 0.6 * inner_c
+
+
+def test_committed_networks_on_held_out_channels(pkg, gpu):
+    """The committed parameters on channels neither network was trained on
+    (another seed): the c2 network on the c2 distribution; on the c3 mixture
+    (ECAL and HCAL-like types), the network trained on that mixture predicts
+    the exact passive set far more often than the c2 one, and the warm path
+    then takes fewer solves."""
+    from test_regression_amd import _abi, engine, warmstart, workloads
+    res = {}
+    for cfg, wts in (("c2", "c2"), ("c3", "c2"), ("c3", "c3")):
+        w = workloads.config(cfg)
+        w.seed += 2000
+        s, t, _ = w.generate(100_000)
+        eng = engine.Engine(list(w.chtypes()))
+        eng.mlp_set_params(np.load(warmstart.weights_for(wts)))
+        tt = torch.from_numpy(t).cuda() if t is not None else None
+        xref = warmstart.label(eng, torch.from_numpy(s).cuda(), tt)
+        s32 = torch.from_numpy(s.astype(np.float32)).cuda()
+        ev = warmstart.evaluate(eng, s32, xref)
+        r = eng.fit(s32, tt, _abi.default_opts(precision=_abi.MF_PREC_FP32, warm_start=1))
+        torch.cuda.synchronize()
+        xw = r.x.double().cpu().numpy()
+        xr = xref.double().cpu().numpy()
+        res[(cfg, wts)] = (ev["exact_active_set"], r.n_inner().float().mean().item())
+        print(f"[gpu-mlp] {cfg} with the {wts} network: exact {ev['exact_active_set']:.3f}, "
+              f"solves {res[(cfg, wts)][1]:.2f}", flush=True)
+        assert ((xw > 0) == (xr > 0)).all(axis=0).mean() >= 0.999
+    assert res[("c2", "c2")][0] > 0.7 and res[("c3", "c3")][0] > 0.7
+    assert res[("c3", "c3")][0] > res[("c3", "c2")][0] + 0.15
+    assert res[("c3", "c3")][1] < res[("c3", "c2")][1]
