@@ -18,14 +18,16 @@ TOL_CHI2 = 1e-4
 
 
 def _fit_fast(cfg, pulse_cov, n, sample_dtype=np.float32, **kw):
-    from test_regression_amd import engine, workloads, _abi
+    from test_regression_amd import engine, workloads, warmstart, _abi
     w = workloads.config(cfg, pulse_cov)
     ct = w.chtypes()
     s, t, _ = w.generate(n)
     s_in = s.astype(sample_dtype)
     eng = engine.Engine(list(ct))
-    print(f"[gpu-fast] cfg={cfg} pulse_cov={pulse_cov} n={n} dtype={sample_dtype.__name__}",
-          flush=True)
+    if kw.get("warm_start"):
+        warmstart.ensure_trained(eng, path=warmstart.weights_for(cfg))
+    print(f"[gpu-fast] cfg={cfg} pulse_cov={pulse_cov} n={n} dtype={sample_dtype.__name__} "
+          f"{kw}", flush=True)
     r = eng.fit(torch.from_numpy(s_in).cuda(),
                 torch.from_numpy(t).cuda() if t is not None else None,
                 _abi.default_opts(precision=_abi.MF_PREC_FP32, **kw))
@@ -63,8 +65,11 @@ def _check(r, ref):
     ("c5", "zero", 20_000),
     ("c5", "live", 10_000),
 ])
-def test_fast_fit_matches_oracle(pkg, gpu, cfg, pulse_cov, n):
-    r, ref = _fit_fast(cfg, pulse_cov, n)
+@pytest.mark.parametrize("warm", [0, 1])
+def test_fast_fit_matches_oracle(pkg, gpu, cfg, pulse_cov, n, warm):
+    """Cold and warm-started (the workload's committed network) fast path
+    against the oracle on the same FP32-rounded samples."""
+    r, ref = _fit_fast(cfg, pulse_cov, n, warm_start=warm)
     _check(r, ref)
     fl = r.flags.cpu().numpy()
     from test_regression_amd import _abi
