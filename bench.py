@@ -375,6 +375,23 @@ def main():
             f"{nch / (res[name]['ms'] / 1e3):.3e} ch/s")
     torch.cuda.synchronize()
 
+    # the warm call captured as a graph (mf_graph_capture runs it once eagerly
+    # first, into the same outputs) and replayed K times, timed as above
+    g = eng.capture(paths["warm"][0], tt, paths["warm"][1], outs["warm"])
+    for _ in range(args.warmup):
+        g.launch()
+    torch.cuda.synchronize()
+    ga, gb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda._sleep(GATE_CYCLES)
+    ga.record(stream)
+    for _ in range(args.steps):
+        g.launch()
+    gb.record(stream)
+    torch.cuda.synchronize()
+    graph_ms = ga.elapsed_time(gb) / args.steps
+    g.close()
+    log(f"warm, graph replays: {graph_ms:.3f} ms/step")
+
     # accuracy: warm (and cold) fast path vs the plain FP64 fit on the whole batch
     xp = outs["plain"].x.cpu().numpy()
     cp = outs["plain"].chi2.cpu().numpy()
@@ -427,6 +444,7 @@ def main():
                        "ms_per_step": res["plain"]["ms"]},
         "fast_cold_fp32": {"channels_per_s": nch / (res["fast"]["ms"] / 1e3),
                            "ms_per_step": res["fast"]["ms"]},
+        "warm_graph_ms_per_step": graph_ms,
         "speedup_warm_vs_plain_fp64": res["plain"]["ms"] / res["warm"]["ms"],
         "speedup_cold_vs_plain_fp64": res["plain"]["ms"] / res["fast"]["ms"],
         "mlp_ms_per_step": res["warm"]["mlp_ms"],
