@@ -224,3 +224,39 @@ def test_fp64_fit_minos_errors(pkg, gpu, pulse_cov):
     xr = ref["x"].T
     scale = np.maximum(np.abs(xr).max(axis=0), 1.0)
     assert np.quantile((np.abs(r.x.cpu().numpy() - xr) / scale).max(axis=0), 0.999) <= 1e-10
+
+
+@pytest.mark.parametrize("mode", ["plain", "fast", "warm"])
+def test_repeated_fits_are_identical(pkg, gpu, mode):
+    """The same call repeated on a mixed batch (type partition, per-type
+    launches, the MLP inside the warm launch) gives the same bits every time:
+    no result depends on the order in which blocks or atomics run."""
+    import hashlib
+    from test_regression_amd import engine, workloads, warmstart, _abi
+    w = workloads.config("c3")
+    s, t, _ = w.generate(100_000)
+    eng = engine.Engine(list(w.chtypes()))
+    if mode == "warm":
+        eng.mlp_set_params(np.load(warmstart.weights_for("c3")))
+    prec = _abi.MF_PREC_FP64 if mode == "plain" else _abi.MF_PREC_FP32
+    o = _abi.default_opts(precision=prec, warm_start=1 if mode == "warm" else 0)
+    x = torch.from_numpy(s).to("cuda", torch.float64 if mode == "plain" else torch.float32)
+    tt = torch.from_numpy(t).cuda()
+    out = eng.alloc_result(x.shape[1], prec)
+    print(f"[gpu-fit] repeated fits mode={mode}", flush=True)
+
+    def digest():
+        h = hashlib.sha1()
+        for a in (out.x, out.chi2, out.flags, out.iters):
+            h.update(a.cpu().numpy().tobytes())
+        return h.hexdigest()
+
+    eng.fit(x, tt, o, out)
+    torch.cuda.synchronize()
+    ref = digest()
+    for _ in range(10):
+        out.x.zero_()
+        out.flags.zero_()
+        eng.fit(x, tt, o, out)
+        torch.cuda.synchronize()
+        assert digest() == ref
