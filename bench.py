@@ -74,6 +74,16 @@ def log(msg):
 GATE_CYCLES = 2_000_000
 
 
+def gate_stream():
+    """Holds the current stream for GATE_CYCLES (torch's spin kernel, a
+    private helper: where it is missing the region starts ungated and also
+    holds the host's latency for its first call)."""
+    import torch
+    sleep = getattr(torch.cuda, "_sleep", None)
+    if sleep is not None:
+        sleep(GATE_CYCLES)
+
+
 def time_steps(eng, fn, steps, warmup, stream, kernel_steps=5):
     """K steps back to back between two events on the launch stream, with a
     synchronize on both sides (ms per step).  A spin kernel holds the stream
@@ -92,7 +102,7 @@ def time_steps(eng, fn, steps, warmup, stream, kernel_steps=5):
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) * 1e3 / steps
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda._sleep(GATE_CYCLES)
+    gate_stream()
     a.record(stream)
     for _ in range(steps):
         fn()
@@ -382,7 +392,7 @@ def main():
         g.launch()
     torch.cuda.synchronize()
     ga, gb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda._sleep(GATE_CYCLES)
+    gate_stream()
     ga.record(stream)
     for _ in range(args.steps):
         g.launch()
