@@ -222,10 +222,42 @@ def event_stream(args, eng, w, log):
         lat = np.array([x.elapsed_time(y) for x, y in zip(st, en)])
         return a.elapsed_time(b), lat
 
+    def run_inflight(k):
+        """Event i replayed on stream i mod k (default priority), so that k
+        events are in flight; k divides the pool size, so a graph (and its
+        output buffers) is only ever replayed on one stream."""
+        streams = [torch.cuda.Stream() for _ in range(k)]
+        for i in range(min(P, E)):
+            graphs[i % P].launch(streams[i % k])
+        torch.cuda.synchronize()
+        st = [torch.cuda.Event(enable_timing=True) for _ in range(E)]
+        en = [torch.cuda.Event(enable_timing=True) for _ in range(E)]
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for sm in streams:
+            sm.wait_event(a)
+        for i in range(E):
+            sm = streams[i % k]
+            st[i].record(sm)
+            graphs[i % P].launch(sm)
+            en[i].record(sm)
+        for sm in streams:
+            d = torch.cuda.Event()
+            d.record(sm)
+            stream.wait_event(d)
+        b.record(stream)
+        torch.cuda.synchronize()
+        lat = np.array([x.elapsed_time(y) for x, y in zip(st, en)])
+        return a.elapsed_time(b), lat
+
     total, lat = run(lambda i: graphs[i % P].launch())
     total_e, lat_e = run(lambda i: eng.fit(pool[i % P], None, o, outs[i % P], type_ranges=ranges))
     log(f"graph: {total:.1f} ms for {E} events, p50 {np.median(lat):.4f} ms, "
         f"p99 {np.quantile(lat, 0.99):.4f} ms; eager: {total_e:.1f} ms")
+    K_INFLIGHT = 4 if P % 4 == 0 else 1
+    total_k, lat_k = run_inflight(K_INFLIGHT)
+    log(f"graph, {K_INFLIGHT} events in flight: {total_k:.1f} ms for {E} events, "
+        f"p50 {np.median(lat_k):.4f} ms, p99 {np.quantile(lat_k, 0.99):.4f} ms")
     # accuracy of one event against the plain FP64 fit
     ref = eng.fit(pool[0].double(), None, _abi.default_opts(precision=_abi.MF_PREC_FP64),
                   type_ranges=ranges)
@@ -260,6 +292,9 @@ def event_stream(args, eng, w, log):
                    "events": E, "channels_per_event": nev, "event_pool": P},
         "latency_ms": {"p50": float(np.median(lat)), "p99": float(np.quantile(lat, 0.99)),
                        "mean": float(lat.mean()), "max": float(lat.max())},
+        "inflight": {"events_in_flight": K_INFLIGHT, "channels_per_s": E * nev / (total_k / 1e3),
+                     "latency_ms_p50": float(np.median(lat_k)),
+                     "latency_ms_p99": float(np.quantile(lat_k, 0.99))},
         "eager": {"channels_per_s": E * nev / (total_e / 1e3),
                   "latency_ms_p50": float(np.median(lat_e)),
                   "latency_ms_p99": float(np.quantile(lat_e, 0.99))},
