@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define MF_ABI_VERSION 2
+#define MF_ABI_VERSION 3
 #define MF_MAX_N 10   /* samples and pulses per channel */
 #define MF_FULL 19    /* FullSampleVector length (EigenMatrixTypes.h:7) */
 
@@ -53,11 +53,14 @@ typedef enum {
  * The pulse matrix column of pulse j holds fullpulse[f_j + o_j ...] in rows
  * f_j .. nsamples-1 with f_j = max(0, bx_j + first_off), o_j = pulse_off - bx_j
  * (PulseChiSqSNNLS.cpp:50-62: first_off = 3, pulse_off = 4 for ECAL).
+ * npulses is bxs.rows() of the reference (its BXVector is dynamic, at most
+ * 10: multifit_cpu/interface/EigenMatrixTypes.h:9-10); amplitudes of the
+ * slots npulses .. nsamples-1 come back 0.
  * Matrices are row-major: noisecor[i*10+j], fullpulsecov[i*19+j].
  * Layout is identical to the oracle's or_chtype. */
 typedef struct mf_chtype {
     int32_t nsamples;          /* 8 or 10                                  */
-    int32_t npulses;           /* == nsamples (8x8 or 10x10)               */
+    int32_t npulses;           /* 1 .. nsamples                            */
     int32_t bx[MF_MAX_N];      /* BX offset of each pulse                  */
     int32_t first_off;
     int32_t pulse_off;
@@ -115,7 +118,9 @@ enum {
     MF_F_MINIMIZE_CAP = 1,   /* Minimize ran max_minimize iterations          */
     MF_F_LLT_FAIL     = 2,   /* a Cholesky pivot was <= 0 or NaN              */
     MF_F_FNNLS_CAP    = 4,   /* an FNNLS call hit max_iter                    */
-    MF_F_NONFINITE    = 8,   /* non-finite input sample: not fitted, x = 0    */
+    MF_F_NONFINITE    = 8,   /* non-finite input sample, or a per-channel pederr
+                                whose square is not positive and finite: not
+                                fitted, x = 0                                 */
     MF_F_FP64_PIVOT   = 16,  /* FP32 path: a pivot was recomputed in FP64     */
     MF_F_WARM_REJECT  = 32,  /* warm start failed its KKT check; the active-set
                                 iterations went on from it                    */
@@ -124,7 +129,14 @@ enum {
 };
 
 /* Device-resident batch, structure of arrays.  samples[i*nch + c] for sample
- * i of channel c (dtype MF_DTYPE_*).  The channel types, one of:
+ * i of channel c (dtype MF_DTYPE_*).  With MF_PREC_FP32 the samples are
+ * used rounded to float32 (also FP64 samples, in both of the fast path's
+ * phases): its accuracy figures are those against the FP64 fit of the
+ * rounded samples.  pederr (device, optional): the noise sigma of each
+ * channel, the pederr argument of DoFit, replacing the type's (NULL: the
+ * type's pederr for every channel); the noise covariance is then
+ * pederr[c]^2 / type.pederr^2 times the type's, formed exactly as
+ * pederr^2 * noisecor when the type's pederr is 1.  The channel types, one of:
  *   type_of (device, one byte per channel): index into the table passed to
  *     mf_set_types; the batch is partitioned by type on the device;
  *   ranges (host, nranges + 1 ascending channel numbers): channels
@@ -139,6 +151,7 @@ typedef struct mf_batch {
     const void    *samples;
     const uint8_t *type_of;
     const int64_t *ranges;
+    const double  *pederr;
 } mf_batch;
 
 /* Device-resident results, structure of arrays, in the compute precision
@@ -163,8 +176,9 @@ int mf_abi_version(void);
 
 mf_status mf_context_create(mf_context **ctx, int device);
 mf_status mf_context_destroy(mf_context *ctx);
-/* Copies ntypes channel types (host memory) to the device.  Shapes other
- * than 10x10 and 8x8 are rejected. */
+/* Copies ntypes channel types (host memory) to the device.  nsamples must be
+ * 10 or 8, npulses 1..nsamples.  Refused while a graph of the context exists
+ * (its kernels read the tables). */
 mf_status mf_set_types(mf_context *ctx, const mf_chtype *types, int32_t ntypes);
 mf_status mf_default_opts(mf_opts *opts);
 

@@ -93,6 +93,10 @@ class Device {
     int device() const { return device_; }
     // uploads types unless the same table is already on the device
     void set_types(const std::vector<mf_chtype> &types);
+    // a device buffer of at least `bytes` for one call's inputs and outputs:
+    // the device's staging area, grown to the largest call and kept (one call
+    // at a time, like the reference's single-threaded drivers)
+    void *staging(size_t bytes);
     ~Device();
     Device(const Device &) = delete;
     Device &operator=(const Device &) = delete;
@@ -103,6 +107,8 @@ class Device {
     mf_context *ctx_ = nullptr;
     void *stream_ = nullptr;
     std::vector<mf_chtype> cached_;
+    void *stage_ = nullptr;
+    size_t stage_cap_ = 0;
 };
 
 // mf_chtype from DoFit's arguments: ECAL layout (firstsamplet = max(0, bx + 3),
@@ -177,8 +183,10 @@ struct Output {
 };
 
 // Batched DoFit (errors disabled, as kernel_multifit runs it): channels that
-// share their constants share a type; one mf_fit_batch, plain FP64 path,
-// FNNLS max_iter 1000 (multifit_gpu/src/PulseChiSqSNNLS.cu:215).
+// share their constants but pederr share a type (at most 256 types), pederr
+// goes per channel (mf_batch.pederr); one mf_fit_batch, plain FP64 path,
+// FNNLS max_iter 1000 (multifit_gpu/src/PulseChiSqSNNLS.cu:215).  BX lists
+// of 1..10 pulses; Output::X has bxs.rows() entries.
 // Output::ampl is X at the first BX == 0, as multifit_cpu computes it
 // (multifit_cpu/src/multifit.cpp:203-211); the reference GPU kernel's test
 // `ip < BXs(ip) == 0` (PulseChiSqSNNLS.cu:245) picks index 0 instead.

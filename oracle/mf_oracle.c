@@ -897,6 +897,7 @@ typedef struct {
     const or_chtype *types;
     const uint8_t *type_of;
     const double *samples;
+    const double *pederr;
     long long nch;
     const or_opts *o;
     or_result *res;
@@ -915,7 +916,13 @@ static void *batch_worker(void *arg)
             const or_chtype *t = b->types + (b->type_of ? b->type_of[c] : 0);
             double s[LD];
             for (int i = 0; i < t->nsamples; ++i) s[i] = b->samples[i * b->nch + c];
-            or_dofit(t, s, b->o, b->res + c);
+            if (b->pederr) {   /* DoFit's pederr argument, channel by channel */
+                or_chtype tc = *t;
+                tc.pederr = b->pederr[c];
+                or_dofit(&tc, s, b->o, b->res + c);
+            } else {
+                or_dofit(t, s, b->o, b->res + c);
+            }
         }
     }
     return NULL;
@@ -925,10 +932,18 @@ void or_dofit_batch(const or_chtype *types, const uint8_t *type_of,
                     const double *samples, long long nch, const or_opts *o,
                     or_result *res, int nthreads)
 {
+    or_dofit_batch_pederr(types, type_of, samples, NULL, nch, o, res, nthreads);
+}
+
+void or_dofit_batch_pederr(const or_chtype *types, const uint8_t *type_of,
+                           const double *samples, const double *pederr, long long nch,
+                           const or_opts *o, or_result *res, int nthreads)
+{
     batchctx b;
     b.types = types;
     b.type_of = type_of;
     b.samples = samples;
+    b.pederr = pederr;
     b.nch = nch;
     b.o = o;
     b.res = res;

@@ -139,6 +139,11 @@ int or_dofit(const or_chtype *t, const double *samples, const or_opts *o,
 void or_dofit_batch(const or_chtype *types, const uint8_t *type_of,
                     const double *samples, long long nch, const or_opts *o,
                     or_result *res, int nthreads);
+/* The same with DoFit's pederr given per channel (pederr[c] replaces the
+ * type's; NULL: the type's). */
+void or_dofit_batch_pederr(const or_chtype *types, const uint8_t *type_of,
+                           const double *samples, const double *pederr, long long nch,
+                           const or_opts *o, or_result *res, int nthreads);
 
 #ifdef __cplusplus
 }

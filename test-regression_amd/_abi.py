@@ -51,6 +51,7 @@ class Batch(C.Structure):
     _fields_ = [
         ("nch", C.c_int64), ("sample_dtype", C.c_int32), ("nranges", C.c_int32),
         ("samples", C.c_void_p), ("type_of", C.c_void_p), ("ranges", C.c_void_p),
+        ("pederr", C.c_void_p),
     ]
 
 

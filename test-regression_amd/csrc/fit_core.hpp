@@ -143,9 +143,11 @@ template <typename T> struct TypeMats {
     T G0[10][10];      // A0^T L0^{-1}: Atb = G0 s
 };
 struct TypeHdr {
-    int32_t ns, np, has_pi, intime;
+    int32_t ns, np, has_pi, intime;  // np: the kernel's pulse slots (= ns)
     int32_t first[10];
-    int32_t pad[2];
+    uint32_t pmask;  // the type's pulses: bits 0 .. npulses-1; slots past them hold
+                     // zero columns, whose w is 0 and which never become passive
+    int32_t npv;     // npulses of the type (<= ns)
 };
 // Inverses of the passive blocks of the x = 0 normal matrix AtA0, one per
 // passive set m (bit mask over the pulses, 1 <= m < 2^np).  At x = 0 the
@@ -168,7 +170,8 @@ struct DevType {
     TypeMats<double> d;
     TypeMats<float> f;
     const MaskTab *tab;  // passive-set factors of AtA0 (same memory space as the DevType)
-    int64_t pad_;
+    double inv_pederr2;  // 1 / pederr^2 of the type (a per-channel pederr p scales C0 by
+                         // rho = p^2 / pederr^2)
 };
 template <typename T> MF_HD const TypeMats<T> &mats(const DevType &t);
 template <> MF_HD const TypeMats<double> &mats<double>(const DevType &t) { return t.d; }
@@ -185,7 +188,17 @@ struct FitParams {
     float eps_rel32;     // FP32 path: rounding allowance of w (see fnnls_inplace)
     float tau32;         // FP32 path: pivot ratio below which a solve goes to FP64
     double eps_rel64;    // FP32 path: rounding allowance of w in the FP64 phase
+    // Per channel (a kernel sets them on its lane's copy):
+    double rho = 1.0;          // (channel pederr / type pederr)^2: the noise covariance is
+                               // rho C0, so the x = 0 normal equations are AtA0 / rho and
+                               // G0 s / rho and chi2 of the type's whitening is rho chi2
+    uint32_t pmask = 0x3FFu;   // the type's pulses (TypeHdr::pmask)
+    int32_t pad_ = 0;
 };
+
+// A per-channel noise sigma must be positive and finite; a channel whose
+// rho is not is not fitted (F_NONFINITE), like one with non-finite samples.
+MF_HD bool rho_ok(double r) { return r > 0.0 && r <= 1.7976931348623157e308; }
 
 enum {
     F_MINIMIZE_CAP = 1, F_LLT_FAIL = 2, F_FNNLS_CAP = 4, F_NONFINITE = 8,
@@ -256,13 +269,17 @@ MF_HD void invert_lower(T (&L)[10][10], const T (&ri)[10])
 // updateCov (PulseChiSqSNNLS.cpp:169-194): lower(C) = pederr^2 R + sum_p x_p^2 Pi_p.
 // A pulse with x_p == 0 adds fma(0, Pi, C) == C exactly, so the reference's
 // `continue` (:178-179) needs no branch.
+// rho: the channel's scale of the type's noise covariance (1: the type's
+// pederr; the type's pederr 1 and rho = pederr^2 give pederr^2 * noisecor
+// exactly as the reference forms it).
 template <typename T, int NS, int NP>
-MF_HD void build_cov(const TypeMats<T> &tm, bool use_pi, const T (&x)[10], T (&C)[10][10])
+MF_HD void build_cov(const TypeMats<T> &tm, bool use_pi, const T (&x)[10], T (&C)[10][10],
+                     T rho = T(1))
 {
 #pragma unroll
     for (int i = 0; i < NS; ++i)
 #pragma unroll
-        for (int j = 0; j <= i; ++j) C[i][j] = tm.C0[i][j];
+        for (int j = 0; j <= i; ++j) C[i][j] = rho * tm.C0[i][j];
     if (use_pi) {
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
@@ -789,11 +806,13 @@ MF_HD bool lh_nnls(const T (&AtA)[10][10], const T (&Atb)[10], T (&x)[10], T eps
 // is still active (:82-95) and then sizes the passive block from a counter
 // that is one short in that case, reading an unset row; here the passive
 // block is the mask itself, as in the oracle.
+// pmask: the columns that exist (a problem of fewer than N columns embedded
+// in N; the "one index short" test counts only these).
 template <typename T, int N>
 MF_HD void fnnls_mask(const T (&AtA)[10][10], const T (&Atb)[10], T (&x)[10], T eps,
-                      int max_iter, FnnlsStats &st)
+                      int max_iter, FnnlsStats &st, uint32_t pmask = 0x3FFu)
 {
-    uint32_t act = (1u << N) - 1u;  // active (= not passive) set
+    uint32_t act = pmask & ((1u << N) - 1u);  // active (= not passive) set
     T s[10];
 #pragma unroll
     for (int i = 0; i < N; ++i) s[i] = T(0);
@@ -819,7 +838,7 @@ MF_HD void fnnls_mask(const T (&AtA)[10][10], const T (&Atb)[10], T (&x)[10], T 
         if (!nact || wm < eps) break;                           // :113
         st.outer++;
         for (;;) {                                              // :118
-            const uint32_t pm = ((1u << N) - 1u) & ~act;
+            const uint32_t pm = pmask & ((1u << N) - 1u) & ~act;
             if (!pm) {
 #pragma unroll
                 for (int i = 0; i < N; ++i) x[i] = s[i];
@@ -875,7 +894,7 @@ MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams 
 {
     const TypeMats<T> &tm = *opaque(&tm0);
     T Li[10][10], ri[10];
-    build_cov<T, NS, NP>(tm, prm.use_pi != 0, x, Li);
+    build_cov<T, NS, NP>(tm, prm.use_pi != 0, x, Li, T(prm.rho));
     if (!cholesky_inplace<T, NS>(Li, ri)) flags |= F_LLT_FAIL;
     invert_lower<T, NS>(Li, ri);
     T AtA[10][10], Atb[10];
@@ -889,7 +908,7 @@ MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams 
     else if (XS == XS_LH)   // PulseChiSqSNNLS.cpp:253-257 (setX, solve, x)
         lh_nnls<T, NP>(AtA, Atb, x, T(prm.eps), prm.max_iter, st);
     else if (prm.solver == 1)
-        fnnls_mask<T, NP>(AtA, Atb, x, T(prm.eps), prm.max_iter, st);
+        fnnls_mask<T, NP>(AtA, Atb, x, T(prm.eps), prm.max_iter, st, prm.pmask);
     else
         fnnls_inplace<T, NP>(AtA, Atb, x, T(prm.eps), T(0), prm.max_iter, st);
 #pragma unroll
@@ -1017,7 +1036,7 @@ MF_HD void fit_plain(const TypeMats<T> &tm, const T (&s)[10], const FitParams &p
     LegacyState ls;   // legacy solver: passive state across Minimize iterations
     ls.reset();
     LegacyState *lsp = XS == XS_LEGACY ? &ls : nullptr;
-    if (all_finite<T, NS>(s))
+    if (all_finite<T, NS>(s) && rho_ok(prm.rho))
         nmin = minimize_from<T, NS, NP, XS>(tm, s, prm, park, x, chi2, 0, st, flags, -1, lsp);
     else
         flags |= F_NONFINITE;
@@ -1244,8 +1263,10 @@ MF_HD void fast_phase1(const DevType &ty, const float (&s32)[10], const FitParam
         Atb[p] = t;
     }
     bool ill = false;
-    fnnls_active<float, NP, true>(tf.AtA0, Atb, x, pm, (float)prm.eps, prm.eps_rel32,
-                                  prm.max_iter, st, prm.tau32, &ill);
+    // AtA0 and G0 s are rho times the channel's normal equations: the same
+    // solutions, w scaled by rho, so the threshold is rho eps
+    fnnls_active<float, NP, true>(tf.AtA0, Atb, x, pm, (float)(prm.eps * prm.rho),
+                                  prm.eps_rel32, prm.max_iter, st, prm.tau32, &ill);
     if (ill) flags |= F_FP64_PIVOT;
 }
 
@@ -1266,7 +1287,7 @@ MF_HD bool fast_p2_begin(const DevType &ty, const float (&s32)[10], const FitPar
     S.pm = 0;
     S.solve = false;
     S.adds = 0;
-    if (!all_finite<float, NS>(s32)) {
+    if (!all_finite<float, NS>(s32) || !rho_ok(prm.rho)) {
         flags |= F_NONFINITE;
         return false;
     }
@@ -1332,13 +1353,14 @@ MF_HD void fast_phase2(const DevType &ty, float (&s32)[10], const FitParams &prm
     if (fast_p2_begin<NS, NP>(ty, s32, prm, x, Atb64, S, flags)) {
         const int outer0 = st.outer;
         const MaskTab *tab = *opaque(&ty.tab);   // wave-uniform: one scalar load
-        while (!fnnls_step<double, NP, true, TAB>(ty.d.AtA0, Atb64, S, prm.eps, prm.eps_rel64,
+        const double eps = prm.eps * prm.rho;
+        while (!fnnls_step<double, NP, true, TAB>(ty.d.AtA0, Atb64, S, eps, prm.eps_rel64,
                                                   prm.max_iter, st, 0.0, nullptr, tab)) {
         }
         flags |= F_REFINED;
         if (warm && st.outer != outer0) flags |= F_WARM_REJECT;
         reload(s32);
-        chi2 = fast_p2_chi2<NS, NP>(ty, s32, S.x);
+        chi2 = fast_p2_chi2<NS, NP>(ty, s32, S.x) / prm.rho;
         // With a live pulse covariance the later iterations run the
         // reference's FP64 iteration body (per-channel covariance,
         // inplace_fnnls with warm x and reset passive set); they are FP64
@@ -1382,7 +1404,7 @@ MF_HD void fit_fast(const DevType &ty, const float (&s32_in)[10], const FitParam
     float s32[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) s32[i] = s32_in[i];
-    if (!warm && all_finite<float, NS>(s32) && prm.max_minimize > 0)
+    if (!warm && all_finite<float, NS>(s32) && rho_ok(prm.rho) && prm.max_minimize > 0)
         fast_phase1<NS, NP>(ty, s32, prm, x, pm, st, flags);
     fast_phase2<NS, NP, LIVE, TAB>(ty, s32, prm, x, warm, st, flags, park, out);
 }

@@ -24,19 +24,17 @@ static void hip_check(hipError_t e, const char *what)
         throw Error(MF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// device buffer owned for the duration of one call
-struct DevBuf {
-    void *p = nullptr;
-    explicit DevBuf(size_t n)
+// Device buffers of one call, carved from the device's staging area (which
+// grows to the largest call and is kept): parts are 256-byte aligned.
+struct Staging {
+    std::vector<size_t> off;
+    size_t total = 0;
+    size_t add(size_t n)
     {
-        if (n) hip_check(hipMalloc(&p, n), "hipMalloc");
+        off.push_back(total);
+        total += (n + 255) / 256 * 256;
+        return off.size() - 1;
     }
-    ~DevBuf()
-    {
-        if (p) (void)hipFree(p);
-    }
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
 };
 
 Device::Device(int device) : device_(device)
@@ -50,8 +48,22 @@ Device::Device(int device) : device_(device)
 
 Device::~Device()
 {
+    if (stage_) (void)hipFree(stage_);
     if (stream_) (void)hipStreamDestroy(static_cast<hipStream_t>(stream_));
     if (ctx_) mf_context_destroy(ctx_);
+}
+
+void *Device::staging(size_t bytes)
+{
+    if (bytes > stage_cap_) {
+        hip_check(hipStreamSynchronize(static_cast<hipStream_t>(stream_)), "hipStreamSynchronize");
+        if (stage_) hip_check(hipFree(stage_), "hipFree");
+        stage_ = nullptr;
+        stage_cap_ = 0;
+        hip_check(hipMalloc(&stage_, bytes), "hipMalloc");
+        stage_cap_ = bytes;
+    }
+    return stage_;
 }
 
 Device &Device::get(int device)
@@ -76,6 +88,8 @@ void Device::set_types(const std::vector<mf_chtype> &types)
 mf_chtype make_chtype(const SampleMatrix &samplecor, double pederr, const BXVector &bxs,
                       const FullSampleVector &fullpulse, const FullSampleMatrix &fullpulsecov)
 {
+    if (bxs.rows() < 1 || bxs.rows() > 10)
+        throw Error(MF_ERR_INVALID, "bxs must hold 1..10 pulses");
     mf_chtype t;
     memset(&t, 0, sizeof(t));
     t.nsamples = 10;
@@ -100,50 +114,59 @@ struct Fitted {
 };
 
 Fitted run_fits(const std::vector<mf_chtype> &types, const std::vector<uint8_t> &type_of,
-                const std::vector<double> &samples_soa, int64_t nch, bool errors,
-                int solver = MF_SOLVER_INPLACE_FNNLS)
+                const std::vector<double> &samples_soa, const std::vector<double> &pederr,
+                int64_t nch, bool errors, int solver = MF_SOLVER_INPLACE_FNNLS)
 {
     Device &dev = Device::get();
     dev.set_types(types);
     hipStream_t st = static_cast<hipStream_t>(dev.stream());
-    DevBuf ds(sizeof(double) * 10 * (size_t)nch), dt(type_of.size()),
-        dx(sizeof(double) * 10 * (size_t)nch), dc(sizeof(double) * (size_t)nch),
-        df(sizeof(int32_t) * (size_t)nch), de(errors ? sizeof(double) * 10 * (size_t)nch : 0);
-    hip_check(hipMemcpyAsync(ds.p, samples_soa.data(), sizeof(double) * 10 * (size_t)nch,
+    const size_t n = (size_t)nch;
+    Staging sg;
+    const size_t ks = sg.add(sizeof(double) * 10 * n), kt = sg.add(type_of.size()),
+                 kp = sg.add(sizeof(double) * pederr.size()), kx = sg.add(sizeof(double) * 10 * n),
+                 kc = sg.add(sizeof(double) * n), kf = sg.add(sizeof(int32_t) * n),
+                 ke = sg.add(errors ? sizeof(double) * 10 * n : 0);
+    char *base = static_cast<char *>(dev.staging(sg.total));
+    auto at = [&](size_t k) { return static_cast<void *>(base + sg.off[k]); };
+    hip_check(hipMemcpyAsync(at(ks), samples_soa.data(), sizeof(double) * 10 * n,
                              hipMemcpyHostToDevice, st), "hipMemcpyAsync");
     if (!type_of.empty())
-        hip_check(hipMemcpyAsync(dt.p, type_of.data(), type_of.size(), hipMemcpyHostToDevice, st),
+        hip_check(hipMemcpyAsync(at(kt), type_of.data(), type_of.size(), hipMemcpyHostToDevice, st),
                   "hipMemcpyAsync");
+    if (!pederr.empty())
+        hip_check(hipMemcpyAsync(at(kp), pederr.data(), sizeof(double) * pederr.size(),
+                                 hipMemcpyHostToDevice, st), "hipMemcpyAsync");
     mf_batch in;
     memset(&in, 0, sizeof(in));
     in.nch = nch;
     in.sample_dtype = MF_DTYPE_F64;
-    in.samples = ds.p;
-    in.type_of = type_of.empty() ? nullptr : static_cast<const uint8_t *>(dt.p);
+    in.samples = at(ks);
+    in.type_of = type_of.empty() ? nullptr : static_cast<const uint8_t *>(at(kt));
+    in.pederr = pederr.empty() ? nullptr : static_cast<const double *>(at(kp));
     mf_result out;
     memset(&out, 0, sizeof(out));
-    out.x = dx.p;
-    out.chi2 = dc.p;
-    out.flags = static_cast<int32_t *>(df.p);
-    out.err = de.p;
+    out.x = at(kx);
+    out.chi2 = at(kc);
+    out.flags = static_cast<int32_t *>(at(kf));
+    out.err = errors ? at(ke) : nullptr;
     mf_opts o;
     mf_default_opts(&o);
     o.compute_errors = errors ? 1 : 0;
     o.solver = solver;
     throw_status(mf_fit_batch(dev.ctx(), &in, &out, &o, st), "mf_fit_batch");
     Fitted f;
-    f.x.resize(10 * (size_t)nch);
-    f.chi2.resize((size_t)nch);
-    f.flags.resize((size_t)nch);
-    hip_check(hipMemcpyAsync(f.x.data(), dx.p, f.x.size() * sizeof(double), hipMemcpyDeviceToHost,
+    f.x.resize(10 * n);
+    f.chi2.resize(n);
+    f.flags.resize(n);
+    hip_check(hipMemcpyAsync(f.x.data(), out.x, f.x.size() * sizeof(double), hipMemcpyDeviceToHost,
                              st), "hipMemcpyAsync");
-    hip_check(hipMemcpyAsync(f.chi2.data(), dc.p, f.chi2.size() * sizeof(double),
+    hip_check(hipMemcpyAsync(f.chi2.data(), out.chi2, f.chi2.size() * sizeof(double),
                              hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-    hip_check(hipMemcpyAsync(f.flags.data(), df.p, f.flags.size() * sizeof(int32_t),
+    hip_check(hipMemcpyAsync(f.flags.data(), out.flags, f.flags.size() * sizeof(int32_t),
                              hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
     if (errors) {
-        f.err.resize(10 * (size_t)nch);
-        hip_check(hipMemcpyAsync(f.err.data(), de.p, f.err.size() * sizeof(double),
+        f.err.resize(10 * n);
+        hip_check(hipMemcpyAsync(f.err.data(), out.err, f.err.size() * sizeof(double),
                                  hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
     }
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
@@ -160,11 +183,12 @@ bool PulseChiSqSNNLS::DoFit(const SampleVector &samples, const SampleMatrix &sam
                             const FullSampleMatrix &fullpulsecov)
 {
     const int np = bxs.rows();
-    if (np != 10) throw Error(MF_ERR_INVALID, "DoFit: the engine fits 10 pulses (bxs.rows() == 10)");
-    std::vector<mf_chtype> types{make_chtype(samplecor, pederr, bxs, fullpulse, fullpulsecov)};
+    // the channel's constants with pederr 1 (the type table stays cached from
+    // one call to the next while they repeat) and its pederr per channel
+    std::vector<mf_chtype> types{make_chtype(samplecor, 1.0, bxs, fullpulse, fullpulsecov)};
     std::vector<double> s(10);
     for (int i = 0; i < 10; ++i) s[(size_t)i] = samples(i);
-    Fitted f = run_fits(types, {}, s, 1, _computeErrors, _solver);
+    Fitted f = run_fits(types, {}, s, {pederr}, 1, _computeErrors, _solver);
     // pulse matrix as built by DoFit (PulseChiSqSNNLS.cpp:50-62)
     _pulsemat = SamplePulseMatrix::Zero();
     for (int p = 0; p < np; ++p) {
@@ -188,14 +212,15 @@ std::vector<Output> doFitWrapper(std::vector<DoFitArgs> const &vargs)
     const int64_t n = (int64_t)vargs.size();
     std::vector<Output> res((size_t)n);
     if (!n) return res;
-    // channels sharing their constants share a type
+    // channels sharing their constants (all but pederr, which goes per
+    // channel) share a type
     std::vector<mf_chtype> types;
     std::vector<uint8_t> type_of((size_t)n);
+    std::vector<double> ped((size_t)n);
     for (int64_t c = 0; c < n; ++c) {
         const DoFitArgs &a = vargs[(size_t)c];
-        if (a.bxs.rows() != 10)
-            throw Error(MF_ERR_INVALID, "doFitWrapper: the engine fits 10 pulses per channel");
-        mf_chtype t = make_chtype(a.samplecor, a.pederr, a.bxs, a.fullpulse, a.fullpulsecov);
+        ped[(size_t)c] = a.pederr;
+        mf_chtype t = make_chtype(a.samplecor, 1.0, a.bxs, a.fullpulse, a.fullpulsecov);
         size_t k = 0;
         for (; k < types.size(); ++k)
             if (!memcmp(&types[k], &t, sizeof(t))) break;
@@ -209,16 +234,18 @@ std::vector<Output> doFitWrapper(std::vector<DoFitArgs> const &vargs)
     std::vector<double> s(10 * (size_t)n);
     for (int64_t c = 0; c < n; ++c)
         for (int i = 0; i < 10; ++i) s[(size_t)(i * n + c)] = vargs[(size_t)c].samples(i);
-    Fitted f = run_fits(types, types.size() > 1 ? type_of : std::vector<uint8_t>{}, s, n, false);
+    Fitted f = run_fits(types, types.size() > 1 ? type_of : std::vector<uint8_t>{}, s, ped, n,
+                        false);
     for (int64_t c = 0; c < n; ++c) {
         Output &o = res[(size_t)c];
         const DoFitArgs &a = vargs[(size_t)c];
         o.chi2 = f.chi2[(size_t)c];
         o.status = 1;
         o.BXs = a.bxs;
-        o.X = PulseVector(10);
+        const int np = a.bxs.rows();
+        o.X = PulseVector(np);
         int it = -1;
-        for (int p = 0; p < 10; ++p) {
+        for (int p = 0; p < np; ++p) {
             o.X[p] = f.x[(size_t)(p * n + c)];
             if (it < 0 && a.bxs.coeff(p) == 0) it = p;
         }
@@ -243,19 +270,23 @@ static std::vector<FixedVector> nnls_many(std::vector<NNLS_args> const &args, do
         memcpy(&b[10 * c], args[c].b.data(), sizeof(double) * 10);
         if (x0) memcpy(&xx[10 * c], (*x0)[c].data(), sizeof(double) * 10);
     }
-    DevBuf dA(sizeof(double) * A.size()), db(sizeof(double) * b.size()),
-        dx(sizeof(double) * xx.size());
-    hip_check(hipMemcpyAsync(dA.p, A.data(), sizeof(double) * A.size(), hipMemcpyHostToDevice, st),
+    Staging sg;
+    const size_t kA = sg.add(sizeof(double) * A.size()), kb = sg.add(sizeof(double) * b.size()),
+                 kx = sg.add(sizeof(double) * xx.size());
+    char *base = static_cast<char *>(dev.staging(sg.total));
+    double *dA = reinterpret_cast<double *>(base + sg.off[kA]),
+           *db = reinterpret_cast<double *>(base + sg.off[kb]),
+           *dx = reinterpret_cast<double *>(base + sg.off[kx]);
+    hip_check(hipMemcpyAsync(dA, A.data(), sizeof(double) * A.size(), hipMemcpyHostToDevice, st),
               "hipMemcpyAsync");
-    hip_check(hipMemcpyAsync(db.p, b.data(), sizeof(double) * b.size(), hipMemcpyHostToDevice, st),
+    hip_check(hipMemcpyAsync(db, b.data(), sizeof(double) * b.size(), hipMemcpyHostToDevice, st),
               "hipMemcpyAsync");
-    hip_check(hipMemcpyAsync(dx.p, xx.data(), sizeof(double) * xx.size(), hipMemcpyHostToDevice,
+    hip_check(hipMemcpyAsync(dx, xx.data(), sizeof(double) * xx.size(), hipMemcpyHostToDevice,
                              st), "hipMemcpyAsync");
     auto f = inplace ? mf_inplace_fnnls_batch : mf_fnnls_batch;
-    throw_status(f(static_cast<double *>(dA.p), static_cast<double *>(db.p),
-                   static_cast<double *>(dx.p), (int64_t)n, 10, eps, max_iter, st),
+    throw_status(f(dA, db, dx, (int64_t)n, 10, eps, max_iter, st),
                  inplace ? "mf_inplace_fnnls_batch" : "mf_fnnls_batch");
-    hip_check(hipMemcpyAsync(xx.data(), dx.p, sizeof(double) * xx.size(), hipMemcpyDeviceToHost,
+    hip_check(hipMemcpyAsync(xx.data(), dx, sizeof(double) * xx.size(), hipMemcpyDeviceToHost,
                              st), "hipMemcpyAsync");
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
     for (size_t c = 0; c < n; ++c) memcpy(x[c].data(), &xx[10 * c], sizeof(double) * 10);
@@ -301,16 +332,20 @@ static void la_call(int op, T *M, T *L, T *b, T *y, int full, int view, int i, i
     Device &dev = Device::get();
     hipStream_t st = static_cast<hipStream_t>(dev.stream());
     const size_t mb = sizeof(T) * (size_t)(full * full), vb = sizeof(T) * (size_t)full;
-    DevBuf dM(M ? mb : 0), dL(L ? mb : 0), db(b ? vb : 0), dy(y ? vb : 0);
-    if (M) hip_check(hipMemcpyAsync(dM.p, M, mb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-    if (L) hip_check(hipMemcpyAsync(dL.p, L, mb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-    if (b) hip_check(hipMemcpyAsync(db.p, b, vb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-    if (y) hip_check(hipMemcpyAsync(dy.p, y, vb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-    throw_status(mf_linalg_batch(op, dtype_of<T>(), dM.p, dL.p, db.p, dy.p, 1, full, view, i, j, st),
+    Staging sg;
+    const size_t kM = sg.add(mb), kL = sg.add(mb), kb = sg.add(vb), ky = sg.add(vb);
+    char *base = static_cast<char *>(dev.staging(sg.total));
+    void *dM = M ? base + sg.off[kM] : nullptr, *dL = L ? base + sg.off[kL] : nullptr,
+         *db = b ? base + sg.off[kb] : nullptr, *dy = y ? base + sg.off[ky] : nullptr;
+    if (M) hip_check(hipMemcpyAsync(dM, M, mb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    if (L) hip_check(hipMemcpyAsync(dL, L, mb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    if (b) hip_check(hipMemcpyAsync(db, b, vb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    if (y) hip_check(hipMemcpyAsync(dy, y, vb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    throw_status(mf_linalg_batch(op, dtype_of<T>(), dM, dL, db, dy, 1, full, view, i, j, st),
                  "mf_linalg_batch");
-    if (M) hip_check(hipMemcpyAsync(M, dM.p, mb, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-    if (L) hip_check(hipMemcpyAsync(L, dL.p, mb, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-    if (y) hip_check(hipMemcpyAsync(y, dy.p, vb, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    if (M) hip_check(hipMemcpyAsync(M, dM, mb, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    if (L) hip_check(hipMemcpyAsync(L, dL, mb, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    if (y) hip_check(hipMemcpyAsync(y, dy, vb, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
 }
 

@@ -124,17 +124,32 @@ __device__ __forceinline__ bool seg_item(const SegArg &a, int &t, int64_t &c)
     return true;
 }
 
+// The lane's copy of the launch's parameters: its channel's noise scale
+// (ped: per-channel pederr, null for the type's) and its type's pulses.
+__device__ __forceinline__ FitParams lane_params(const FitParams &prm, const DevType &ty,
+                                                 const double *__restrict__ ped, int64_t c)
+{
+    FitParams p = prm;
+    p.pmask = ty.h.pmask;
+    if (ped) {
+        const double e = ped[c];
+        p.rho = e * e * ty.inv_pederr2;
+    }
+    return p;
+}
+
 template <typename T, typename TS, int N, bool ERRORS, int XS>
 __global__ __launch_bounds__(256)
 void k_fit_plain(const DevType *__restrict__ types, SegArg seg,
-                 const TS *__restrict__ samples, int64_t nch, T *__restrict__ xo,
-                 T *__restrict__ chi2o, int32_t *__restrict__ flo, int32_t *__restrict__ ito,
-                 T *__restrict__ erro, FitParams prm)
+                 const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
+                 T *__restrict__ xo, T *__restrict__ chi2o, int32_t *__restrict__ flo,
+                 int32_t *__restrict__ ito, T *__restrict__ erro, FitParams prm0)
 {
     extern __shared__ unsigned char lds_raw[];
     int type_id;
     int64_t c;
     if (!seg_item(seg, type_id, c)) return;
+    const FitParams prm = lane_params(prm0, types[type_id], ped, c);
     const TypeMats<T> &tm = mats<T>(types[type_id]);
     T s[10];
 #pragma unroll
@@ -175,13 +190,13 @@ struct WarmIn {
 };
 template <int N>
 __device__ __forceinline__ uint32_t warm_load(const WarmIn &w, int64_t c, int64_t nch,
-                                              float (&x)[10])
+                                              float (&x)[10], uint32_t pmask)
 {
     uint32_t pm = 0;
 #pragma unroll
     for (int i = 0; i < 10; ++i) x[i] = 0.0f;
     if (w.pm) {
-        pm = w.pm[c] & ((1u << N) - 1u);
+        pm = w.pm[c] & pmask & ((1u << N) - 1u);
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             const float v = w.x[(int64_t)i * nch + c];
@@ -194,20 +209,21 @@ __device__ __forceinline__ uint32_t warm_load(const WarmIn &w, int64_t c, int64_
 template <typename TS, int N>
 __global__ __launch_bounds__(256)
 void k_fast_p1(const DevType *__restrict__ types, SegArg seg,
-               const TS *__restrict__ samples, int64_t nch,
+               const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
                float *__restrict__ xo, int32_t *__restrict__ flo, int32_t *__restrict__ ito,
-               WarmIn warm, FitParams prm)
+               WarmIn warm, FitParams prm0)
 {
     int type_id;
     int64_t c;
     if (!seg_item(seg, type_id, c)) return;
+    const FitParams prm = lane_params(prm0, types[type_id], ped, c);
     float s[10], x[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) s[i] = i < N ? (float)samples[(int64_t)i * nch + c] : 0.0f;
-    uint32_t pm = warm_load<N>(warm, c, nch, x);
+    uint32_t pm = warm_load<N>(warm, c, nch, x, prm.pmask);
     int flags = 0;
     FnnlsStats st;
-    if (!warm.pm && all_finite<float, N>(s) && prm.max_minimize > 0)
+    if (!warm.pm && all_finite<float, N>(s) && rho_ok(prm.rho) && prm.max_minimize > 0)
         fast_phase1<N, N>(types[type_id], s, prm, x, pm, st, flags);
 #pragma unroll
     for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = ((pm >> p) & 1u) ? x[p] : 0.0f;
@@ -242,21 +258,22 @@ template <typename TS, int N> struct SampleReload {
 template <typename TS, int N>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 void k_fast_fused(const DevType *__restrict__ types, SegArg seg,
-                  const TS *__restrict__ samples, int64_t nch, float *__restrict__ xo,
-                  float *__restrict__ chi2o, int32_t *__restrict__ flo,
-                  int32_t *__restrict__ ito, WarmIn warm, FitParams prm)
+                  const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
+                  float *__restrict__ xo, float *__restrict__ chi2o, int32_t *__restrict__ flo,
+                  int32_t *__restrict__ ito, WarmIn warm, FitParams prm0)
 {
     int type_id;
     int64_t c;
     if (!seg_item(seg, type_id, c)) return;
+    const FitParams prm = lane_params(prm0, types[type_id], ped, c);
     float s[10], x[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) s[i] = i < N ? (float)samples[(int64_t)i * nch + c] : 0.0f;
-    uint32_t pm = warm_load<N>(warm, c, nch, x);
+    uint32_t pm = warm_load<N>(warm, c, nch, x, prm.pmask);
     int flags = 0;
     FnnlsStats st;
     const DevType &ty = types[type_id];
-    if (!warm.pm && all_finite<float, N>(s) && prm.max_minimize > 0)
+    if (!warm.pm && all_finite<float, N>(s) && rho_ok(prm.rho) && prm.max_minimize > 0)
         fast_phase1<N, N>(ty, s, prm, x, pm, st, flags);
 #pragma unroll
     for (int p = 0; p < 10; ++p) x[p] = (p < N && ((pm >> p) & 1u)) ? x[p] : 0.0f;
@@ -279,10 +296,10 @@ constexpr int FAST_WARM_BLOCK = 256;
 template <typename TS, int N>
 __global__ __launch_bounds__(FAST_WARM_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 void k_fast_warm(const DevType *__restrict__ types, SegArg seg,
-                 const TS *__restrict__ samples, int64_t nch, float *__restrict__ xo,
-                 float *__restrict__ chi2o, int32_t *__restrict__ flo,
+                 const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
+                 float *__restrict__ xo, float *__restrict__ chi2o, int32_t *__restrict__ flo,
                  int32_t *__restrict__ ito, const uint16_t *__restrict__ frag,
-                 const float *__restrict__ pq, FitParams prm)
+                 const float *__restrict__ pq, FitParams prm0)
 {
     __shared__ mlp::MlpLds L;
     __shared__ __bf16 xs[FAST_WARM_BLOCK / 64][64][16];
@@ -303,13 +320,14 @@ void k_fast_warm(const DevType *__restrict__ types, SegArg seg,
     // every lane of the wavefront takes part in the forward pass
     uint32_t pm = mlp::mlp_warm_wave(L, xs[wv], pz[wv], pmw[wv], lane, s, N, xp);
     if (!live) return;
-    pm &= (1u << N) - 1u;
+    const DevType &ty = types[type_id];
+    const FitParams prm = lane_params(prm0, ty, ped, c);
+    pm &= prm.pmask & ((1u << N) - 1u);
     float x[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) x[i] = (i < N && ((pm >> i) & 1u)) ? fmaxf(xp[i], 1e-6f) : 0.0f;
     // a warm start goes straight to the FP64 phase (see k_fast_p1)
     FnnlsStats st;
-    const DevType &ty = types[type_id];
     NoPark park;
     FitOut<float> out;
     fast_phase2<N, N, false>(ty, s, prm, x, true, st, 0, park, out,
@@ -324,14 +342,15 @@ void k_fast_warm(const DevType *__restrict__ types, SegArg seg,
 template <typename TS, int N, bool LIVE>
 __global__ __launch_bounds__(256)
 void k_fast_p2(const DevType *__restrict__ types, SegArg seg,
-               const TS *__restrict__ samples, int64_t nch,
+               const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
                float *__restrict__ xo, float *__restrict__ chi2o, int32_t *__restrict__ flo,
-               int32_t *__restrict__ ito, int warm, FitParams prm)
+               int32_t *__restrict__ ito, int warm, FitParams prm0)
 {
     extern __shared__ unsigned char lds_raw[];
     int type_id;
     int64_t c;
     if (!seg_item(seg, type_id, c)) return;
+    const FitParams prm = lane_params(prm0, types[type_id], ped, c);
     float s[10], x[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) {
@@ -425,9 +444,9 @@ __device__ __forceinline__ void warm_fetch(const WarmIn &w, int64_t c, int64_t n
 }
 template <int N>
 __device__ __forceinline__ uint32_t warm_start_state(uint32_t pm, const float (&xr)[10],
-                                                     float (&x)[10])
+                                                     float (&x)[10], uint32_t pmask)
 {
-    pm &= (1u << N) - 1u;
+    pm &= pmask & ((1u << N) - 1u);
 #pragma unroll
     for (int i = 0; i < 10; ++i) x[i] = (i < N && ((pm >> i) & 1u)) ? fmaxf(xr[i], 1e-6f) : 0.0f;
     return pm;
@@ -441,7 +460,7 @@ __device__ __forceinline__ uint32_t warm_start_state(uint32_t pm, const float (&
 template <typename TS, int N>
 __global__ __launch_bounds__(256)
 void k_fast_p1q(const DevType *__restrict__ types, int type_id, Seg seg,
-                const TS *__restrict__ samples, int64_t nch,
+                const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
                 float *__restrict__ xo, int32_t *__restrict__ flo, int32_t *__restrict__ ito,
                 WarmIn warm, FitParams prm, unsigned *__restrict__ head, int need)
 {
@@ -450,6 +469,8 @@ void k_fast_p1q(const DevType *__restrict__ types, int type_id, Seg seg,
     const int32_t *idx = seg_list(seg);
     int64_t c = -1, cn = -1;
     float Atb[10], svn[10], xn[10];
+    float eps1 = 0.0f;   // the lane's threshold: eps times its channel's noise scale
+    double rhon = 1.0;
     uint32_t pmn = 0;
     ActiveState<float> S;
     FnnlsStats st;
@@ -466,13 +487,18 @@ void k_fast_p1q(const DevType *__restrict__ types, int type_id, Seg seg,
 #pragma unroll
                 for (int i = 0; i < 10; ++i) svn[i] = i < N ? (float)samples[(int64_t)i * nch + cn] : 0.0f;
                 warm_fetch<N>(warm, cn, nch, pmn, xn);
+                if (ped) {
+                    const double e = ped[cn];
+                    rhon = e * e * ty.inv_pederr2;
+                }
             }
         }
         // a lane without a channel takes its prefetched one
         if (c < 0 && cn >= 0) {
             c = cn;
             cn = -1;
-            S.pm = warm_start_state<N>(pmn, xn, S.x);
+            S.pm = warm_start_state<N>(pmn, xn, S.x, ty.h.pmask);
+            eps1 = (float)(prm.eps * rhon);
             S.solve = S.pm != 0;
             S.adds = 0;
             st = FnnlsStats();
@@ -485,7 +511,7 @@ void k_fast_p1q(const DevType *__restrict__ types, int type_id, Seg seg,
                 flo[c] = 0;
                 ito[c] = 0;
                 c = -1;
-            } else if (all_finite<float, N>(svn) && prm.max_minimize > 0) {
+            } else if (all_finite<float, N>(svn) && rho_ok(rhon) && prm.max_minimize > 0) {
                 const TypeMats<float> &tf = *opaque(&ty.f);
 #pragma unroll
                 for (int p = 0; p < 10; ++p) {
@@ -503,7 +529,7 @@ void k_fast_p1q(const DevType *__restrict__ types, int type_id, Seg seg,
             }
         }
         if (c >= 0 &&
-            fnnls_step<float, N, true>(ty.f.AtA0, Atb, S, (float)prm.eps, prm.eps_rel32,
+            fnnls_step<float, N, true>(ty.f.AtA0, Atb, S, eps1, prm.eps_rel32,
                                        prm.max_iter, st, prm.tau32, &ill)) {
 #pragma unroll
             for (int p = 0; p < N; ++p)
@@ -523,11 +549,13 @@ void k_fast_p1q(const DevType *__restrict__ types, int type_id, Seg seg,
 template <typename TS, int N>
 __global__ __launch_bounds__(256)
 void k_fast_p2q(const DevType *__restrict__ types, int type_id, Seg seg,
-                const TS *__restrict__ samples, int64_t nch,
+                const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
                 float *__restrict__ xo, float *__restrict__ chi2o, int32_t *__restrict__ flo,
-                int32_t *__restrict__ ito, int warm, FitParams prm, unsigned *__restrict__ head,
+                int32_t *__restrict__ ito, int warm, FitParams prm0, unsigned *__restrict__ head,
                 int need)
 {
+    FitParams prm = prm0;   // rho: the lane's current channel
+    double rhon = 1.0;
     const DevType &ty = types[type_id];
     const int64_t nitems = seg_count(seg);
     const int32_t *idx = seg_list(seg);
@@ -553,11 +581,16 @@ void k_fast_p2q(const DevType *__restrict__ types, int type_id, Seg seg,
                 }
                 fn = flo[cn];
                 itn = ito[cn];
+                if (ped) {
+                    const double e = ped[cn];
+                    rhon = e * e * ty.inv_pederr2;
+                }
             }
         }
         if (c < 0 && cn >= 0) {
             c = cn;
             cn = -1;
+            prm.rho = rhon;
             st = FnnlsStats();
             st.outer = (itn >> 8) & 255;
             st.inner = (itn >> 16) & 0xFFFF;
@@ -575,7 +608,7 @@ void k_fast_p2q(const DevType *__restrict__ types, int type_id, Seg seg,
             }
         }
         if (c >= 0 &&
-            fnnls_step<double, N, true, true>(ty.d.AtA0, Atb, S, prm.eps, prm.eps_rel64,
+            fnnls_step<double, N, true, true>(ty.d.AtA0, Atb, S, prm.eps * prm.rho, prm.eps_rel64,
                                               prm.max_iter, st, 0.0, nullptr, *opaque(&ty.tab))) {
             flags |= F_REFINED;
             if (warm && st.outer != outer0) flags |= F_WARM_REJECT;
@@ -583,7 +616,7 @@ void k_fast_p2q(const DevType *__restrict__ types, int type_id, Seg seg,
             float sv[10];
 #pragma unroll
             for (int i = 0; i < 10; ++i) sv[i] = i < N ? (float)samples[(int64_t)i * nch + c] : 0.0f;
-            const double chi2 = fast_p2_chi2<N, N>(ty, sv, S.x);
+            const double chi2 = fast_p2_chi2<N, N>(ty, sv, S.x) / prm.rho;
             const int nmin = fixed_cov_nmin(chi2, prm, flags);
             if (st.hit_max) flags |= F_FNNLS_CAP;
             if (st.llt_fail) flags |= F_LLT_FAIL;
@@ -959,6 +992,10 @@ struct mf_context {
     // live mf_graph objects whose kernels point into the prediction / the
     // partition scratch (that scratch must not move while they exist)
     int graphs_pred = 0, graphs_part = 0;
+    // every live graph of the context: their kernels also point into the
+    // type tables, so mf_set_types is refused while any exists, and
+    // mf_context_destroy detaches them (a detached graph refuses to launch)
+    std::vector<mf_graph *> graphs;
     // persistent kernels: compute units, queue heads, launch shape
     int cus = 0;
     unsigned *d_queue = nullptr;  // QUEUE_SLOTS heads
@@ -968,6 +1005,15 @@ struct mf_context {
     hipEvent_t ev[2 + 2 * 256] = {};
     int n_fit_ev = 0;
     bool mlp_timed = false;
+};
+
+// A captured fit call (mf_graph_capture); ctx is null once its context is gone.
+struct mf_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int device = 0;
+    mf_context *ctx = nullptr;
+    bool pred = false, part = false;  // scratch its kernels use
 };
 
 static void time_mark(mf_context *ctx, int slot, hipStream_t st)
@@ -1114,6 +1160,7 @@ extern "C" mf_status mf_context_destroy(mf_context *ctx)
 {
     if (!ctx) return MF_OK;
     (void)hipSetDevice(ctx->device);
+    for (mf_graph *g : ctx->graphs) g->ctx = nullptr;
     if (ctx->d_types) (void)hipFree(ctx->d_types);
     if (ctx->d_tabs) (void)hipFree(ctx->d_tabs);
     if (ctx->d_queue) (void)hipFree(ctx->d_queue);
@@ -1135,6 +1182,9 @@ extern "C" mf_status mf_set_types(mf_context *ctx, const mf_chtype *types, int32
 {
     if (!ctx || !types) return fail(MF_ERR_INVALID, "ctx or types is NULL");
     if (ntypes <= 0 || ntypes > 256) return fail(MF_ERR_INVALID, "ntypes must be in 1..256");
+    if (!ctx->graphs.empty())
+        return fail(MF_ERR_INVALID, "%d captured graph(s) of this context read its type tables; "
+                                    "destroy them before mf_set_types", (int)ctx->graphs.size());
     std::vector<DevType> h((size_t)ntypes);
     for (int t = 0; t < ntypes; ++t) {
         const char *e = expand_type(types[t], h[(size_t)t]);
@@ -1166,9 +1216,9 @@ static int64_t seg_threads(const SegArg &a) { return a.nrange ? a.wave0[a.nrange
 static int seg_type0(const SegArg &a) { return a.nrange ? a.rtype[0] : a.type_id; }
 
 template <typename T, typename TS, int N>
-static mf_status launch_fit(mf_context *ctx, const SegArg &seg, const void *samples, int64_t nch,
-                            const mf_result *out, const FitParams &prm, int block, bool errors,
-                            hipStream_t st)
+static mf_status launch_fit(mf_context *ctx, const SegArg &seg, const void *samples,
+                            const double *ped, int64_t nch, const mf_result *out,
+                            const FitParams &prm, int block, bool errors, hipStream_t st)
 {
     const int64_t nt = seg_threads(seg);
     if (nt <= 0) return MF_OK;
@@ -1182,7 +1232,7 @@ static mf_status launch_fit(mf_context *ctx, const SegArg &seg, const void *samp
     };
     auto kern = errors ? pick(std::true_type{}) : pick(std::false_type{});
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), lds, st, ctx->d_types, seg,
-                       static_cast<const TS *>(samples), nch, static_cast<T *>(out->x),
+                       static_cast<const TS *>(samples), ped, nch, static_cast<T *>(out->x),
                        static_cast<T *>(out->chi2), out->flags, out->iters,
                        static_cast<T *>(out->err), prm);
     HIP_TRY(hipGetLastError());
@@ -1193,8 +1243,9 @@ static mf_status launch_fit(mf_context *ctx, const SegArg &seg, const void *samp
 // (k_fast_warm); warm then holds no prediction.
 template <typename TS, int N, bool LIVE>
 static mf_status launch_fast(mf_context *ctx, const SegArg &seg, const void *samples,
-                             int64_t nch, const mf_result *out, const FitParams &prm, int block,
-                             WarmIn warm, bool persistent, hipStream_t st, bool mlp_here = false)
+                             const double *ped, int64_t nch, const mf_result *out,
+                             const FitParams &prm, int block, WarmIn warm, bool persistent,
+                             unsigned *queue, hipStream_t st, bool mlp_here = false)
 {
     const int64_t nt = seg_threads(seg);
     if (nt <= 0) return MF_OK;
@@ -1203,7 +1254,7 @@ static mf_status launch_fast(mf_context *ctx, const SegArg &seg, const void *sam
         if (LIVE || persistent) return fail(MF_ERR_INVALID, "in-kernel warm start: not for this launch");
         const int64_t g = (nt + FAST_WARM_BLOCK - 1) / FAST_WARM_BLOCK;
         hipLaunchKernelGGL((k_fast_warm<TS, N>), dim3((unsigned)g), dim3(FAST_WARM_BLOCK), 0, st,
-                           ctx->d_types, seg, static_cast<const TS *>(samples), nch,
+                           ctx->d_types, seg, static_cast<const TS *>(samples), ped, nch,
                            static_cast<float *>(out->x), static_cast<float *>(out->chi2),
                            out->flags, out->iters, ctx->mlp_frag, ctx->mlp_pq, prm);
         HIP_TRY(hipGetLastError());
@@ -1215,16 +1266,16 @@ static mf_status launch_fast(mf_context *ctx, const SegArg &seg, const void *sam
         const int64_t need_w = (nt + 63) / 64;
         const int64_t g1 = std::max<int64_t>(1, std::min<int64_t>(need_w, (int64_t)ctx->cus * tu.p1_wpc) / wpb);
         const int64_t g2 = std::max<int64_t>(1, std::min<int64_t>(need_w, (int64_t)ctx->cus * tu.p2_wpc) / wpb);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, ctx->d_queue, 2);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, queue, 2);
         hipLaunchKernelGGL((k_fast_p1q<TS, N>), dim3((unsigned)g1), dim3(tu.block), 0, st,
                            ctx->d_types, seg.type_id, seg.list, static_cast<const TS *>(samples),
-                           nch, static_cast<float *>(out->x), out->flags, out->iters, warm, prm,
-                           ctx->d_queue, tu.p1_need);
+                           ped, nch, static_cast<float *>(out->x), out->flags, out->iters, warm,
+                           prm, queue, tu.p1_need);
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL((k_fast_p2q<TS, N>), dim3((unsigned)g2), dim3(tu.block), 0, st,
                            ctx->d_types, seg.type_id, seg.list, static_cast<const TS *>(samples),
-                           nch, static_cast<float *>(out->x), static_cast<float *>(out->chi2),
-                           out->flags, out->iters, warm.pm ? 1 : 0, prm, ctx->d_queue + 1,
+                           ped, nch, static_cast<float *>(out->x), static_cast<float *>(out->chi2),
+                           out->flags, out->iters, warm.pm ? 1 : 0, prm, queue + 1,
                            tu.p2_need);
         HIP_TRY(hipGetLastError());
         return MF_OK;
@@ -1232,19 +1283,19 @@ static mf_status launch_fast(mf_context *ctx, const SegArg &seg, const void *sam
     const int64_t grid = (nt + block - 1) / block;
     if (!LIVE) {
         hipLaunchKernelGGL((k_fast_fused<TS, N>), dim3((unsigned)grid), dim3(block), 0, st,
-                           ctx->d_types, seg, static_cast<const TS *>(samples), nch,
+                           ctx->d_types, seg, static_cast<const TS *>(samples), ped, nch,
                            static_cast<float *>(out->x), static_cast<float *>(out->chi2),
                            out->flags, out->iters, warm, prm);
         HIP_TRY(hipGetLastError());
         return MF_OK;
     }
     hipLaunchKernelGGL((k_fast_p1<TS, N>), dim3((unsigned)grid), dim3(block), 0, st,
-                       ctx->d_types, seg, static_cast<const TS *>(samples), nch,
+                       ctx->d_types, seg, static_cast<const TS *>(samples), ped, nch,
                        static_cast<float *>(out->x), out->flags, out->iters, warm, prm);
     HIP_TRY(hipGetLastError());
     const size_t lds = LIVE ? sizeof(double) * 55 * (size_t)block : 0;
     hipLaunchKernelGGL((k_fast_p2<TS, N, LIVE>), dim3((unsigned)grid), dim3(block), lds, st,
-                       ctx->d_types, seg, static_cast<const TS *>(samples), nch,
+                       ctx->d_types, seg, static_cast<const TS *>(samples), ped, nch,
                        static_cast<float *>(out->x), static_cast<float *>(out->chi2), out->flags,
                        out->iters, warm.pm ? 1 : 0, prm);
     HIP_TRY(hipGetLastError());
@@ -1263,19 +1314,20 @@ static bool type_live(const mf_chtype &t, const FitParams &prm)
 template <typename TS>
 static mf_status dispatch_fast(mf_context *ctx, const SegArg &seg, const mf_batch *in,
                                const mf_result *out, const FitParams &prm, int block,
-                               WarmIn warm, bool persistent, hipStream_t st, bool mlp_here)
+                               WarmIn warm, bool persistent, unsigned *queue, hipStream_t st,
+                               bool mlp_here)
 {
     const mf_chtype &t = ctx->types[(size_t)seg_type0(seg)];
     const bool live = type_live(t, prm);
     if (t.nsamples == 10)
-        return live ? launch_fast<TS, 10, true>(ctx, seg, in->samples, in->nch, out, prm, block,
-                                                warm, persistent, st)
-                    : launch_fast<TS, 10, false>(ctx, seg, in->samples, in->nch, out, prm, block,
-                                                 warm, persistent, st, mlp_here);
-    return live ? launch_fast<TS, 8, true>(ctx, seg, in->samples, in->nch, out, prm, block, warm,
-                                           persistent, st)
-                : launch_fast<TS, 8, false>(ctx, seg, in->samples, in->nch, out, prm, block, warm,
-                                            persistent, st, mlp_here);
+        return live ? launch_fast<TS, 10, true>(ctx, seg, in->samples, in->pederr, in->nch, out,
+                                                prm, block, warm, persistent, queue, st)
+                    : launch_fast<TS, 10, false>(ctx, seg, in->samples, in->pederr, in->nch, out,
+                                                 prm, block, warm, persistent, queue, st, mlp_here);
+    return live ? launch_fast<TS, 8, true>(ctx, seg, in->samples, in->pederr, in->nch, out, prm,
+                                           block, warm, persistent, queue, st)
+                : launch_fast<TS, 8, false>(ctx, seg, in->samples, in->pederr, in->nch, out, prm,
+                                            block, warm, persistent, queue, st, mlp_here);
 }
 
 template <typename T>
@@ -1286,14 +1338,14 @@ static mf_status dispatch_fit(mf_context *ctx, const SegArg &seg, const mf_batch
     const int ns = ctx->types[(size_t)seg_type0(seg)].nsamples;
     const bool f64in = in->sample_dtype == MF_DTYPE_F64;
     if (ns == 10)
-        return f64in ? launch_fit<T, double, 10>(ctx, seg, in->samples, in->nch, out, prm, block,
-                                                 errors, st)
-                     : launch_fit<T, float, 10>(ctx, seg, in->samples, in->nch, out, prm, block,
-                                                errors, st);
-    return f64in ? launch_fit<T, double, 8>(ctx, seg, in->samples, in->nch, out, prm, block,
-                                            errors, st)
-                 : launch_fit<T, float, 8>(ctx, seg, in->samples, in->nch, out, prm, block,
-                                           errors, st);
+        return f64in ? launch_fit<T, double, 10>(ctx, seg, in->samples, in->pederr, in->nch, out,
+                                                 prm, block, errors, st)
+                     : launch_fit<T, float, 10>(ctx, seg, in->samples, in->pederr, in->nch, out,
+                                                prm, block, errors, st);
+    return f64in ? launch_fit<T, double, 8>(ctx, seg, in->samples, in->pederr, in->nch, out, prm,
+                                            block, errors, st)
+                 : launch_fit<T, float, 8>(ctx, seg, in->samples, in->pederr, in->nch, out, prm,
+                                           block, errors, st);
 }
 
 static SegArg seg_single(int type_id, const Seg &list)
@@ -1306,8 +1358,78 @@ static SegArg seg_single(int type_id, const Seg &list)
     return a;
 }
 
-extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_result *out,
-                                  const mf_opts *opts, void *stream)
+// Device memory a fit call uses beyond its inputs and outputs.
+struct CallScratch {
+    void *part = nullptr;       // PART_BYTES: type-partition counts and slots
+    int32_t *idx = nullptr;     // nch entries: the types' index lists
+    float *pred = nullptr;      // 11 nch floats: warm-start prediction, x [10][nch] then pm [nch]
+    unsigned *queue = nullptr;  // 2 heads: the persistent kernels' work queues
+};
+// What a call needs of it.
+struct ScratchNeed {
+    bool part = false, pred = false, queue = false;
+    int64_t nch = 0;
+};
+// Offsets of the parts in one contiguous region (256-byte aligned): queue
+// heads, partition counts, index lists, prediction.
+static size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+static size_t scratch_layout(const ScratchNeed &n, size_t off[4])
+{
+    size_t k = 0;
+    off[0] = k;
+    if (n.queue) k += 256;
+    off[1] = k;
+    if (n.part) k += align256(PART_BYTES);
+    off[2] = k;
+    if (n.part) k += align256(sizeof(int32_t) * (size_t)n.nch);
+    off[3] = k;
+    if (n.pred) k += align256(sizeof(float) * 11 * (size_t)n.nch);
+    return k;
+}
+
+// Grows the context's own scratch to what the call needs (never under a
+// graph that uses it) and returns it.
+static mf_status ctx_scratch(mf_context *ctx, const ScratchNeed &n, CallScratch &sc)
+{
+    if (n.pred && ctx->pred_cap < n.nch) {
+        if (ctx->graphs_pred)
+            return fail(MF_ERR_INVALID, "the scratch would grow under a captured graph; "
+                                        "destroy the context's graphs first");
+        if (ctx->pred) HIP_TRY(hipFree(ctx->pred));
+        ctx->pred = nullptr;
+        ctx->pred_cap = 0;
+        HIP_TRY(hipMalloc(&ctx->pred, sizeof(float) * 11 * (size_t)n.nch));
+        ctx->pred_cap = n.nch;
+    }
+    if (n.part) {
+        if ((!ctx->d_part || ctx->idx_cap < n.nch) && ctx->graphs_part)
+            return fail(MF_ERR_INVALID, "the scratch would grow under a captured graph; "
+                                        "destroy the context's graphs first");
+        if (!ctx->d_part) HIP_TRY(hipMalloc(&ctx->d_part, PART_BYTES));
+        if (ctx->idx_cap < n.nch) {
+            if (ctx->d_idx) HIP_TRY(hipFree(ctx->d_idx));
+            ctx->d_idx = nullptr;
+            ctx->idx_cap = 0;
+            HIP_TRY(hipMalloc(&ctx->d_idx, sizeof(int32_t) * (size_t)n.nch));
+            ctx->idx_cap = n.nch;
+        }
+    }
+    sc.part = ctx->d_part;
+    sc.idx = ctx->d_idx;
+    sc.pred = ctx->pred;
+    sc.queue = ctx->d_queue;
+    return MF_OK;
+}
+
+// Validates a call and fills the per-call constants; shared by the entry points.
+struct CallPlan {
+    FitParams prm;
+    int block = 0;
+    bool pers = false, mlp_here = false;
+    ScratchNeed need;
+};
+static mf_status plan_call(mf_context *ctx, const mf_batch *in, const mf_result *out,
+                           const mf_opts *opts, CallPlan &pl)
 {
     if (!ctx || !in || !out || !opts) return fail(MF_ERR_INVALID, "NULL argument");
     if (ctx->types.empty()) return fail(MF_ERR_INVALID, "no channel types set (mf_set_types)");
@@ -1340,11 +1462,10 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
         return fail(MF_ERR_UNSUPPORTED, "warm_start needs MF_PREC_FP32");
     if (opts->warm_start && !ctx->mlp_ready)
         return fail(MF_ERR_INVALID, "warm_start needs MLP parameters (mf_mlp_init / mf_mlp_set_params)");
-    int block = opts->block > 0 ? opts->block : ctx->tune.fit_block;
-    if (block % 64 || block > 256) return fail(MF_ERR_INVALID, "block must be 64, 128, 192 or 256");
-    HIP_TRY(hipSetDevice(ctx->device));
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    FitParams prm;
+    pl.block = opts->block > 0 ? opts->block : ctx->tune.fit_block;
+    if (pl.block % 64 || pl.block > 256)
+        return fail(MF_ERR_INVALID, "block must be 64, 128, 192 or 256");
+    FitParams &prm = pl.prm;
     prm.eps = opts->eps;
     prm.chi2_tol = opts->chi2_tol;
     prm.max_iter = opts->max_iter;
@@ -1360,36 +1481,63 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
         return fail(MF_ERR_UNSUPPORTED, "compute_errors needs MF_PREC_FP64");
     if (opts->compute_errors && !out->err)
         return fail(MF_ERR_INVALID, "compute_errors needs mf_result.err");
-    const int ntypes = (int)ctx->types.size();
-    WarmIn warm{nullptr, nullptr};
-    const bool pers = opts->persistent == 1 ||
-                      (opts->persistent == MF_PERSISTENT_AUTO && ctx->tune.use_persistent);
+    pl.pers = opts->persistent == 1 ||
+              (opts->persistent == MF_PERSISTENT_AUTO && ctx->tune.use_persistent);
     // A warm start without a live pulse covariance in any type runs the MLP
     // inside the fit kernel; otherwise the MLP kernel writes the prediction
     // for all channels first.
-    bool mlp_here = opts->warm_start && !pers;
-    for (const mf_chtype &t : ctx->types) mlp_here = mlp_here && !type_live(t, prm);
-    if (opts->warm_start && !mlp_here) {
-        if (ctx->pred_cap < in->nch) {
-            if (ctx->graphs_pred)
-                return fail(MF_ERR_INVALID, "the scratch would grow under a captured graph; "
-                                            "destroy the context's graphs first");
-            if (ctx->pred) HIP_TRY(hipFree(ctx->pred));
-            ctx->pred = nullptr;
-            HIP_TRY(hipMalloc(&ctx->pred, sizeof(float) * 11 * (size_t)in->nch));
-            ctx->pred_cap = in->nch;
-        }
-        float *px = ctx->pred;
-        uint32_t *ppm = reinterpret_cast<uint32_t *>(ctx->pred + 10 * (size_t)in->nch);
-        const int ns = ctx->types[0].nsamples > 8 ? 10 : 8;
+    pl.mlp_here = opts->warm_start && !pl.pers;
+    for (const mf_chtype &t : ctx->types) pl.mlp_here = pl.mlp_here && !type_live(t, prm);
+    pl.need.nch = in->nch;
+    pl.need.pred = opts->warm_start && !pl.mlp_here;
+    pl.need.part = in->type_of != nullptr && in->nranges == 0;
+    pl.need.queue = opts->precision == MF_PREC_FP32 && pl.pers;
+    return MF_OK;
+}
+
+// The fit itself, once the call is planned and its scratch is in place.
+static mf_status run_call(mf_context *ctx, const mf_batch *in, const mf_result *out,
+                          const mf_opts *opts, const CallPlan &pl, const CallScratch &sc,
+                          hipStream_t st)
+{
+    const FitParams &prm = pl.prm;
+    const int block = pl.block;
+    const bool pers = pl.pers, mlp_here = pl.mlp_here;
+    const int ntypes = (int)ctx->types.size();
+    WarmIn warm{nullptr, nullptr};
+    if (pl.need.pred) {
+        float *px = sc.pred;
+        uint32_t *ppm = reinterpret_cast<uint32_t *>(sc.pred + 10 * (size_t)in->nch);
+        // the prediction sees each channel's own sample count
+        mf::mlp::NsOfType nst;
+        memset(&nst, 0, sizeof(nst));
+        for (int t = 0; t < ntypes; ++t)
+            if (ctx->types[(size_t)t].nsamples <= 8) nst.is8[t >> 5] |= 1u << (t & 31);
+        auto warm_run = [&](int ns, int64_t c0, int64_t n, const uint8_t *tof) {
+            if (in->sample_dtype == MF_DTYPE_F64)
+                mf::mlp::launch_warm<double>(in->samples, in->nch, ns, c0, n, tof, nst,
+                                             ctx->mlp_frag, ctx->mlp_pq, px, ppm, st);
+            else
+                mf::mlp::launch_warm<float>(in->samples, in->nch, ns, c0, n, tof, nst,
+                                            ctx->mlp_frag, ctx->mlp_pq, px, ppm, st);
+        };
         time_mark(ctx, 0, st);
         ctx->mlp_timed = ctx->timing;
-        if (in->sample_dtype == MF_DTYPE_F64)
-            mf::mlp::launch_warm<double>(in->samples, in->nch, ns, in->nch, ctx->mlp_frag,
-                                         ctx->mlp_pq, px, ppm, st);
-        else
-            mf::mlp::launch_warm<float>(in->samples, in->nch, ns, in->nch, ctx->mlp_frag,
-                                        ctx->mlp_pq, px, ppm, st);
+        if (in->type_of && in->nranges == 0) {
+            warm_run(10, 0, in->nch, in->type_of);
+        } else if (in->nranges > 0) {
+            // runs of ranges with the same sample count
+            int r = 0;
+            while (r < in->nranges) {
+                const int ns = nst.samples_of((unsigned)r);
+                int e = r + 1;
+                while (e < in->nranges && nst.samples_of((unsigned)e) == ns) ++e;
+                warm_run(ns, in->ranges[r], in->ranges[e] - in->ranges[r], nullptr);
+                r = e;
+            }
+        } else {
+            warm_run(nst.samples_of(0), 0, in->nch, nullptr);
+        }
         HIP_TRY(hipGetLastError());
         time_mark(ctx, 1, st);
         warm = WarmIn{px, ppm};
@@ -1401,8 +1549,10 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
             return dispatch_fit<double>(ctx, seg, in, out, prm, block, opts->compute_errors != 0,
                                         st);
         return in->sample_dtype == MF_DTYPE_F64
-                   ? dispatch_fast<double>(ctx, seg, in, out, prm, block, warm, pers, st, mlp_here)
-                   : dispatch_fast<float>(ctx, seg, in, out, prm, block, warm, pers, st, mlp_here);
+                   ? dispatch_fast<double>(ctx, seg, in, out, prm, block, warm, pers, sc.queue, st,
+                                           mlp_here)
+                   : dispatch_fast<float>(ctx, seg, in, out, prm, block, warm, pers, sc.queue, st,
+                                          mlp_here);
     };
     ctx->n_fit_ev = 0;
     auto go = [&](const SegArg &seg) {
@@ -1415,16 +1565,18 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
     if (in->nranges > 0) {
         // type ranges: one launch per shape (samples, live pulse covariance),
         // up to MAX_RANGES ranges each, every wavefront within one range
-        std::vector<int> pending;
+        // (fixed arrays: no host allocation on this path)
+        int pending[256], rest[256], np = 0;
         for (int r = 0; r < in->nranges; ++r)
-            if (in->ranges[r + 1] > in->ranges[r]) pending.push_back(r);
-        while (!pending.empty()) {
+            if (in->ranges[r + 1] > in->ranges[r]) pending[np++] = r;
+        while (np > 0) {
             const mf_chtype &t0 = ctx->types[(size_t)pending[0]];
             const bool live0 = type_live(t0, prm);
             SegArg a;
             memset(&a, 0, sizeof(a));
-            std::vector<int> rest;
-            for (int r : pending) {
+            int nr = 0;
+            for (int k = 0; k < np; ++k) {
+                const int r = pending[k];
                 const mf_chtype &t = ctx->types[(size_t)r];
                 if (a.nrange < MAX_RANGES && t.nsamples == t0.nsamples && type_live(t, prm) == live0) {
                     a.rtype[a.nrange] = r;
@@ -1433,29 +1585,20 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
                     a.wave0[a.nrange + 1] = a.wave0[a.nrange] + (a.rcount[a.nrange] + 63) / 64;
                     a.nrange++;
                 } else {
-                    rest.push_back(r);
+                    rest[nr++] = r;
                 }
             }
             const mf_status r = go(a);
             if (r != MF_OK) return r;
-            pending.swap(rest);
+            for (int k = 0; k < nr; ++k) pending[k] = rest[k];
+            np = nr;
         }
         return MF_OK;
     }
     if (!in->type_of) return go(seg_single(0, Seg{nullptr, nullptr, nullptr, in->nch}));
     // mixed batch: partition channel indices by type on the device; each
     // type's launch reads its count and offset from device memory
-    if ((!ctx->d_part || ctx->idx_cap < in->nch) && ctx->graphs_part)
-        return fail(MF_ERR_INVALID, "the scratch would grow under a captured graph; "
-                                    "destroy the context's graphs first");
-    if (!ctx->d_part) HIP_TRY(hipMalloc(&ctx->d_part, PART_BYTES));
-    if (ctx->idx_cap < in->nch) {
-        if (ctx->d_idx) HIP_TRY(hipFree(ctx->d_idx));
-        ctx->d_idx = nullptr;
-        HIP_TRY(hipMalloc(&ctx->d_idx, sizeof(int32_t) * (size_t)in->nch));
-        ctx->idx_cap = in->nch;
-    }
-    int64_t *counts = static_cast<int64_t *>(ctx->d_part);
+    int64_t *counts = static_cast<int64_t *>(sc.part);
     int64_t *offsets = counts + 512;
     int32_t *bcount = reinterpret_cast<int32_t *>(counts + 768);
     int32_t *boff = bcount + 256 * PART_BLOCKS;
@@ -1475,33 +1618,39 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
     if (opts->precision == MF_PREC_FP64)
         hipLaunchKernelGGL(k_part_scatter<double>, dim3(nblk), dim3(256), 0, st, in->type_of,
                            in->nch, ntypes, chunk, boff,
-                           reinterpret_cast<const long long *>(offsets), ctx->d_idx,
+                           reinterpret_cast<const long long *>(offsets), sc.idx,
                            static_cast<double *>(out->x), static_cast<double *>(out->chi2),
                            out->flags, out->iters);
     else
         hipLaunchKernelGGL(k_part_scatter<float>, dim3(nblk), dim3(256), 0, st, in->type_of,
                            in->nch, ntypes, chunk, boff,
-                           reinterpret_cast<const long long *>(offsets), ctx->d_idx,
+                           reinterpret_cast<const long long *>(offsets), sc.idx,
                            static_cast<float *>(out->x), static_cast<float *>(out->chi2),
                            out->flags, out->iters);
     HIP_TRY(hipGetLastError());
     for (int t = 0; t < ntypes; ++t) {
-        mf_status r = go(seg_single(t, Seg{ctx->d_idx, counts + t, offsets + t, in->nch}));
+        mf_status r = go(seg_single(t, Seg{sc.idx, counts + t, offsets + t, in->nch}));
         if (r != MF_OK) return r;
     }
     return MF_OK;
 }
 
+extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_result *out,
+                                  const mf_opts *opts, void *stream)
+{
+    CallPlan pl;
+    mf_status r = plan_call(ctx, in, out, opts, pl);
+    if (r != MF_OK || in->nch == 0) return r;
+    HIP_TRY(hipSetDevice(ctx->device));
+    CallScratch sc;
+    r = ctx_scratch(ctx, pl.need, sc);
+    if (r != MF_OK) return r;
+    return run_call(ctx, in, out, opts, pl, sc, static_cast<hipStream_t>(stream));
+}
+
 // ---------------------------------------------------------------------------
 // graph capture of a fit call
 // ---------------------------------------------------------------------------
-struct mf_graph {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int device = 0;
-    mf_context *ctx = nullptr;
-    bool pred = false, part = false;  // scratch its kernels use
-};
 
 extern "C" mf_status mf_graph_capture(mf_context *ctx, const mf_batch *in, const mf_result *out,
                                       const mf_opts *opts, void *stream, mf_graph **graph)
@@ -1548,6 +1697,7 @@ extern "C" mf_status mf_graph_capture(mf_context *ctx, const mf_batch *in, const
     h->part = in->type_of != nullptr && in->nranges == 0;
     ctx->graphs_pred += h->pred;
     ctx->graphs_part += h->part;
+    ctx->graphs.push_back(h);
     *graph = h;
     return MF_OK;
 }
@@ -1555,6 +1705,9 @@ extern "C" mf_status mf_graph_capture(mf_context *ctx, const mf_batch *in, const
 extern "C" mf_status mf_graph_launch(mf_graph *graph, void *stream)
 {
     if (!graph) return fail(MF_ERR_INVALID, "graph is NULL");
+    if (!graph->ctx)
+        return fail(MF_ERR_INVALID, "the graph's context was destroyed: its kernels would read "
+                                    "freed tables");
     HIP_TRY(hipSetDevice(graph->device));
     HIP_TRY(hipGraphLaunch(graph->exec, static_cast<hipStream_t>(stream)));
     return MF_OK;
@@ -1567,8 +1720,14 @@ extern "C" mf_status mf_graph_destroy(mf_graph *graph)
     if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
     if (graph->graph) (void)hipGraphDestroy(graph->graph);
     if (graph->ctx) {
-        graph->ctx->graphs_pred -= graph->pred;
-        graph->ctx->graphs_part -= graph->part;
+        mf_context *c = graph->ctx;
+        c->graphs_pred -= graph->pred;
+        c->graphs_part -= graph->part;
+        for (size_t k = 0; k < c->graphs.size(); ++k)
+            if (c->graphs[k] == graph) {
+                c->graphs.erase(c->graphs.begin() + (long)k);
+                break;
+            }
     }
     delete graph;
     return MF_OK;

@@ -298,9 +298,12 @@ __global__ __launch_bounds__(INFER_BLOCK) void k_mlp_infer(
 // channel); the outputs go through LDS (pz: softplus by pulse and channel,
 // pmw: logit bits by channel) so that each lane stores its own channel's 10
 // amplitudes as 10 coalesced rows.  The per-output work is branch-free.
+// Channels c0 .. c0+n-1; a channel's sample count is ns, or, with type_of,
+// that of its type (nst).
 template <typename TS>
 __global__ __launch_bounds__(INFER_BLOCK) void k_mlp_warm(
-    const TS *__restrict__ s, int64_t ld, int ns, int64_t n, const uint16_t *__restrict__ frag,
+    const TS *__restrict__ s, int64_t ld, int ns, int64_t c0, int64_t n,
+    const uint8_t *__restrict__ type_of, NsOfType nst, const uint16_t *__restrict__ frag,
     const float *__restrict__ pq, float *__restrict__ px, uint32_t *__restrict__ ppm)
 {
     __shared__ MlpLds L;
@@ -312,12 +315,14 @@ __global__ __launch_bounds__(INFER_BLOCK) void k_mlp_warm(
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int64_t base = ((int64_t)blockIdx.x * (INFER_BLOCK / 64) + wv) * 64; base < n;
          base += (int64_t)gridDim.x * INFER_BLOCK) {
-        const int64_t c = base + lane;
-        const bool live = c < n;
+        const bool live = base + lane < n;
+        const int64_t c = c0 + base + lane;
+        int nsl = ns;
+        if (type_of && live) nsl = nst.samples_of(type_of[c]);
         float v[10], x[10];
 #pragma unroll
-        for (int i = 0; i < 10; ++i) v[i] = (live && i < ns) ? (float)s[(int64_t)i * ld + c] : 0.0f;
-        const uint32_t pm = mlp_warm_wave(L, xs[wv], pz[wv], pmw[wv], lane, v, ns, x);
+        for (int i = 0; i < 10; ++i) v[i] = (live && i < nsl) ? (float)s[(int64_t)i * ld + c] : 0.0f;
+        const uint32_t pm = mlp_warm_wave(L, xs[wv], pz[wv], pmw[wv], lane, v, nsl, x);
         if (live) {
             ppm[c] = pm;
 #pragma unroll
@@ -355,17 +360,20 @@ template void launch_infer<double>(const void *, int64_t, int, const int32_t *, 
                                    const uint16_t *, const float *, float *, int64_t, hipStream_t);
 
 template <typename TS>
-void launch_warm(const void *s, int64_t ld, int ns, int64_t n, const uint16_t *frag,
+void launch_warm(const void *s, int64_t ld, int ns, int64_t c0, int64_t n,
+                 const uint8_t *type_of, const NsOfType &nst, const uint16_t *frag,
                  const float *pq, float *px, uint32_t *ppm, hipStream_t st)
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_mlp_warm<TS>, dim3((unsigned)infer_grid(n)), dim3(INFER_BLOCK), 0, st,
-                       static_cast<const TS *>(s), ld, ns, n, frag, pq, px, ppm);
+                       static_cast<const TS *>(s), ld, ns, c0, n, type_of, nst, frag, pq, px, ppm);
 }
-template void launch_warm<float>(const void *, int64_t, int, int64_t, const uint16_t *,
-                                 const float *, float *, uint32_t *, hipStream_t);
-template void launch_warm<double>(const void *, int64_t, int, int64_t, const uint16_t *,
-                                  const float *, float *, uint32_t *, hipStream_t);
+template void launch_warm<float>(const void *, int64_t, int, int64_t, int64_t, const uint8_t *,
+                                 const NsOfType &, const uint16_t *, const float *, float *,
+                                 uint32_t *, hipStream_t);
+template void launch_warm<double>(const void *, int64_t, int, int64_t, int64_t, const uint8_t *,
+                                  const NsOfType &, const uint16_t *, const float *, float *,
+                                  uint32_t *, hipStream_t);
 
 static void gemm(bool ta, bool tb, int M, int N, int K, const float *A, int lda, const float *B,
                  int ldb, float *C, int ldc, int splits, hipStream_t st)

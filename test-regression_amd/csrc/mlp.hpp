@@ -35,10 +35,23 @@ template <typename TS>
 void launch_infer(const void *s, int64_t ld, int ns, const int32_t *idx, int64_t n,
                   const uint16_t *frag, const float *pq, float *out, int64_t ldo, hipStream_t st);
 
-// warm-start form over channels 0..n-1 of s ([10][ld]): passive-set bits
-// ppm[n] and predicted amplitudes px[10][ld]
+// Sample counts of up to 256 channel types (8 or 10): bit t of is8 set when
+// type t has 8 samples.
+struct NsOfType {
+    uint32_t is8[8];
+    __host__ __device__ int samples_of(unsigned t) const
+    {
+        return ((is8[(t >> 5) & 7u] >> (t & 31u)) & 1u) ? 8 : 10;
+    }
+};
+
+// warm-start form over channels c0..c0+n-1 of s ([10][ld]): passive-set bits
+// ppm[c] and predicted amplitudes px[i*ld + c]; each channel has ns samples,
+// or, when type_of is given (device, one byte per channel), its type's count
+// in nst
 template <typename TS>
-void launch_warm(const void *s, int64_t ld, int ns, int64_t n, const uint16_t *frag,
+void launch_warm(const void *s, int64_t ld, int ns, int64_t c0, int64_t n,
+                 const uint8_t *type_of, const NsOfType &nst, const uint16_t *frag,
                  const float *pq, float *px, uint32_t *ppm, hipStream_t st);
 
 }  // namespace mlp

@@ -12,16 +12,26 @@ namespace mf {
 
 inline int imax_(int a, int b) { return a > b ? a : b; }
 
-// Returns 0 or an error string.
+// Returns 0 or an error string.  A type of npulses < nsamples pulses (the
+// reference's dynamic BXVector, multifit_cpu/interface/EigenMatrixTypes.h:9-10)
+// occupies the first npulses of the kernel's nsamples pulse slots; the other
+// slots hold zero columns: their w is 0, below any eps > 0, so they never
+// enter the passive set and their amplitudes stay 0.
 inline const char *expand_type(const mf_chtype &t, DevType &d)
 {
-    if (!((t.nsamples == 10 && t.npulses == 10) || (t.nsamples == 8 && t.npulses == 8)))
-        return "channel type shape must be 10x10 or 8x8 (nsamples x npulses)";
+    if (t.nsamples != 10 && t.nsamples != 8) return "nsamples must be 10 or 8";
+    if (t.npulses < 1 || t.npulses > t.nsamples) return "npulses must be in 1..nsamples";
+    if (!(t.pederr > 0.0) || !(t.pederr <= 1.7976931348623157e308))
+        return "pederr must be positive and finite";
     memset(&d, 0, sizeof(d));
     const int ns = t.nsamples, np = t.npulses;
     d.h.ns = ns;
-    d.h.np = np;
+    d.h.np = ns;   // kernel pulse slots
+    d.h.npv = np;
+    d.h.pmask = (1u << np) - 1u;
     d.h.intime = -1;
+    d.inv_pederr2 = 1.0 / (t.pederr * t.pederr);
+    for (int p = np; p < 10; ++p) d.h.first[p] = ns;
     bool has_pi = false;
     for (int i = 0; i < MF_FULL * MF_FULL; ++i)
         if (t.fullpulsecov[i] != 0.0) has_pi = true;
@@ -110,6 +120,10 @@ inline void build_mask_tab(const DevType &d, MaskTab &t)
     const double(&M)[10][10] = d.d.AtA0;
     memset(&t, 0, sizeof(t));
     for (int m = 1; m < (1 << n) && m < MASK_SETS; ++m) {
+        if (m & ~d.h.pmask) {   // a slot past the type's pulses: never passive
+            t.inv[m][MASK_PACK - 1] = NAN;
+            continue;
+        }
         double L[10][10] = {{0}};
         double ratio = INFINITY;
         for (int k = 0; k < n; ++k) {

@@ -7,6 +7,7 @@ gfx950 kernels (mf_fit_batch).  Nothing here computes a fit on the CPU.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 
 import torch
@@ -40,11 +41,14 @@ def _stream_ptr(stream) -> int | None:
 
 class FitGraph:
     """A captured fit call (mf_graph_*): launch() repeats it on the same
-    buffers; result holds the outputs."""
+    buffers; result holds the outputs.  The engine keeps track of its graphs:
+    Engine.close() destroys them first, and Engine.set_types() is refused by
+    the library while any of them is open (their kernels read the tables)."""
 
     def __init__(self, eng, handle, keep):
         self._eng, self._h, self._keep = eng, handle, keep
         self.result = keep[2]
+        eng._graphs.add(self)
 
     def launch(self, stream=None):
         _abi.check(self._eng._lib.mf_graph_launch(self._h, C.c_void_p(_stream_ptr(stream))),
@@ -54,6 +58,7 @@ class FitGraph:
         if self._h:
             self._eng._lib.mf_graph_destroy(self._h)
             self._h = None
+        self._eng._graphs.discard(self)
 
     def __del__(self):
         try:
@@ -70,6 +75,7 @@ class Engine:
         if not torch.cuda.is_available():
             raise _abi.MFError("no GPU visible: the multifit engine runs on the GPU only")
         self.device = device
+        self._graphs = weakref.WeakSet()
         self._ctx = C.c_void_p()
         _abi.check(self._lib.mf_context_create(C.byref(self._ctx), device), "mf_context_create")
         self.set_types(types)
@@ -82,6 +88,8 @@ class Engine:
         self.types = list(arr)
 
     def close(self):
+        for g in list(self._graphs):
+            g.close()
         if self._ctx:
             self._lib.mf_context_destroy(self._ctx)
             self._ctx = C.c_void_p()
@@ -103,7 +111,7 @@ class Engine:
             err=torch.zeros((10, nch), dtype=dt, device=dev),
         )
 
-    def _batch_result(self, samples, type_of, opts, out, type_ranges):
+    def _batch_result(self, samples, type_of, opts, out, type_ranges, pederr=None):
         """ctypes mf_batch / mf_result for a call (and the objects to keep alive)."""
         if samples.dim() != 2 or not samples.is_cuda:
             raise ValueError("samples must be a 2-D device tensor [samples, nch]")
@@ -115,6 +123,10 @@ class Engine:
         if type_of is not None and (type_of.dtype != torch.uint8 or type_of.numel() != nch
                                     or not type_of.is_cuda):
             raise ValueError("type_of must be a uint8 device tensor with one entry per channel")
+        if pederr is not None and (pederr.dtype != torch.float64 or pederr.numel() != nch
+                                   or not pederr.is_cuda or not pederr.is_contiguous()):
+            raise ValueError("pederr must be a contiguous float64 device tensor with one entry "
+                             "per channel")
         if samples.dtype == torch.float64:
             sdt = _abi.MF_DTYPE_F64
         elif samples.dtype == torch.float32:
@@ -131,19 +143,21 @@ class Engine:
                        nranges=len(type_ranges) - 1 if type_ranges is not None else 0,
                        samples=samples.data_ptr(),
                        type_of=type_of.data_ptr() if type_of is not None else None,
-                       ranges=C.cast(rng, C.c_void_p) if rng is not None else None)
+                       ranges=C.cast(rng, C.c_void_p) if rng is not None else None,
+                       pederr=pederr.data_ptr() if pederr is not None else None)
         r = _abi.Result(x=out.x.data_ptr(), chi2=out.chi2.data_ptr(),
                         flags=out.flags.data_ptr(), iters=out.iters.data_ptr(),
                         err=out.err.data_ptr() if out.err is not None else None)
-        return b, r, (samples, type_of, out, rng)
+        return b, r, (samples, type_of, out, rng, pederr)
 
     def fit(self, samples: torch.Tensor, type_of: torch.Tensor | None = None, opts=None,
             out: FitResult | None = None, stream=None, capture: bool = False,
-            type_ranges=None):
-        """samples: [>=nsamples, nch] float64/float32 on the device (row i = sample i)."""
+            type_ranges=None, pederr: torch.Tensor | None = None):
+        """samples: [>=nsamples, nch] float64/float32 on the device (row i = sample i);
+        pederr: optional [nch] float64 device tensor, DoFit's pederr per channel."""
         if opts is None:
             opts = _abi.default_opts()
-        b, r, keep = self._batch_result(samples, type_of, opts, out, type_ranges)
+        b, r, keep = self._batch_result(samples, type_of, opts, out, type_ranges, pederr)
         if capture:
             g = C.c_void_p()
             _abi.check(self._lib.mf_graph_capture(self._ctx, C.byref(b), C.byref(r),
@@ -155,11 +169,12 @@ class Engine:
         return keep[2]
 
     def capture(self, samples: torch.Tensor, type_of: torch.Tensor | None = None, opts=None,
-                out: FitResult | None = None, stream=None, type_ranges=None) -> "FitGraph":
+                out: FitResult | None = None, stream=None, type_ranges=None,
+                pederr: torch.Tensor | None = None) -> "FitGraph":
         """mf_graph_capture: the fit of these buffers as a replayable graph
         (runs it once eagerly first).  The graph keeps the tensors alive."""
         return self.fit(samples, type_of, opts, out, stream, capture=True,
-                        type_ranges=type_ranges)
+                        type_ranges=type_ranges, pederr=pederr)
 
 
     # ---- kernel timing (HIP events on the launch stream) ----

@@ -15,14 +15,79 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <string>
 #include <vector>
 
 #include "../../include/mf_host.hpp"
 
+// --wrapper in.bin out.bin: doFitWrapper alone on a large batch in a compact
+// format.  input : int32 n, int32 ntypes; per type 100 samplecor, 19
+// fullpulse, 361 fullpulsecov, npulses, 10 bx (all as double); per channel
+// type id, pederr, 10 samples (as double).  output: per channel X[10] (0 past
+// npulses), chi2, ampl, status.
+static int wrapper_mode(const char *inp, const char *outp)
+{
+    FILE *f = fopen(inp, "rb");
+    if (!f) return 2;
+    int32_t n = 0, nt = 0;
+    if (fread(&n, 4, 1, f) != 1 || fread(&nt, 4, 1, f) != 1) return 2;
+    struct T {
+        mf::SampleMatrix sc;
+        mf::FullSampleVector fp;
+        mf::FullSampleMatrix fpc;
+        mf::BXVector bx;
+    };
+    std::vector<T> types((size_t)nt);
+    for (auto &t : types) {
+        std::vector<double> b(100 + 19 + 361 + 1 + 10);
+        if (fread(b.data(), 8, b.size(), f) != b.size()) return 2;
+        size_t k = 0;
+        for (int i = 0; i < 10; ++i)
+            for (int j = 0; j < 10; ++j) t.sc(i, j) = b[k++];
+        for (int i = 0; i < 19; ++i) t.fp(i) = b[k++];
+        for (int i = 0; i < 19; ++i)
+            for (int j = 0; j < 19; ++j) t.fpc(i, j) = b[k++];
+        const int np = (int)b[k++];
+        t.bx = mf::BXVector(np);
+        for (int i = 0; i < np; ++i) t.bx.coeffRef(i) = (char)b[k + (size_t)i];
+    }
+    std::vector<mf::DoFitArgs> args;
+    args.reserve((size_t)n);
+    for (int c = 0; c < n; ++c) {
+        double b[12];
+        if (fread(b, 8, 12, f) != 12) return 2;
+        const T &t = types[(size_t)b[0]];
+        mf::SampleVector s;
+        for (int i = 0; i < 10; ++i) s(i) = b[2 + i];
+        args.emplace_back(s, t.sc, b[1], t.bx, t.fp, t.fpc);
+    }
+    fclose(f);
+    std::vector<double> out;
+    try {
+        auto res = mf::doFitWrapper(args);
+        for (auto &r : res) {
+            for (int p = 0; p < 10; ++p) out.push_back(p < r.X.rows() ? r.X[p] : 0.0);
+            out.push_back(r.chi2);
+            out.push_back(r.ampl);
+            out.push_back(r.status);
+        }
+    } catch (const mf::Error &e) {
+        fprintf(stderr, "mf::Error %d: %s\n", e.status, e.what());
+        return 1;
+    }
+    FILE *o = fopen(outp, "wb");
+    if (!o) return 2;
+    fwrite(out.data(), 8, out.size(), o);
+    fclose(o);
+    printf("host_api_check --wrapper: %d channels, %d types\n", n, nt);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
+    if (argc == 4 && std::string(argv[1]) == "--wrapper") return wrapper_mode(argv[2], argv[3]);
     if (argc != 3) {
-        fprintf(stderr, "usage: %s in.bin out.bin\n", argv[0]);
+        fprintf(stderr, "usage: %s [--wrapper] in.bin out.bin\n", argv[0]);
         return 2;
     }
     FILE *f = fopen(argv[1], "rb");

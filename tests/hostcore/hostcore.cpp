@@ -97,9 +97,11 @@ extern "C" void hc_set_tolerances(float e32, float t32, double e64)
     eps_rel64 = e64;
 }
 
-extern "C" int hc_fit_batch(const mf_chtype *types, int ntypes, const uint8_t *type_of,
-                            const double *samples, int64_t nch, const mf_opts *o, double *x,
-                            double *chi2, int32_t *flags, int32_t *iters, double *err)
+// pederr (optional): per-channel noise sigma, as mf_batch.pederr.
+extern "C" int hc_fit_batch2(const mf_chtype *types, int ntypes, const uint8_t *type_of,
+                             const double *samples, const double *pederr, int64_t nch,
+                             const mf_opts *o, double *x, double *chi2, int32_t *flags,
+                             int32_t *iters, double *err)
 {
     g_err = o->compute_errors ? err : nullptr;
     std::vector<DevType> dt((size_t)ntypes);
@@ -121,8 +123,13 @@ extern "C" int hc_fit_batch(const mf_chtype *types, int ntypes, const uint8_t *t
     prm.eps_rel32 = eps_rel32;
     prm.tau32 = tau32;
     prm.eps_rel64 = eps_rel64;
+    const FitParams prm0 = prm;
     for (int64_t c = 0; c < nch; ++c) {
         const DevType &d = dt[type_of ? type_of[c] : 0];
+        // the kernels' lane_params
+        prm = prm0;
+        prm.pmask = d.h.pmask;
+        if (pederr) prm.rho = pederr[c] * pederr[c] * d.inv_pederr2;
         const bool f64 = o->precision == MF_PREC_FP64;
         const bool live = d.h.has_pi && o->use_pulse_cov;
         if (d.h.ns == 10) {
@@ -136,6 +143,14 @@ extern "C" int hc_fit_batch(const mf_chtype *types, int ntypes, const uint8_t *t
         }
     }
     return 0;
+}
+
+extern "C" int hc_fit_batch(const mf_chtype *types, int ntypes, const uint8_t *type_of,
+                            const double *samples, int64_t nch, const mf_opts *o, double *x,
+                            double *chi2, int32_t *flags, int32_t *iters, double *err)
+{
+    return hc_fit_batch2(types, ntypes, type_of, samples, nullptr, nch, o, x, chi2, flags, iters,
+                         err);
 }
 
 // single NNLS problems (A column-major ld 10, m x m) through fit_core's solvers

@@ -91,6 +91,8 @@ def oracle():
     L.or_dofit.restype = C.c_int
     L.or_dofit_batch.argtypes = [C.c_void_p, C.c_void_p, dp, C.c_longlong,
                                  C.POINTER(OrOpts), C.POINTER(OrResult), C.c_int]
+    L.or_dofit_batch_pederr.argtypes = [C.c_void_p, C.c_void_p, dp, C.c_void_p, C.c_longlong,
+                                        C.POINTER(OrOpts), C.POINTER(OrResult), C.c_int]
     _cache["oracle"] = L
     return L
 
@@ -113,6 +115,9 @@ def hostcore():
     L.hc_fit_batch.argtypes = [C.c_void_p, C.c_int, C.c_void_p, dp, C.c_int64, C.c_void_p,
                                dp, dp, C.c_void_p, C.c_void_p, dp]
     L.hc_fit_batch.restype = C.c_int
+    L.hc_fit_batch2.argtypes = [C.c_void_p, C.c_int, C.c_void_p, dp, C.c_void_p, C.c_int64,
+                                C.c_void_p, dp, dp, C.c_void_p, C.c_void_p, dp]
+    L.hc_fit_batch2.restype = C.c_int
     L.hc_nnls.argtypes = [dp, dp, dp, C.c_int, C.c_int, C.c_double, C.c_uint]
     L.hc_nnls.restype = C.c_int
     L.hc_mask_inverse.argtypes = [C.c_void_p, C.c_int, dp, dp]
@@ -134,15 +139,21 @@ def oracle_opts(eps=1e-11, max_iter=1000, chi2_tol=1e-3, max_minimize=50,
 
 
 def oracle_fit(chtypes, samples: np.ndarray, type_of=None, opts: OrOpts | None = None,
-               nthreads: int = 1):
-    """Runs the oracle's DoFit on every channel.  samples [10, n] float64."""
+               nthreads: int = 1, pederr: np.ndarray | None = None):
+    """Runs the oracle's DoFit on every channel.  samples [10, n] float64;
+    pederr (optional, [n] float64): DoFit's pederr per channel."""
     L = oracle()
     n = samples.shape[1]
     samples = np.ascontiguousarray(samples, dtype=np.float64)
     res = (OrResult * n)()
-    L.or_dofit_batch(C.cast(chtypes, C.c_void_p),
-                     type_of.ctypes.data_as(C.c_void_p) if type_of is not None else None,
-                     dptr(samples), n, C.byref(opts or oracle_opts()), res, nthreads)
+    if pederr is not None:
+        pederr = np.ascontiguousarray(pederr, dtype=np.float64)
+        assert pederr.shape == (n,)
+    L.or_dofit_batch_pederr(C.cast(chtypes, C.c_void_p),
+                            type_of.ctypes.data_as(C.c_void_p) if type_of is not None else None,
+                            dptr(samples),
+                            pederr.ctypes.data_as(C.c_void_p) if pederr is not None else None,
+                            n, C.byref(opts or oracle_opts()), res, nthreads)
     dt = np.dtype([
         ("x", "f8", 10), ("err", "f8", 10), ("chi2", "f8"), ("status", "i4"),
         ("flags", "i4"), ("n_minimize", "i4"), ("n_outer", "i4"), ("n_inner", "i4"),
@@ -151,8 +162,9 @@ def oracle_fit(chtypes, samples: np.ndarray, type_of=None, opts: OrOpts | None =
     return np.frombuffer(res, dtype=dt)
 
 
-def hostcore_fit(chtypes, samples: np.ndarray, type_of, opts):
-    """Runs fit_core.hpp's host build.  opts is the product's mf_opts."""
+def hostcore_fit(chtypes, samples: np.ndarray, type_of, opts, pederr=None):
+    """Runs fit_core.hpp's host build.  opts is the product's mf_opts;
+    pederr (optional) as mf_batch.pederr."""
     L = hostcore()
     n = samples.shape[1]
     samples = np.ascontiguousarray(samples, dtype=np.float64)
@@ -161,11 +173,15 @@ def hostcore_fit(chtypes, samples: np.ndarray, type_of, opts):
     flags = np.zeros(n, dtype=np.int32)
     iters = np.zeros(n, dtype=np.int32)
     err = np.zeros((10, n))
-    rc = L.hc_fit_batch(C.cast(chtypes, C.c_void_p), len(chtypes),
-                        type_of.ctypes.data_as(C.c_void_p) if type_of is not None else None,
-                        dptr(samples), n, C.cast(C.byref(opts), C.c_void_p), dptr(x),
-                        dptr(chi2), flags.ctypes.data_as(C.c_void_p),
-                        iters.ctypes.data_as(C.c_void_p), dptr(err))
+    if pederr is not None:
+        pederr = np.ascontiguousarray(pederr, dtype=np.float64)
+    rc = L.hc_fit_batch2(C.cast(chtypes, C.c_void_p), len(chtypes),
+                         type_of.ctypes.data_as(C.c_void_p) if type_of is not None else None,
+                         dptr(samples),
+                         pederr.ctypes.data_as(C.c_void_p) if pederr is not None else None,
+                         n, C.cast(C.byref(opts), C.c_void_p), dptr(x),
+                         dptr(chi2), flags.ctypes.data_as(C.c_void_p),
+                         iters.ctypes.data_as(C.c_void_p), dptr(err))
     assert rc == 0
     if opts.compute_errors:
         return x, chi2, flags, iters, err

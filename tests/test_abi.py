@@ -35,10 +35,11 @@ def test_struct_layouts(pkg):
     assert a.ChType.pederr.offset == 56
     assert C.sizeof(a.Opts) == 56
     assert a.Opts.eps.offset == 16
-    assert C.sizeof(a.Batch) == 40 and C.sizeof(a.Result) == 40
+    assert C.sizeof(a.Batch) == 48 and C.sizeof(a.Result) == 40
+    assert a.Batch.pederr.offset == 40
     assert C.sizeof(a.GenSpec) == 8 + 11 * 8
     assert C.sizeof(a.Tune) == 32
-    assert pkg._abi.lib().mf_abi_version() == 2
+    assert pkg._abi.lib().mf_abi_version() == 3
 
 
 def test_default_opts_are_the_reference_constants(pkg):
@@ -55,3 +56,27 @@ def test_product_does_not_link_test_code():
         t = py.read_text()
         for banned in ("oracle_binding", "liboracle", "or_dofit", "hostcore"):
             assert banned not in t, (py, banned)
+
+
+def test_struct_layouts_match_the_c_compiler(pkg, tmp_path):
+    """Every struct of mf_abi.h: size and field offsets as gcc lays them out."""
+    import subprocess
+    a = pkg._abi
+    structs = {"mf_chtype": a.ChType, "mf_opts": a.Opts, "mf_batch": a.Batch,
+               "mf_result": a.Result, "mf_timings": a.Timings, "mf_tune": a.Tune}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mf_abi.h"', 'int main(void){']
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, *_ in py._fields_:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0;}")
+    src = tmp_path / "lay.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "lay"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), "-o", str(exe), str(src)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                 check=True).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for f, *_ in py._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)

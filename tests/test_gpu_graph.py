@@ -70,3 +70,40 @@ def test_graph_replays_equal_eager(pkg, gpu, mode, layout):
     assert not np.any(fl & _abi.FLAGS["bad_type"])
     assert (g.result.x[5].double().cpu().numpy() > 0).mean() > 0.99
     g.close()
+
+
+def test_graph_guards_type_tables(pkg, gpu):
+    """A captured graph's kernels read the context's type tables: mf_set_types
+    is refused while the graph exists (the graph still replays the fit it
+    captured), allowed again once it is destroyed; a graph whose context is
+    destroyed refuses to launch."""
+    from test_regression_amd import engine, workloads, _abi
+    w = workloads.config("c5")
+    eng = engine.Engine(list(w.chtypes()))
+    s, t = _event(w, 1, 3)
+    x = torch.from_numpy(s).cuda()
+    tt = torch.from_numpy(t).cuda()
+    o = _abi.default_opts()
+    print("[gpu-graph] set_types under a live graph", flush=True)
+    ref = eng.fit(x, tt, o)
+    torch.cuda.synchronize()
+    g = eng.capture(x, tt, o)
+    with pytest.raises(_abi.MFError, match="graph"):
+        eng.set_types(list(w.chtypes())[::-1])
+    g.launch()
+    torch.cuda.synchronize()
+    assert torch.equal(g.result.x, ref.x) and torch.equal(g.result.flags, ref.flags)
+    g.close()
+    eng.set_types(list(w.chtypes()))
+    # a graph outliving its context: the C ABI detaches it at mf_context_destroy
+    g2 = eng.capture(x, tt, o)
+    h = g2._h
+    lib = eng._lib
+    eng._graphs.discard(g2)          # keep it open across the context's destruction
+    lib.mf_context_destroy(eng._ctx)
+    eng._ctx = type(eng._ctx)()
+    import ctypes as C
+    assert lib.mf_graph_launch(h, C.c_void_p(None)) == _abi.MF_ERR_INVALID
+    assert b"destroyed" in lib.mf_last_error()
+    assert lib.mf_graph_destroy(h) == _abi.MF_OK
+    g2._h = None
