@@ -157,7 +157,8 @@ typedef struct mf_batch {
 /* Device-resident results, structure of arrays, in the compute precision
  * (double for MF_PREC_FP64, float for MF_PREC_FP32): x[p*nch + c] amplitude
  * of pulse p (pulse order of the channel's type, the BXs() order of the
- * reference), chi2[c], flags[c] (MF_F_*), and optionally iters[c] =
+ * reference; all 10 rows are written, 0 past the type's npulses),
+ * chi2[c], flags[c] (MF_F_*), and optionally iters[c] =
  * n_minimize | n_fnnls_outer << 8 | n_fnnls_inner << 16 (NULL: skipped) and
  * err[p*nch + c] (Errors(): only the in-time pulse's entry is nonzero; needed
  * when compute_errors is set, else ignored). */

@@ -158,10 +158,10 @@ void k_fit_plain(const DevType *__restrict__ types, SegArg seg,
     FitOut<T> out;
     fit_plain<T, N, N, ERRORS, XS>(tm, s, prm, park, out, types[type_id].h.intime);
 #pragma unroll
-    for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = out.x[p];
+    for (int p = 0; p < 10; ++p) xo[(int64_t)p * nch + c] = out.x[p];
     if (ERRORS) {
 #pragma unroll
-        for (int p = 0; p < N; ++p) erro[(int64_t)p * nch + c] = out.err[p];
+        for (int p = 0; p < 10; ++p) erro[(int64_t)p * nch + c] = out.err[p];
     }
     chi2o[c] = out.chi2;
     flo[c] = out.flags;
@@ -282,7 +282,7 @@ void k_fast_fused(const DevType *__restrict__ types, SegArg seg,
     fast_phase2<N, N, false>(ty, s, prm, x, warm.pm != nullptr, st, flags & F_FP64_PIVOT, park,
                              out, SampleReload<TS, N>{samples, nch, c});
 #pragma unroll
-    for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = out.x[p];
+    for (int p = 0; p < 10; ++p) xo[(int64_t)p * nch + c] = out.x[p];
     chi2o[c] = out.chi2;
     flo[c] = out.flags;
     ito[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
@@ -333,7 +333,7 @@ void k_fast_warm(const DevType *__restrict__ types, SegArg seg,
     fast_phase2<N, N, false>(ty, s, prm, x, true, st, 0, park, out,
                              SampleReload<TS, N>{samples, nch, c});
 #pragma unroll
-    for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = out.x[p];
+    for (int p = 0; p < 10; ++p) xo[(int64_t)p * nch + c] = out.x[p];
     chi2o[c] = out.chi2;
     flo[c] = out.flags;
     ito[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
@@ -368,7 +368,7 @@ void k_fast_p2(const DevType *__restrict__ types, SegArg seg,
     fast_phase2<N, N, LIVE>(types[type_id], s, prm, x, warm != 0, st, f1 & F_FP64_PIVOT, park,
                             out);
 #pragma unroll
-    for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = out.x[p];
+    for (int p = 0; p < 10; ++p) xo[(int64_t)p * nch + c] = out.x[p];
     chi2o[c] = out.chi2;
     flo[c] = out.flags;
     ito[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
@@ -600,7 +600,7 @@ void k_fast_p2q(const DevType *__restrict__ types, int type_id, Seg seg,
             flags = fn & F_FP64_PIVOT;
             if (!fast_p2_begin<N, N>(ty, svn, prm, xn, Atb, S, flags)) {
 #pragma unroll
-                for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = 0.0f;
+                for (int p = 0; p < 10; ++p) xo[(int64_t)p * nch + c] = 0.0f;
                 chi2o[c] = 0.0f;
                 flo[c] = flags | (st.hit_max ? F_FNNLS_CAP : 0) | (st.llt_fail ? F_LLT_FAIL : 0);
                 ito[c] = (st.outer << 8) | (st.inner << 16);
@@ -621,7 +621,7 @@ void k_fast_p2q(const DevType *__restrict__ types, int type_id, Seg seg,
             if (st.hit_max) flags |= F_FNNLS_CAP;
             if (st.llt_fail) flags |= F_LLT_FAIL;
 #pragma unroll
-            for (int p = 0; p < N; ++p) xo[(int64_t)p * nch + c] = (float)S.x[p];
+            for (int p = 0; p < 10; ++p) xo[(int64_t)p * nch + c] = (float)S.x[p];
             chi2o[c] = (float)chi2;
             flo[c] = flags;
             ito[c] = nmin | (st.outer << 8) | (st.inner << 16);

@@ -209,7 +209,7 @@ __device__ __forceinline__ uint32_t mlp_warm_wave(const MlpLds &L, __bf16 (*xs)[
             const int o = 4 * g + j;
             bits |= (o < 10 && d0[j] > 0.0f) ? 1u << o : 0u;
         }
-        if (bits) atomicOr(&pmw[t], bits & valid);
+        if (bits) atomicOr(&pmw[t], bits);
         // amplitude head, o = 10..19: d0 of lanes g = 2 (o = 8 + j, j >= 2) and
         // g = 3 (o = 12 + j), d1 of lanes g = 0 (o = 16 + j); one softplus per j
         // on the value this lane holds
@@ -223,7 +223,9 @@ __device__ __forceinline__ uint32_t mlp_warm_wave(const MlpLds &L, __bf16 (*xs)[
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint32_t pm = pmw[lane];
+    // masked by this lane's own channel's sample count (the bits of channel
+    // t were set by the lanes that hold its outputs, whose ns may differ)
+    const uint32_t pm = pmw[lane] & valid;
 #pragma unroll
     for (int i = 0; i < 10; ++i) px[i] = pz[i][lane] * sc;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
