@@ -192,6 +192,23 @@ mf_status mf_default_opts(mf_opts *opts);
 mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_result *out,
                        const mf_opts *opts, void *stream);
 
+/* mf_fit_batch that allocates nothing (no counterpart in the reference): for
+ * callers that cannot allocate device memory -- code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  The same fit and the
+ * same results as mf_fit_batch; its scratch (type-partition lists and counts,
+ * warm-start prediction, persistent-queue heads) lives in a fixed device
+ * array of MF_NOALLOC_SCRATCH_BYTES that each GPU holds from the first
+ * mf_context_create on it until the process ends.  Calls whose scratch would
+ * not fit are refused with MF_ERR_INVALID: 4 bytes per channel with type_of
+ * (plus 1 MiB), 44 per channel for a warm start that runs its prediction as
+ * a separate kernel (live pulse covariance or persistent kernels); about
+ * 16.5M channels with type_of alone, 1.37M with both.  Every such call on a
+ * GPU shares that GPU's array, whatever its context: calls that overlap on
+ * one GPU (other streams, other threads, other contexts) race. */
+#define MF_NOALLOC_SCRATCH_BYTES (64u << 20)
+mf_status mf_fit_batch_noalloc(mf_context *ctx, const mf_batch *in, const mf_result *out,
+                               const mf_opts *opts, void *stream);
+
 /* A fit call captured in a HIP graph, for launch-bound repeats such as one
  * event after another (no counterpart in the reference).  mf_graph_capture
  * runs the call once eagerly (which sizes the context's scratch and checks

@@ -18,6 +18,7 @@ MF_PREC_FP64, MF_PREC_FP32 = 0, 1
 MF_SOLVER_INPLACE_FNNLS, MF_SOLVER_FNNLS, MF_SOLVER_LEGACY, MF_SOLVER_LAWSON_HANSON = 0, 1, 2, 3
 MF_DTYPE_F64, MF_DTYPE_F32 = 0, 1
 MF_MLP_NPARAM = 6164
+MF_NOALLOC_SCRATCH_BYTES = 64 << 20
 (MF_LA_CHOLESKY, MF_LA_FORWARD, MF_LA_BACKWARD, MF_LA_SWAP, MF_LA_RCADDITION,
  MF_LA_INBETWEEN) = range(6)
 
@@ -103,6 +104,8 @@ SIGNATURES = [
     ("mf_default_opts", C.c_int, [C.POINTER(Opts)]),
     ("mf_fit_batch", C.c_int, [C.c_void_p, C.POINTER(Batch), C.POINTER(Result),
                                C.POINTER(Opts), C.c_void_p]),
+    ("mf_fit_batch_noalloc", C.c_int, [C.c_void_p, C.POINTER(Batch), C.POINTER(Result),
+                                       C.POINTER(Opts), C.c_void_p]),
     ("mf_inplace_fnnls_batch", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                          C.c_int32, C.c_double, C.c_uint32, C.c_void_p]),
     ("mf_fnnls_batch", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,

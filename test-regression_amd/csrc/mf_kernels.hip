@@ -15,6 +15,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "../../include/mf_abi.h"
@@ -996,6 +997,8 @@ struct mf_context {
     // type tables, so mf_set_types is refused while any exists, and
     // mf_context_destroy detaches them (a detached graph refuses to launch)
     std::vector<mf_graph *> graphs;
+    // the device address of this GPU's scratch of mf_fit_batch_noalloc
+    void *noalloc = nullptr;
     // persistent kernels: compute units, queue heads, launch shape
     int cus = 0;
     unsigned *d_queue = nullptr;  // QUEUE_SLOTS heads
@@ -1035,6 +1038,34 @@ extern "C" mf_status mf_default_opts(mf_opts *o)
     return MF_OK;
 }
 
+// Scratch of mf_fit_batch_noalloc: a file-scope device array, one copy on
+// each GPU the library runs on, held until the process ends.  On the host
+// the name is only a stand-in; the device address of a GPU's copy comes from
+// hipGetSymbolAddress with that GPU current, once per GPU (at the first
+// mf_context_create on it), and only that address is passed to kernels.
+static __device__ __align__(256) unsigned char scratch[MF_NOALLOC_SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[MAX_DEVICES] = {};
+
+static mf_status noalloc_scratch(int device, void **ptr)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= MAX_DEVICES)
+        return fail(MF_ERR_INVALID, "device %d: the library keeps scratch for devices 0..%d",
+                    device, MAX_DEVICES - 1);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(MF_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s",
+                        hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return MF_OK;
+}
+
 extern "C" mf_status mf_context_create(mf_context **ctx, int device)
 {
     if (!ctx) return fail(MF_ERR_INVALID, "ctx is NULL");
@@ -1045,12 +1076,16 @@ extern "C" mf_status mf_context_create(mf_context **ctx, int device)
     HIP_TRY(hipSetDevice(device));
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    void *na = nullptr;
+    const mf_status sr = noalloc_scratch(device, &na);
+    if (sr != MF_OK) return sr;
     unsigned *q = nullptr;
     HIP_TRY(hipMalloc(&q, sizeof(unsigned) * QUEUE_SLOTS));
     mf_context *c = new mf_context;
     c->device = device;
     c->cus = cus > 0 ? cus : 1;
     c->d_queue = q;
+    c->noalloc = na;
     *ctx = c;
     return MF_OK;
 }
@@ -1645,6 +1680,32 @@ extern "C" mf_status mf_fit_batch(mf_context *ctx, const mf_batch *in, const mf_
     CallScratch sc;
     r = ctx_scratch(ctx, pl.need, sc);
     if (r != MF_OK) return r;
+    return run_call(ctx, in, out, opts, pl, sc, static_cast<hipStream_t>(stream));
+}
+
+extern "C" mf_status mf_fit_batch_noalloc(mf_context *ctx, const mf_batch *in,
+                                          const mf_result *out, const mf_opts *opts, void *stream)
+{
+    CallPlan pl;
+    mf_status r = plan_call(ctx, in, out, opts, pl);
+    if (r != MF_OK || in->nch == 0) return r;
+    size_t off[4];
+    const size_t need = scratch_layout(pl.need, off);
+    if (need > (size_t)MF_NOALLOC_SCRATCH_BYTES)
+        return fail(MF_ERR_INVALID,
+                    "mf_fit_batch_noalloc: this call needs %zu bytes of scratch (%lld channels%s%s), "
+                    "more than its %zu; use mf_fit_batch or fewer channels per call",
+                    need, (long long)in->nch, pl.need.part ? ", type partition" : "",
+                    pl.need.pred ? ", warm-start prediction" : "",
+                    (size_t)MF_NOALLOC_SCRATCH_BYTES);
+    if (!ctx->noalloc) return fail(MF_ERR_INVALID, "the context has no scratch address");
+    HIP_TRY(hipSetDevice(ctx->device));
+    char *base = static_cast<char *>(ctx->noalloc);
+    CallScratch sc;
+    sc.queue = pl.need.queue ? reinterpret_cast<unsigned *>(base + off[0]) : nullptr;
+    sc.part = pl.need.part ? static_cast<void *>(base + off[1]) : nullptr;
+    sc.idx = pl.need.part ? reinterpret_cast<int32_t *>(base + off[2]) : nullptr;
+    sc.pred = pl.need.pred ? reinterpret_cast<float *>(base + off[3]) : nullptr;
     return run_call(ctx, in, out, opts, pl, sc, static_cast<hipStream_t>(stream));
 }
 

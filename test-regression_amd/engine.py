@@ -168,6 +168,20 @@ class Engine:
                                           C.c_void_p(_stream_ptr(stream))), "mf_fit_batch")
         return keep[2]
 
+    def fit_noalloc(self, samples: torch.Tensor, type_of: torch.Tensor | None = None,
+                    opts=None, out: FitResult | None = None, stream=None, type_ranges=None,
+                    pederr: torch.Tensor | None = None):
+        """mf_fit_batch_noalloc: the same fit with the library's fixed per-GPU
+        scratch (no device allocation in the call; out must then be given by
+        a caller that cannot allocate)."""
+        if opts is None:
+            opts = _abi.default_opts()
+        b, r, keep = self._batch_result(samples, type_of, opts, out, type_ranges, pederr)
+        _abi.check(self._lib.mf_fit_batch_noalloc(self._ctx, C.byref(b), C.byref(r),
+                                                  C.byref(opts), C.c_void_p(_stream_ptr(stream))),
+                   "mf_fit_batch_noalloc")
+        return keep[2]
+
     def capture(self, samples: torch.Tensor, type_of: torch.Tensor | None = None, opts=None,
                 out: FitResult | None = None, stream=None, type_ranges=None,
                 pederr: torch.Tensor | None = None) -> "FitGraph":
