@@ -143,7 +143,7 @@ def cpu_baseline(ct, s, t, seconds, threads):
         b = t[:nsl] if t is not None else None
         done, t0 = 0, time.perf_counter()
         while True:
-            oracle_fit(ct, a, b, oracle_opts(), nthreads=th)
+            oracle_fit(ct, a, b, oracle_opts(record_margins=0), nthreads=th)
             done += nsl
             dt = time.perf_counter() - t0
             if dt >= budget:

@@ -212,6 +212,91 @@ static void atb_mn(const double *A, const double *b, double *Atb, int m, int n)
     }
 }
 
+/* Decision margins of a fit (or_result.m_*): how close each data-dependent
+ * decision of the reference's algorithm came to going the other way, in
+ * units of the rounding error of the quantity decided on.  That error is
+ * estimated as |operands| u (n + 1/rcond): n u for the sums themselves,
+ * u / rcond for the amplitudes they are formed from, rcond being the
+ * reciprocal condition number of the passive block that last produced
+ * them.  Another code (another operation order or precision) may decide
+ * differently only where a margin is a modest number of these units. */
+#define OR_U 1.1102230246251565e-16
+typedef struct {
+    double m_eps;    /* stop test: |max w - eps| / err(w_q)                            */
+    double m_arg;    /* argmax over the free w: (w_best - w_next) / err(w_q)           */
+    double m_pos;    /* feasibility of a solve: min_i |s_i| / err(s)                   */
+    double m_alpha;  /* which index leaves: relative gap of the two smallest step
+                        ratios / their relative error                                  */
+    double rc_last;  /* rcond of the passive block solved last (1 before any)         */
+    double rc_min;   /* smallest rcond of all passive blocks solved                   */
+    double rc_of[LD];/* inplace_fnnls: rcond of the solve that last set amplitude i    */
+    int tracked;     /* rc_of is kept (the inplace_fnnls solvers)                     */
+    int on;
+} margins;
+
+static void margins_init(margins *g)
+{
+    g->m_eps = g->m_arg = g->m_pos = g->m_alpha = INFINITY;
+    g->rc_last = 1.0;
+    g->rc_min = INFINITY;
+    for (int i = 0; i < LD; ++i) g->rc_of[i] = INFINITY;
+    g->tracked = 0;
+    g->on = 1;
+}
+static double rnd_units(double scale, int n, double rc)
+{
+    return scale * OR_U * ((double)n + 1.0 / (rc > 1e-300 ? rc : 1e-300));
+}
+static void mmin(double *m, double v)
+{
+    if (v < *m) *m = v;   /* a NaN leaves it */
+}
+
+/* The margins of the fit in progress on this thread (set by nnls_step), for
+ * the solvers that record only their passive solves' conditioning and
+ * signs (fnnls, legacy, Lawson-Hanson). */
+static _Thread_local margins *g_mg;
+
+/* 1 / (|M|_1 |M^-1|_1) of the leading k x k block of M (symmetric positive
+ * definite; 0 when its Cholesky factorization fails).  M^-1 is formed from
+ * the factor column by column: an exact-enough estimate of the conditioning
+ * for a test's rounding band. */
+static double rcond1(const double *M, int k)
+{
+    double L[LD * LD], e[LD], y[LD], z[LD];
+    if (chol_core(M, L, LD, k, 1) >= 0) return 0.0;
+    double nm = 0.0, ni = 0.0;
+    for (int j = 0; j < k; ++j) {
+        double cs = 0.0;
+        for (int i = 0; i < k; ++i) cs += fabs(M[IDX(i, j)]);
+        if (cs > nm) nm = cs;
+        for (int i = 0; i < k; ++i) e[i] = i == j ? 1.0 : 0.0;
+        or_forward_subst(L, e, y, LD, k);
+        or_backward_subst(L, y, z, LD, k);
+        double ci = 0.0;
+        for (int i = 0; i < k; ++i) ci += fabs(z[i]);
+        if (ci > ni) ni = ci;
+    }
+    const double r = 1.0 / (nm * ni);
+    return r == r ? r : 0.0;
+}
+
+/* a passive solve of the block M (leading k x k, leading dimension LD) with
+ * solution s[0..k): its conditioning, and how near an s_i came to the
+ * other sign */
+static void record_solve(margins *g, const double *M, const double *s, int k)
+{
+    if (!g || !g->on) return;
+    g->rc_last = rcond1(M, k);
+    mmin(&g->rc_min, g->rc_last);
+    double smax = 0.0, smin = INFINITY;
+    for (int i = 0; i < k; ++i) {
+        if (fabs(s[i]) > smax) smax = fabs(s[i]);
+        mmin(&smin, fabs(s[i]));
+    }
+    if (smax > 0.0) mmin(&g->m_pos, smin / rnd_units(smax, k, g->rc_last));
+}
+
 /* s = LLT(M[0:k,0:k]).solve(v[0:k]) with full refactorisation */
 static int llt_solve(const double *M, const double *v, double *s, int k)
 {
@@ -219,6 +304,7 @@ static int llt_solve(const double *M, const double *v, double *s, int k)
     int f = chol_core(M, L, LD, k, 1);
     or_forward_subst(L, v, y, LD, k);
     or_backward_subst(L, y, s, LD, k);
+    if (g_mg && g_mg->on) record_solve(g_mg, M, s, k);
     return f;
 }
 
@@ -259,10 +345,13 @@ static void stats_solve(or_nnls_stats *st, int k)
 
 /* Shared driver of inplace_fnnls and its incremental-Cholesky twin
  * (inplace_fnnls.cu:72-233 and inplace_fnnls_test_cholesky.cu:35-227): the
- * loops are identical, only the passive-set solve differs. */
+ * loops are identical, only the passive-set solve differs.  g (optional):
+ * the fit's decision margins, updated; pm_out (optional): the final
+ * passive set as a bit mask over the original indices. */
 static void inplace_fnnls_mn(const double *A, const double *b, double *x, int m,
                              int n, double eps, unsigned max_iter,
-                             or_nnls_stats *st, int incremental)
+                             or_nnls_stats *st, int incremental, margins *g,
+                             unsigned *pm_out)
 {
     double AtA[LD * LD], Atb[LD], s[LD], w[LD], xp[LD];
     double pL[LD * LD], py[LD];
@@ -290,6 +379,21 @@ static void inplace_fnnls_mn(const double *A, const double *b, double *x, int m,
         double maxw = w[np];
         for (int i = np + 1; i < n; ++i)
             if (w[i] > maxw) { maxw = w[i]; q = i; }
+        if (g && g->on) {
+            double sc = fabs(Atb[q]);
+            for (int j = 0; j < n; ++j) sc = fma(fabs(AtA[IDX(q, j)]), fabs(xp[j]), sc);
+            if (sc > 0.0) {
+                /* w_q of an index with x_q > 0 is a residual of the solve that
+                 * set x: backward stability bounds its error by n u |operands|
+                 * whatever the conditioning; otherwise the error of x enters */
+                const double ew = rnd_units(sc, n, xp[q] > 0.0 ? 1.0 : g->rc_last);
+                mmin(&g->m_eps, fabs(maxw - eps) / ew);
+                double next = -INFINITY;
+                for (int i = np; i < n; ++i)
+                    if (i != q && w[i] > next) next = w[i];
+                if (maxw >= eps && next > -INFINITY) mmin(&g->m_arg, (maxw - next) / ew);
+            }
+        }
         if (maxw < eps) break;                           /* :147 */
         sym_swap(AtA, np, q);                            /* :157-164 */
         swap_d(Atb, np, q);
@@ -319,7 +423,12 @@ static void inplace_fnnls_mn(const double *A, const double *b, double *x, int m,
                 or_backward_subst(pL, py, s, LD, np);
             }
             stats_solve(st, np);
+            if (incremental && g && g->on) record_solve(g, AtA, s, np);
 
+            if (g && g->on) {   /* every passive amplitude is set from this solve */
+                g->tracked = 1;
+                for (int i = 0; i < np; ++i) g->rc_of[perm[i]] = g->rc_last;
+            }
             double mins = s[0];                          /* :179 */
             for (int i = 1; i < np; ++i) if (s[i] < mins) mins = s[i];
             if (mins > 0.0) {
@@ -328,12 +437,18 @@ static void inplace_fnnls_mn(const double *A, const double *b, double *x, int m,
             }
             double alpha = DBL_MAX;                      /* :188-200 */
             int aidx = 0;
+            double alpha2 = DBL_MAX;
             for (int i = 0; i < np; ++i) {
                 if (s[i] <= 0.0) {
                     double ratio = xp[i] / (xp[i] - s[i]);
-                    if (ratio < alpha) { alpha = ratio; aidx = i; }
+                    if (ratio < alpha) { alpha2 = alpha; alpha = ratio; aidx = i; }
+                    else if (ratio < alpha2) alpha2 = ratio;
+                    if (xp[i] == 0.0) alpha2 = DBL_MAX;   /* ties at an exact 0: no rounding */
                 }
             }
+            if (g && g->on && alpha2 < DBL_MAX)
+                mmin(&g->m_alpha, (alpha2 - alpha) / (alpha > 0.0 ? alpha : 1.0) /
+                                      rnd_units(1.0, np, g->rc_last));
             if (alpha == DBL_MAX) {                      /* :202-205 */
                 for (int i = 0; i < np; ++i) xp[i] = s[i];
                 break;
@@ -352,18 +467,23 @@ static void inplace_fnnls_mn(const double *A, const double *b, double *x, int m,
     }
     if (st && iter == max_iter && max_iter > 0) st->hit_max = 1;
     for (int i = 0; i < n; ++i) x[perm[i]] = xp[i];      /* :232 */
+    if (pm_out) {
+        unsigned pm = 0;
+        for (int i = 0; i < np; ++i) pm |= 1u << perm[i];
+        *pm_out = pm;
+    }
 }
 
 void or_inplace_fnnls(const double *A, const double *b, double *x, int n,
                       double eps, unsigned max_iter, or_nnls_stats *st)
 {
-    inplace_fnnls_mn(A, b, x, n, n, eps, max_iter, st, 0);
+    inplace_fnnls_mn(A, b, x, n, n, eps, max_iter, st, 0, NULL, NULL);
 }
 
 void or_inplace_fnnls_chol(const double *A, const double *b, double *x, int n,
                            double eps, unsigned max_iter, or_nnls_stats *st)
 {
-    inplace_fnnls_mn(A, b, x, n, n, eps, max_iter, st, 1);
+    inplace_fnnls_mn(A, b, x, n, n, eps, max_iter, st, 1, NULL, NULL);
 }
 
 /* fnnls.cu:28-173.  Bool-mask FNNLS working on the full-length x.
@@ -525,6 +645,12 @@ static int lh_nnls_mn(const double *A, const double *b, double *x, int m, int n,
             for (int i = 0; i < np; ++i) y[perm[i]] = c[i];
             ++nls;
             stats_solve(st, np);
+            if (g_mg && g_mg->on) {   /* the normal matrix of the passive columns */
+                double Mp[LD * LD];
+                for (int a = 0; a < np; ++a)
+                    for (int b2 = 0; b2 < np; ++b2) Mp[IDX(a, b2)] = AtA[IDX(perm[a], perm[b2])];
+                record_solve(g_mg, Mp, c, np);
+            }
             int feasible = 1, rem = 0;                           /* :384-396 */
             double alpha = DBL_MAX;
             for (int i = 0; i < np; ++i) {
@@ -593,6 +719,12 @@ typedef struct {
      * pulse index) and _nP, both kept from one NNLS call to the next */
     int lperm[LD];
     int lnp;
+    /* decision margins of the main fit, and its last NNLS problem (normal
+     * equations and final passive set) for the rounding estimate */
+    margins mg;
+    double m_chi2;
+    double lAtA[LD * LD], lAtb[LD];
+    unsigned lpm;
 } fitctx;
 
 /* updateCov, PulseChiSqSNNLS.cpp:169-194 */
@@ -724,6 +856,10 @@ static void nnls_step(fitctx *c)
     for (int i = 0; i < np; ++i) xx[i] = c->x[i];         /* :258 */
     or_nnls_stats st;
     memset(&st, 0, sizeof(st));
+    unsigned lpm = 0;
+    g_mg = c->mg.on && c->o->solver != OR_SOLVER_INPLACE && c->o->solver != OR_SOLVER_INPLACE_CHOL
+               ? &c->mg : NULL;
+    if (c->o->solver == OR_SOLVER_INPLACE && c->mg.on) g_mg = &c->mg;  /* its LLT solves */
     if (c->o->solver == OR_SOLVER_LEGACY) {
         legacy_nnls(c, A, b, ns, np, &st);
         for (int i = 0; i < np; ++i) xx[i] = c->x[i];
@@ -733,15 +869,24 @@ static void nnls_step(fitctx *c)
         if (!lh_nnls_mn(A, b, xx, ns, np, c->o->eps, (int)c->o->max_iter, &st)) st.hit_max = 1;
         break;
     case OR_SOLVER_INPLACE_CHOL:
-        inplace_fnnls_mn(A, b, xx, ns, np, c->o->eps, c->o->max_iter, &st, 1);
+        inplace_fnnls_mn(A, b, xx, ns, np, c->o->eps, c->o->max_iter, &st, 1, &c->mg, &lpm);
         break;
     case OR_SOLVER_FNNLS:
         fnnls_mn(A, b, xx, ns, np, c->o->eps, c->o->max_iter, &st);
         break;
     default:
-        inplace_fnnls_mn(A, b, xx, ns, np, c->o->eps, c->o->max_iter, &st, 0);
+        inplace_fnnls_mn(A, b, xx, ns, np, c->o->eps, c->o->max_iter, &st, 0, &c->mg, &lpm);
         break;
     }
+    if (c->mg.on && st.inner > 0) {
+        /* the last call that solved sets the final amplitudes: later calls
+         * that stop at once leave them */
+        memset(c->lAtA, 0, sizeof(c->lAtA));
+        tmul_mn(A, c->lAtA, ns, np);
+        atb_mn(A, b, c->lAtb, ns, np);
+        c->lpm = lpm;
+    }
+    g_mg = NULL;
     for (int i = 0; i < np; ++i) c->x[i] = xx[i];         /* :273 */
     c->st.outer += st.outer;
     c->st.inner += st.inner;
@@ -782,6 +927,12 @@ static int minimize(fitctx *c, int *iters)
         double now = compute_chisq(c);
         double d = now - c->chisq;
         c->chisq = now;
+        if (c->mg.on) {   /* in units of the rounding of the two chi2 values */
+            const double sc = fabs(now) > 1.0 ? fabs(now) : 1.0;
+            const double m = fabs(fabs(d) - c->o->chi2_tol) /
+                             rnd_units(2.0 * sc, c->t->nsamples, c->mg.rc_last);
+            if (m < c->m_chi2) c->m_chi2 = m;
+        }
         if (fabs(d) < c->o->chi2_tol) {
             if (iters) *iters = it + 1;
             return 1;
@@ -804,6 +955,49 @@ static double approx_uncertainty(const fitctx *c, int ip)
     return 1.0 / sqrt(q);
 }
 
+/* Rounding estimate of a fit's amplitudes: the last NNLS problem's passive
+ * system (pm) solved again in long double; the largest difference from the
+ * FP64 amplitudes, relative to the largest amplitude (floor 1).  Zero when
+ * no passive set was recorded (the solvers other than inplace_fnnls). */
+static double rounding_estimate(const double *AtA, const double *Atb, unsigned pm,
+                                const double *x, int np)
+{
+    int idx[LD], k = 0;
+    for (int i = 0; i < np; ++i)
+        if ((pm >> i) & 1u) idx[k++] = i;
+    if (!k) return 0.0;
+    long double L[LD][LD], y[LD], z[LD];
+    for (int a = 0; a < k; ++a)
+        for (int b2 = 0; b2 <= a; ++b2) {
+            long double v = AtA[IDX(idx[a], idx[b2])];
+            for (int q = 0; q < b2; ++q) v -= L[a][q] * L[b2][q];
+            if (a == b2) {
+                if (!(v > 0)) return INFINITY;
+                L[a][a] = sqrtl(v);
+            } else {
+                L[a][b2] = v / L[b2][b2];
+            }
+        }
+    for (int a = 0; a < k; ++a) {
+        long double v = Atb[idx[a]];
+        for (int q = 0; q < a; ++q) v -= L[a][q] * y[q];
+        y[a] = v / L[a][a];
+    }
+    for (int a = k - 1; a >= 0; --a) {
+        long double v = y[a];
+        for (int q = a + 1; q < k; ++q) v -= L[q][a] * z[q];
+        z[a] = v / L[a][a];
+    }
+    double sc = 1.0, e = 0.0;
+    for (int i = 0; i < np; ++i)
+        if (fabs(x[i]) > sc) sc = fabs(x[i]);
+    for (int a = 0; a < k; ++a) {
+        const double d = fabs((double)(z[a] - (long double)x[idx[a]]));
+        if (d > e) e = d;
+    }
+    return e / sc;
+}
+
 /* DoFit, PulseChiSqSNNLS.cpp:21-147 */
 int or_dofit(const or_chtype *t, const double *samples, const or_opts *o,
              or_result *r)
@@ -824,8 +1018,31 @@ int or_dofit(const or_chtype *t, const double *samples, const or_opts *o,
     or_build_pulsemat(t, c.P);
     for (int i = 0; i < LD; ++i) c.lperm[i] = i;          /* legacy: _nP = 0, pulse order */
     c.lnp = 0;
+    margins_init(&c.mg);
+    c.mg.on = o->record_margins;
+    c.m_chi2 = INFINITY;
+    c.lpm = 0;
     int nmin = 0;
     int status = minimize(&c, &nmin);                    /* :65 */
+    r->m_eps = c.mg.m_eps;
+    r->m_arg = c.mg.m_arg;
+    r->m_pos = c.mg.m_pos;
+    r->m_alpha = c.mg.m_alpha;
+    r->m_chi2 = c.m_chi2;
+    /* rcond behind the final amplitudes: inplace_fnnls tracks the solve
+     * that last set each amplitude (stale ones included); the other solvers
+     * report the worst block they solved */
+    r->pivot = INFINITY;
+    if (c.mg.on) {
+        if (c.mg.tracked) {
+            for (int i = 0; i < np; ++i)
+                if (c.x[i] != 0.0) mmin(&r->pivot, c.mg.rc_of[i]);
+        } else {
+            r->pivot = c.mg.rc_min;
+        }
+    }
+    r->xerr = c.mg.on ? rounding_estimate(c.lAtA, c.lAtb, c.lpm, c.x, np) : 0.0;
+    c.mg.on = 0;
     r->n_minimize = nmin;
     for (int i = 0; i < np; ++i) r->x[i] = c.x[i];       /* :66 _ampvecmin */
     r->n_outer = c.st.outer;

@@ -100,6 +100,7 @@ typedef struct {
     int compute_errors;  /* MINOS-like in-time error (DoFit :73-146)          */
     int solver;          /* OR_SOLVER_*                                       */
     int count_flops;     /* 1: fill or_result.aflops (not free: off for timing) */
+    int record_margins;  /* 1: fill or_result.m_* and xerr (not free: off for timing) */
 } or_opts;
 
 /* status flags (bit mask) shared in meaning with the product's flags */
@@ -120,6 +121,24 @@ typedef struct {
     int n_outer;         /* FNNLS outer iterations, summed over the fit */
     int n_inner;         /* FNNLS inner solves, summed                  */
     long long ksum, k2sum, k3sum;
+    /* Decision margins of the main fit (record_margins; infinite where no
+     * such decision was taken): how near each data-dependent decision came
+     * to going the other way, in units of the rounding error of the quantity
+     * decided on, |operands| u (n + 1/rcond) with rcond the reciprocal
+     * condition number of the passive block that last produced the
+     * amplitudes involved --
+     *   m_eps   |max w - eps| over the stop tests;
+     *   m_arg   gap between the largest and next free w;
+     *   m_pos   smallest |s_i| of a passive solve;
+     *   m_alpha relative gap between the two smallest step ratios;
+     *   m_chi2  ||dchi2| - chi2_tol| over the Minimize iterations;
+     *   pivot   rcond = 1/(|M|_1 |M^-1|_1) behind the final amplitudes: the
+     *           smallest over the solves that last set a nonzero amplitude
+     *           (inplace_fnnls), or over all passive solves (other solvers);
+     *           infinite: none; 0: a block not positive definite;
+     *   xerr    that block re-solved in long double: largest change of an
+     *           amplitude relative to the largest amplitude (floor 1). */
+    double m_eps, m_arg, m_pos, m_alpha, pivot, m_chi2, xerr;
     double aflops;       /* algorithmic flops of the fit when counted: the
                             plain method's operation count, 2 per multiply-add,
                             1 per divide / square root, per call actually made:

@@ -26,7 +26,7 @@ static void run_fast(const DevType &dt, const double *samples, int64_t nch, int6
         s[i] = i < N ? (float)samples[i * nch + c] : 0.0f;
         x0[i] = 0.0f;
     }
-    RegPark<float> park;
+    RegPark<double> park;   // the live-covariance iterations park L^{-1} in FP64, as the kernel's LDS
     FitOut<float> out;
     if (use_tables)
         fit_fast<N, N, LIVE, true>(dt, s, prm, x0, 0u, false, park, out);

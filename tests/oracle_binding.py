@@ -37,14 +37,17 @@ def load_pkg():
 class OrOpts(C.Structure):
     _fields_ = [("eps", C.c_double), ("max_iter", C.c_uint), ("chi2_tol", C.c_double),
                 ("max_minimize", C.c_int), ("compute_errors", C.c_int), ("solver", C.c_int),
-                ("count_flops", C.c_int)]
+                ("count_flops", C.c_int), ("record_margins", C.c_int)]
 
 
 class OrResult(C.Structure):
     _fields_ = [("x", C.c_double * 10), ("err", C.c_double * 10), ("chi2", C.c_double),
                 ("status", C.c_int), ("flags", C.c_int), ("n_minimize", C.c_int),
                 ("n_outer", C.c_int), ("n_inner", C.c_int), ("ksum", C.c_longlong),
-                ("k2sum", C.c_longlong), ("k3sum", C.c_longlong), ("aflops", C.c_double)]
+                ("k2sum", C.c_longlong), ("k3sum", C.c_longlong),
+                ("m_eps", C.c_double), ("m_arg", C.c_double), ("m_pos", C.c_double),
+                ("m_alpha", C.c_double), ("pivot", C.c_double), ("m_chi2", C.c_double),
+                ("xerr", C.c_double), ("aflops", C.c_double)]
 
 
 class OrStats(C.Structure):
@@ -133,9 +136,13 @@ def dptr(a: np.ndarray):
 
 
 def oracle_opts(eps=1e-11, max_iter=1000, chi2_tol=1e-3, max_minimize=50,
-                compute_errors=0, solver=OR_SOLVER_INPLACE, count_flops=0) -> OrOpts:
+                compute_errors=0, solver=OR_SOLVER_INPLACE, count_flops=0,
+                record_margins=1) -> OrOpts:
+    """Oracle options; decision margins are recorded by default (the parity
+    tests use them), the timed CPU baseline turns them off."""
     return OrOpts(eps=eps, max_iter=max_iter, chi2_tol=chi2_tol, max_minimize=max_minimize,
-                  compute_errors=compute_errors, solver=solver, count_flops=count_flops)
+                  compute_errors=compute_errors, solver=solver, count_flops=count_flops,
+                  record_margins=record_margins)
 
 
 def oracle_fit(chtypes, samples: np.ndarray, type_of=None, opts: OrOpts | None = None,
@@ -157,7 +164,9 @@ def oracle_fit(chtypes, samples: np.ndarray, type_of=None, opts: OrOpts | None =
     dt = np.dtype([
         ("x", "f8", 10), ("err", "f8", 10), ("chi2", "f8"), ("status", "i4"),
         ("flags", "i4"), ("n_minimize", "i4"), ("n_outer", "i4"), ("n_inner", "i4"),
-        ("ksum", "i8"), ("k2sum", "i8"), ("k3sum", "i8"), ("aflops", "f8")], align=True)
+        ("ksum", "i8"), ("k2sum", "i8"), ("k3sum", "i8"), ("m_eps", "f8"), ("m_arg", "f8"),
+        ("m_pos", "f8"), ("m_alpha", "f8"), ("pivot", "f8"), ("m_chi2", "f8"), ("xerr", "f8"),
+        ("aflops", "f8")], align=True)
     assert dt.itemsize == C.sizeof(OrResult)
     return np.frombuffer(res, dtype=dt)
 

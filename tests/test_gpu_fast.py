@@ -1,14 +1,18 @@
 """GPU parity of the fast path (MF_PREC_FP32) against the FP64 oracle.
 
-Bar (FP32 bulk, north-star accuracy): identical active sets on >= 99.9% of
-channels; amplitudes within 1e-4 of the oracle relative to the channel's
-largest amplitude (floor 1 ADC) on >= 99.9% of channels; chi2 within 1e-4
-relative (floor 1); Minimize iteration counts equal on >= 99.9%.
+Bar (FP32 bulk, north-star accuracy), channel by channel (tests/parity.py):
+every amplitude within 1e-4 of the oracle's relative to itself (floor 1 ADC),
+chi2 within 1e-4 relative (floor 1), the same active set and Minimize
+iteration count; a channel outside that must be explained by the oracle's
+record of its conditioning or of a decision taken within rounding, and at
+most 0.1% of channels may be outside.  (The fast path reaches the optimum by
+its own route, so its FNNLS counts are not compared.)
 """
 import numpy as np
 import pytest
 import torch
 
+import parity
 from oracle_binding import oracle_fit, oracle_opts
 
 pytestmark = pytest.mark.gpu
@@ -38,22 +42,10 @@ def _fit_fast(cfg, pulse_cov, n, sample_dtype=np.float32, **kw):
     return r, ref
 
 
-def _check(r, ref):
-    x = r.x.double().cpu().numpy()
-    xr = ref["x"].T
-    scale = np.maximum(np.abs(xr).max(axis=0), 1.0)
-    err = (np.abs(x - xr) / scale).max(axis=0)
-    aset = ((x > 0) == (xr > 0)).all(axis=0)
-    chi2 = r.chi2.double().cpu().numpy()
-    cerr = np.abs(chi2 - ref["chi2"]) / np.maximum(ref["chi2"], 1.0)
-    nmin = (r.iters.cpu().numpy() & 255) == ref["n_minimize"]
-    print(f"err max {err.max():.2e} p999 {np.quantile(err, 0.999):.2e} aset {aset.mean():.5f} "
-          f"chi2 p999 {np.quantile(cerr, 0.999):.2e} nmin {nmin.mean():.5f}")
-    assert aset.mean() >= 0.999
-    assert (err <= TOL_X).mean() >= 0.999
-    assert (cerr <= TOL_CHI2).mean() >= 0.999
-    assert nmin.mean() >= 0.999
-    return err, aset
+def _check(r, ref, label=""):
+    return parity.gate(r.x.double().cpu().numpy(), r.chi2.double().cpu().numpy(), ref, TOL_X,
+                       tol_chi2=TOL_CHI2, iters=r.iters.cpu().numpy(), compare_outer=False,
+                       label=label)
 
 
 @pytest.mark.parametrize("cfg,pulse_cov,n", [
@@ -70,7 +62,7 @@ def test_fast_fit_matches_oracle(pkg, gpu, cfg, pulse_cov, n, warm):
     """Cold and warm-started (the workload's committed network) fast path
     against the oracle on the same FP32-rounded samples."""
     r, ref = _fit_fast(cfg, pulse_cov, n, warm_start=warm)
-    _check(r, ref)
+    _check(r, ref, f"fast {cfg} {pulse_cov} warm={warm}")
     fl = r.flags.cpu().numpy()
     from test_regression_amd import _abi
     assert not np.any(fl & _abi.FLAGS["nonfinite"])

@@ -1,16 +1,19 @@
 """GPU parity of mf_fit_batch (plain FP64 path) against the FP64 oracle.
 
-Bar (FP64): identical active sets, amplitudes within 1e-10 of the oracle
-relative to the channel's largest amplitude (floor 1 ADC), chi2 within 1e-10
-relative (floor 1), identical Minimize / FNNLS iteration counts on >= 99.9%
-of channels.  The oracle follows the reference's operation order; the kernel
-solves the passive system in ascending index order (masked Cholesky), so the
-two agree to rounding, not bit for bit.
+Bar (FP64), channel by channel (tests/parity.py): every amplitude within
+1e-10 of the oracle's relative to itself (floor 1 ADC), chi2 within 1e-10
+relative (floor 1), the same active set, Minimize / FNNLS iteration counts
+and flags; a channel outside that must be explained by the oracle's own
+record of its conditioning or of a decision taken within rounding, and at
+most 0.1% of channels may be outside.  The oracle follows the reference's
+operation order; the kernel solves the passive system in ascending index
+order (masked Cholesky), so the two agree to rounding, not bit for bit.
 """
 import numpy as np
 import pytest
 import torch
 
+import parity
 from oracle_binding import oracle_fit, oracle_opts
 
 pytestmark = pytest.mark.gpu
@@ -35,18 +38,11 @@ def _run(pkg, cfg, pulse_cov, n, precision=0, seed_off=0, **kw):
     return ct, s, t, r
 
 
-def _compare(ct, s, t, r, oo=None):
+def _gate(ct, s, t, r, oo=None, label="", **kw):
     ref = oracle_fit(ct, s, t, oo or oracle_opts(), nthreads=8)
-    x = r.x.double().cpu().numpy()
-    xr = ref["x"].T
-    scale = np.maximum(np.abs(xr).max(axis=0), 1.0)
-    err = (np.abs(x - xr) / scale).max(axis=0)
-    aset = ((x > 0) == (xr > 0)).all(axis=0)
-    chi2 = r.chi2.double().cpu().numpy()
-    cerr = np.abs(chi2 - ref["chi2"]) / np.maximum(ref["chi2"], 1.0)
-    it = r.iters.cpu().numpy()
-    same_it = ((it & 255) == ref["n_minimize"]) & (((it >> 8) & 255) == ref["n_outer"])
-    return err, aset, cerr, same_it, ref, r
+    return parity.gate(r.x.double().cpu().numpy(), r.chi2.double().cpu().numpy(), ref, TOL_X,
+                       tol_chi2=TOL_CHI2, iters=r.iters.cpu().numpy(),
+                       flags=r.flags.cpu().numpy(), label=label, **kw), ref
 
 
 @pytest.mark.parametrize("cfg,pulse_cov,n", [
@@ -58,32 +54,20 @@ def _compare(ct, s, t, r, oo=None):
 ])
 def test_fp64_fit_matches_oracle(pkg, gpu, cfg, pulse_cov, n):
     ct, s, t, r = _run(pkg, cfg, pulse_cov, n)
-    err, aset, cerr, same_it, ref, r = _compare(ct, s, t, r)
-    print(f"max err {err.max():.3e} aset {aset.mean():.6f} chi2 {cerr.max():.3e} "
-          f"same iters {same_it.mean():.6f}")
-    assert aset.mean() >= 0.999
-    assert np.quantile(err, 0.999) <= TOL_X
-    assert np.quantile(cerr, 0.999) <= TOL_CHI2
-    assert same_it.mean() >= 0.999
-    fl = r.flags.cpu().numpy()
-    assert (fl == ref["flags"]).mean() >= 0.999
+    _gate(ct, s, t, r, label=f"fp64 {cfg} {pulse_cov}")
 
 
 def test_fp64_fit_fnnls_solver(pkg, gpu):
     from test_regression_amd import _abi
     from oracle_binding import OR_SOLVER_FNNLS
     ct, s, t, r = _run(pkg, "c2", "zero", 5000, solver=_abi.MF_SOLVER_FNNLS)
-    err, aset, cerr, same_it, ref, r = _compare(ct, s, t, r, oracle_opts(solver=OR_SOLVER_FNNLS))
-    assert aset.mean() >= 0.999
-    assert np.quantile(err, 0.999) <= TOL_X
+    _gate(ct, s, t, r, oracle_opts(solver=OR_SOLVER_FNNLS), label="fp64 fnnls")
 
 
 def test_fp64_fit_cpu_max_iter(pkg, gpu):
     """multifit_cpu's FNNLS cap of 10 (PulseChiSqSNNLS.cpp:243) truncates some fits."""
     ct, s, t, r = _run(pkg, "c2", "live", 5000, max_iter=10)
-    err, aset, cerr, same_it, ref, r = _compare(ct, s, t, r, oracle_opts(max_iter=10))
-    assert aset.mean() >= 0.999
-    assert np.quantile(err, 0.999) <= TOL_X
+    _gate(ct, s, t, r, oracle_opts(max_iter=10), label="fp64 max_iter 10")
 
 
 def test_fp64_fit_edge_cases(pkg, gpu):
@@ -128,10 +112,8 @@ def test_fp64_fit_float_samples(pkg, gpu):
     r = eng.fit(torch.from_numpy(s32).cuda(), None, _abi.default_opts())
     torch.cuda.synchronize()
     ref = oracle_fit(ct, s32.astype(np.float64), None, oracle_opts(), nthreads=8)
-    xr = ref["x"].T
-    scale = np.maximum(np.abs(xr).max(axis=0), 1.0)
-    err = (np.abs(r.x.cpu().numpy() - xr) / scale).max(axis=0)
-    assert np.quantile(err, 0.999) <= TOL_X
+    parity.gate(r.x.cpu().numpy(), r.chi2.cpu().numpy(), ref, TOL_X, iters=r.iters.cpu().numpy(),
+                flags=r.flags.cpu().numpy(), label="fp64 float samples")
 
 
 def test_fit_rejects_bad_input(pkg, gpu):
@@ -213,17 +195,12 @@ def test_fp64_fit_minos_errors(pkg, gpu, pulse_cov):
     r = eng.fit(torch.from_numpy(s).cuda(), None, _abi.default_opts(compute_errors=1))
     torch.cuda.synchronize()
     ref = oracle_fit(ct, s, None, oracle_opts(compute_errors=1), nthreads=8)
-    e, er = r.err.cpu().numpy()[5], ref["err"][:, 5]
-    nanm = np.isnan(er)
-    assert (np.isnan(e) == nanm).mean() >= 0.999
-    ok = ~nanm & ~np.isnan(e)
-    rel = np.abs(e[ok] - er[ok]) / np.abs(er[ok])
-    assert np.quantile(rel, 0.999) <= 1e-8
+    parity.errors_gate(r.err.cpu().numpy()[5], ref["err"][:, 5], ref, 1e-8,
+                       label=f"minos {pulse_cov}")
     assert np.all(r.err.cpu().numpy()[[0, 1, 2, 3, 4, 6, 7, 8, 9]] == 0)
     # the amplitudes are those of the fit without errors
-    xr = ref["x"].T
-    scale = np.maximum(np.abs(xr).max(axis=0), 1.0)
-    assert np.quantile((np.abs(r.x.cpu().numpy() - xr) / scale).max(axis=0), 0.999) <= 1e-10
+    parity.gate(r.x.cpu().numpy(), r.chi2.cpu().numpy(), ref, TOL_X,
+                label=f"minos fit {pulse_cov}")
 
 
 @pytest.mark.parametrize("mode", ["plain", "fast", "warm"])

@@ -14,6 +14,8 @@ import ctypes as C
 
 import numpy as np
 import pytest
+
+import parity
 import torch
 
 from oracle_binding import OR_SOLVER_LH, OrStats, dptr, oracle, oracle_fit, oracle_opts
@@ -73,16 +75,6 @@ def test_lh_fit_matches_oracle(pkg, gpu, cfg, pulse_cov, n):
                 _abi.default_opts(solver=_abi.MF_SOLVER_LAWSON_HANSON))
     torch.cuda.synchronize()
     ref = oracle_fit(ct, s, t, oracle_opts(solver=OR_SOLVER_LH), nthreads=8)
-    x = r.x.double().cpu().numpy()
-    xr = ref["x"].T
-    scale = np.maximum(np.abs(xr).max(axis=0), 1.0)
-    err = (np.abs(x - xr) / scale).max(axis=0)
-    it = r.iters.cpu().numpy()
-    same_it = ((it & 255) == ref["n_minimize"]) & (((it >> 8) & 255) == ref["n_outer"]) & \
-              ((it >> 16) == ref["n_inner"])
-    fl = r.flags.cpu().numpy()
-    print(f"max err {err.max():.3e} same iters {same_it.mean():.6f}")
-    assert ((x > 0) == (xr > 0)).all(axis=0).mean() >= 0.999
-    assert np.quantile(err, 0.999) <= 1e-10
-    assert same_it.mean() >= 0.999
-    assert (fl == ref["flags"]).mean() >= 0.999
+    parity.gate(r.x.double().cpu().numpy(), r.chi2.double().cpu().numpy(), ref, 1e-10,
+                iters=r.iters.cpu().numpy(), flags=r.flags.cpu().numpy(), compare_inner=True,
+                label=f"lawson-hanson {cfg} {pulse_cov}")

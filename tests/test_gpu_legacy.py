@@ -10,6 +10,8 @@ same NaN pattern.
 """
 import numpy as np
 import pytest
+
+import parity
 import torch
 
 from oracle_binding import OR_SOLVER_LEGACY, oracle_fit, oracle_opts
@@ -41,24 +43,9 @@ def _fit(cfg, pulse_cov, n, **kw):
 def test_legacy_fit_matches_oracle(pkg, gpu, cfg, pulse_cov, n):
     ct, s, t, r = _fit(cfg, pulse_cov, n)
     ref = oracle_fit(ct, s, t, oracle_opts(solver=OR_SOLVER_LEGACY), nthreads=8)
-    x = r.x.double().cpu().numpy()
-    xr = ref["x"].T
-    scale = np.maximum(np.abs(xr).max(axis=0), 1.0)
-    err = (np.abs(x - xr) / scale).max(axis=0)
-    aset = ((x > 0) == (xr > 0)).all(axis=0)
-    chi2 = r.chi2.double().cpu().numpy()
-    cerr = np.abs(chi2 - ref["chi2"]) / np.maximum(ref["chi2"], 1.0)
-    it = r.iters.cpu().numpy()
-    same_it = ((it & 255) == ref["n_minimize"]) & (((it >> 8) & 255) == ref["n_outer"]) & \
-              ((it >> 16) == ref["n_inner"])
-    fl = r.flags.cpu().numpy()
-    print(f"max err {err.max():.3e} aset {aset.mean():.6f} chi2 {cerr.max():.3e} "
-          f"same iters {same_it.mean():.6f} same flags {(fl == ref['flags']).mean():.6f}")
-    assert aset.mean() >= 0.999
-    assert np.quantile(err, 0.999) <= 1e-10
-    assert np.quantile(cerr, 0.999) <= 1e-10
-    assert same_it.mean() >= 0.999
-    assert (fl == ref["flags"]).mean() >= 0.999
+    parity.gate(r.x.double().cpu().numpy(), r.chi2.double().cpu().numpy(), ref, 1e-10,
+                iters=r.iters.cpu().numpy(), flags=r.flags.cpu().numpy(), compare_inner=True,
+                label=f"legacy {cfg} {pulse_cov}")
 
 
 @pytest.mark.parametrize("pulse_cov", ["zero", "live"])
@@ -66,13 +53,8 @@ def test_legacy_minos_errors_match_oracle(pkg, gpu, pulse_cov):
     ct, s, t, r = _fit("c2", pulse_cov, 5000, compute_errors=1)
     ref = oracle_fit(ct, s, t, oracle_opts(solver=OR_SOLVER_LEGACY, compute_errors=1),
                      nthreads=8)
-    e, er = r.err.double().cpu().numpy()[5], ref["err"][:, 5]
-    nan_same = np.isnan(e) == np.isnan(er)
-    ok = ~np.isnan(er) & ~np.isnan(e)
-    rel = np.abs(e[ok] - er[ok]) / np.abs(er[ok])
-    print(f"nan same {nan_same.mean():.6f} max rel {rel.max():.3e}")
-    assert nan_same.mean() >= 0.999
-    assert np.quantile(rel, 0.999) <= 1e-9
+    parity.errors_gate(r.err.double().cpu().numpy()[5], ref["err"][:, 5], ref, 1e-9,
+                       label=f"legacy minos {pulse_cov}")
 
 
 def test_legacy_needs_fp64(pkg, gpu):

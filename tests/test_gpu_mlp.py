@@ -12,6 +12,8 @@
 """
 import numpy as np
 import pytest
+
+import parity
 import torch
 
 import mlp_ref as M
@@ -120,12 +122,8 @@ def test_mlp_training_and_warm_start(pkg, gpu):
     torch.cuda.synchronize()
     ref = oracle_fit(ct, sh.astype(np.float32).astype(np.float64), None, oracle_opts(),
                      nthreads=8)
-    xr = ref["x"].T
-    scale = np.maximum(np.abs(xr).max(axis=0), 1.0)
-    xw = warm.x.double().cpu().numpy()
-    err = (np.abs(xw - xr) / scale).max(axis=0)
-    assert ((xw > 0) == (xr > 0)).all(axis=0).mean() >= 0.999
-    assert (err <= 1e-4).mean() >= 0.999
+    parity.gate(warm.x.double().cpu().numpy(), warm.chi2.double().cpu().numpy(), ref, 1e-4,
+                iters=warm.iters.cpu().numpy(), compare_outer=False, label="freshly trained warm")
     # ablation: the prediction replaces most of the active-set solves
     inner_c = cold.n_inner().float().mean().item()
     inner_w = warm.n_inner().float().mean().item()
@@ -153,12 +151,14 @@ def test_committed_networks_on_held_out_channels(pkg, gpu):
         ev = warmstart.evaluate(eng, s32, xref)
         r = eng.fit(s32, tt, _abi.default_opts(precision=_abi.MF_PREC_FP32, warm_start=1))
         torch.cuda.synchronize()
-        xw = r.x.double().cpu().numpy()
-        xr = xref.double().cpu().numpy()
         res[(cfg, wts)] = (ev["exact_active_set"], r.n_inner().float().mean().item())
         print(f"[gpu-mlp] {cfg} with the {wts} network: exact {ev['exact_active_set']:.3f}, "
               f"solves {res[(cfg, wts)][1]:.2f}", flush=True)
-        assert ((xw > 0) == (xr > 0)).all(axis=0).mean() >= 0.999
+        ref = oracle_fit(w.chtypes(), s.astype(np.float32).astype(np.float64), t, oracle_opts(),
+                         nthreads=8)
+        parity.gate(r.x.double().cpu().numpy(), r.chi2.double().cpu().numpy(), ref, 1e-4,
+                    iters=r.iters.cpu().numpy(), compare_outer=False,
+                    label=f"{cfg} warm with the {wts} network")
     assert res[("c2", "c2")][0] > 0.7 and res[("c3", "c3")][0] > 0.7
     assert res[("c3", "c3")][0] > res[("c3", "c2")][0] + 0.15
     assert res[("c3", "c3")][1] < res[("c3", "c2")][1]
