@@ -339,8 +339,10 @@ def main():
                     help="event stream (c5): distinct events resident in HBM, cycled")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for counter runs (rocprofv3 --pmc): a 256 MiB device copy, the "
-                         "calibration of the byte counters, then --steps steps of the warm and "
-                         "cold fast paths; no timing, no JSON")
+                         "calibration of the byte counters, then --steps steps of each path "
+                         "of --probe-paths; no timing, no JSON")
+    ap.add_argument("--probe-paths", default="warm,fast",
+                    help="paths of --traffic-probe, in order: any of plain, fast, warm")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the multifit path has no cross-GPU exchange "
@@ -384,11 +386,14 @@ def main():
         b = torch.empty_like(a)
         b.copy_(a)
         torch.cuda.synchronize()
-        for name, o in (("warm", O(precision=_abi.MF_PREC_FP32, warm_start=1)),
-                        ("fast", O(precision=_abi.MF_PREC_FP32))):
-            out = eng.alloc_result(nch, _abi.MF_PREC_FP32)
+        probe = {"warm": (s32, O(precision=_abi.MF_PREC_FP32, warm_start=1), _abi.MF_PREC_FP32),
+                 "fast": (s32, O(precision=_abi.MF_PREC_FP32), _abi.MF_PREC_FP32),
+                 "plain": (s64, O(precision=_abi.MF_PREC_FP64), _abi.MF_PREC_FP64)}
+        for name in args.probe_paths.split(","):
+            x_in, o, prec = probe[name]
+            out = eng.alloc_result(nch, prec)
             for _ in range(args.steps):
-                eng.fit(s32, tt, o, out)
+                eng.fit(x_in, tt, o, out)
             torch.cuda.synchronize()
             log(f"traffic probe: {args.steps} steps of {name}")
         return

@@ -21,6 +21,8 @@
  *                              (nnls/interface/inplace_fnnls.h:9-14, nnls/interface/kernel_wrapper.h:7-12)
  *   mf_fnnls_batch ........... fnnls / fnnls_kernel / fnnls_wrapper
  *                              (nnls/interface/fnnls.h:9-14, nnls/interface/kernel_wrapper.h:14-24)
+ *   mf_inplace_fnnls_chol_batch  inplace_fnnls_test_cholesky
+ *                              (nnls/interface/inplace_fnnls_test_cholesky.h:9-14)
  *   mf_lh_nnls_batch ......... Eigen::NNLS<FixedMatrix>::solve (nnls/interface/eigen_nnls.h:51-414)
  *   mf_linalg_batch .......... cholesky_decomp, solve_forward_substitution,
  *                              solve_backward_substitution, swap_row_column,
@@ -88,10 +90,15 @@ enum {
                                      over from one Minimize iteration to the next
                                      (and into the MINOS refits); FP64 only; use
                                      max_iter = 1000 as the legacy code does      */
-    MF_SOLVER_LAWSON_HANSON = 3   /* Eigen::NNLS started at the current amplitudes
+    MF_SOLVER_LAWSON_HANSON = 3,  /* Eigen::NNLS started at the current amplitudes
                                      (setX), the alternative kept in
                                      multifit_cpu/src/PulseChiSqSNNLS.cpp:253-257;
                                      max_iter bounds least-squares solves; FP64 */
+    MF_SOLVER_INPLACE_FNNLS_CHOL = 4  /* inplace_fnnls_test_cholesky, the alternative
+                                     kept in PulseChiSqSNNLS.cpp:267: the same
+                                     loop with the passive factor updated from
+                                     one solve to the next (nnls/src/
+                                     inplace_fnnls_test_cholesky.cu:35-227); FP64 */
 };
 enum { MF_DTYPE_F64 = 0, MF_DTYPE_F32 = 1 };
 
@@ -277,6 +284,9 @@ mf_status mf_inplace_fnnls_batch(const double *A, const double *b, double *x,
 mf_status mf_fnnls_batch(const double *A, const double *b, double *x,
                          int64_t n, int32_t m, double eps, uint32_t max_iter,
                          void *stream);
+mf_status mf_inplace_fnnls_chol_batch(const double *A, const double *b, double *x,
+                                      int64_t n, int32_t m, double eps, uint32_t max_iter,
+                                      void *stream);
 /* Lawson-Hanson NNLS of the reference's Eigen::NNLS class
  * (nnls/interface/eigen_nnls.h:84-98 constructor, :124 setX, :336-414
  * solve, as tests/main.cpp:53-58 calls it).  Same layout as above; x in is

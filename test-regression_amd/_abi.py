@@ -15,7 +15,8 @@ LIB_PATH = PKG_DIR / "libmf_amd.so"
 
 MF_OK, MF_ERR_INVALID, MF_ERR_HIP, MF_ERR_NOMEM, MF_ERR_UNSUPPORTED = range(5)
 MF_PREC_FP64, MF_PREC_FP32 = 0, 1
-MF_SOLVER_INPLACE_FNNLS, MF_SOLVER_FNNLS, MF_SOLVER_LEGACY, MF_SOLVER_LAWSON_HANSON = 0, 1, 2, 3
+(MF_SOLVER_INPLACE_FNNLS, MF_SOLVER_FNNLS, MF_SOLVER_LEGACY, MF_SOLVER_LAWSON_HANSON,
+ MF_SOLVER_INPLACE_FNNLS_CHOL) = 0, 1, 2, 3, 4
 MF_DTYPE_F64, MF_DTYPE_F32 = 0, 1
 MF_MLP_NPARAM = 6164
 MF_NOALLOC_SCRATCH_BYTES = 64 << 20
@@ -110,6 +111,8 @@ SIGNATURES = [
                                          C.c_int32, C.c_double, C.c_uint32, C.c_void_p]),
     ("mf_fnnls_batch", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                  C.c_int32, C.c_double, C.c_uint32, C.c_void_p]),
+    ("mf_inplace_fnnls_chol_batch", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                              C.c_int32, C.c_double, C.c_uint32, C.c_void_p]),
     ("mf_lh_nnls_batch", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                    C.c_int32, C.c_double, C.c_int32, C.c_void_p, C.c_void_p]),
     ("mf_linalg_batch", C.c_int, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,

@@ -283,6 +283,9 @@ MF_HD void build_cov(const TypeMats<T> &tm, bool use_pi, const T (&x)[10], T (&C
     if (use_pi) {
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
+            // a pulse of amplitude 0 adds exactly 0: skipped (a wavefront
+            // skips the block when none of its lanes needs it)
+            if (x[p] == T(0)) continue;
             const T a2 = x[p] * x[p];
 #pragma unroll
             for (int i = 0; i < NS; ++i)
@@ -596,6 +599,166 @@ MF_HD void fnnls_inplace(const T (&AtA)[10][10], const T (&Atb)[10], T (&x)[10],
     if (iter == max_iter && max_iter > 0) st.hit_max = true;
 }
 
+// inplace_fnnls_test_cholesky (nnls/src/inplace_fnnls_test_cholesky.cu:35-227):
+// the loop of fnnls_inplace with the Cholesky factor of the passive block
+// kept from one solve to the next.  The reference keeps the factor in its
+// permuted (position) order: a 1 x 1 factor for one passive index
+// (:145-155), the new last row appended after an add (:156-159, through
+// fused_cholesky_substitution.hpp:18-57) and, after a removal (the removed
+// position swapped with the last passive one), the rows from that position
+// on recomputed (:160-165, :62-102).  Here the factor of AtA[P,P] is kept
+// in ascending index order inside a masked 10 x 10 factor (rows and columns
+// of free indices the identity), which keeps every register index static:
+// adding index q changes rows >= q of the factor, removing index a rows >=
+// a; those rows of the factor and of the forward solution y = L^{-1} Atb_P
+// are recomputed (row-oriented, as linalgebra/interface/cholesky.hpp:8-38
+// and the fused forward step), the rows before them are kept.  A row whose
+// index no lane of a wavefront needs is skipped.  The permutation is kept as
+// in fnnls_inplace and decides the ties the same way.
+template <typename T> struct CholState {
+    T L[10][10];  // masked factor, lower triangle (L[i][i] unused: ri holds 1/L_ii)
+    T ri[10];
+    T y[10];      // L^{-1} Atb_P
+};
+
+// Rows from .. N-1 of the masked factor of M[P,P] and of y; returns false on
+// a non-positive pivot inside P among them.
+template <typename T, int N>
+MF_HD bool chol_rows_from(const T (&M)[10][10], const T (&b)[10], uint32_t pm, int from,
+                          CholState<T> &C)
+{
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        if (i < from) continue;   // kept from the solve before
+        const bool ini = (pm >> i) & 1u;
+        T sumsq = T(0), t = ini ? b[i] : T(0);
+#pragma unroll
+        for (int j = 0; j < i; ++j) {
+            const bool inj = (pm >> j) & 1u;
+            T s2 = T(0);
+#pragma unroll
+            for (int k = 0; k < j; ++k) s2 = mfma(C.L[i][k], C.L[j][k], s2);
+            const T v = (ini && inj) ? (M[i][j] - s2) * C.ri[j] : T(0);
+            C.L[i][j] = v;
+            sumsq = mfma(v, v, sumsq);
+            t = mfma(-v, C.y[j], t);
+        }
+        const T d = ini ? M[i][i] - sumsq : T(1);
+        ok = ok && (d > T(0));
+        const T r = mrsqrt(d);
+        C.ri[i] = r;
+        C.y[i] = t * r;
+    }
+    return ok;
+}
+
+template <typename T, int N>
+MF_HD void fnnls_inplace_chol(const T (&AtA)[10][10], const T (&Atb)[10], T (&x)[10], T eps,
+                              int max_iter, FnnlsStats &st)
+{
+    uint32_t pm = 0;
+    uint64_t perm = ident_nibbles();
+    uint64_t ipos = perm;
+    int np = 0;
+    int from = 0;   // first row of the factor that no longer holds
+    CholState<T> C;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        C.ri[i] = T(1);
+        C.y[i] = T(0);
+#pragma unroll
+        for (int j = 0; j < 10; ++j) C.L[i][j] = T(0);
+    }
+    int iter = 0;
+    for (; iter < max_iter; ++iter) {
+        if (np == N) break;                                     // :101-102
+        T wmax = T(0);
+        int q = -1, qpos = 16;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            T t = T(0);
+#pragma unroll
+            for (int j = 0; j < N; ++j) t = mfma(sym(AtA, i, j), x[j], t);
+            const T w = Atb[i] - t;
+            const int pos = nib(ipos, i);
+            const bool isfree = !((pm >> i) & 1u);
+            const bool better = (q < 0) || (w > wmax) || (w == wmax && pos < qpos);
+            if (isfree && better) {
+                wmax = w;
+                q = i;
+                qpos = pos;
+            }
+        }
+        if (wmax < eps) break;                                  // :114
+        {   // swap positions np <-> qpos                          :124-133
+            const int inp = nib(perm, np);
+            perm = setnib(setnib(perm, np, q), qpos, inp);
+            ipos = setnib(setnib(ipos, q, np), inp, qpos);
+        }
+        pm |= 1u << q;
+        ++np;
+        from = q < from ? q : from;                             // rows >= q change
+        st.outer++;
+        for (;;) {                                              // :144
+            if (!chol_rows_from<T, N>(AtA, Atb, pm, from, C)) st.llt_fail = true;
+            from = N;
+            T s[10];
+#pragma unroll
+            for (int i = N - 1; i >= 0; --i) {                  // :166
+                T t = C.y[i];
+#pragma unroll
+                for (int j = i + 1; j < N; ++j) t = mfma(-C.L[j][i], s[j], t);
+                s[i] = t * C.ri[i];
+            }
+            st.inner++;
+            bool allpos = true;
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                if ((pm >> i) & 1u) allpos = allpos && (s[i] > T(0));
+            if (allpos) {                                       // :170-173
+#pragma unroll
+                for (int i = 0; i < N; ++i)
+                    if ((pm >> i) & 1u) x[i] = s[i];
+                break;
+            }
+            T alpha = Lim<T>::max();                            // :179-191
+            int a = -1, apos = 16;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                if (((pm >> i) & 1u) && s[i] <= T(0)) {
+                    const T ratio = mdiv(x[i], x[i] - s[i]);
+                    const int pos = nib(ipos, i);
+                    if (ratio < alpha || (a >= 0 && ratio == alpha && pos < apos)) {
+                        alpha = ratio;
+                        a = i;
+                        apos = pos;
+                    }
+                }
+            }
+            if (a < 0) {                                        // :193-196
+#pragma unroll
+                for (int i = 0; i < N; ++i)
+                    if ((pm >> i) & 1u) x[i] = s[i];
+                break;
+            }
+#pragma unroll
+            for (int i = 0; i < N; ++i)                         // :206-207
+                if ((pm >> i) & 1u) x[i] = (i == a) ? T(0) : mfma(alpha, s[i] - x[i], x[i]);
+            --np;
+            {   // swap positions apos <-> np                      :213-220
+                const int il = nib(perm, np);
+                perm = setnib(setnib(perm, apos, il), np, a);
+                ipos = setnib(setnib(ipos, il, apos), a, np);
+            }
+            pm &= ~(1u << a);
+            from = a;                                           // rows >= a change
+            if (np == 0) break;
+        }
+    }
+    if (iter == max_iter && max_iter > 0) st.hit_max = true;
+}
+
 // State of the legacy NNLS that lives across the Minimize iterations of one
 // fit: the passive set, the permutation (position -> index and its inverse)
 // and the number of passive positions (PulseChiSqSNNLS members _nP, and the
@@ -884,8 +1047,8 @@ template <typename T> struct FitOut {
 // the covariance) is parked in `park` between the normal equations and chi2;
 // Park is any type with store(k, v) / load(k) for k < 55.  Returns chi2.
 // XS: solver compiled in besides inplace_fnnls / fnnls (chosen at run time
-// by prm.solver): XS_LEGACY (with ls) or XS_LH.
-enum { XS_NONE = 0, XS_LEGACY = 2, XS_LH = 3 };
+// by prm.solver): XS_LEGACY (with ls), XS_LH or XS_CHOL.
+enum { XS_NONE = 0, XS_LEGACY = 2, XS_LH = 3, XS_CHOL = 4 };
 
 template <typename T, int NS, int NP, int XS = XS_NONE, typename Park>
 MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams &prm,
@@ -907,6 +1070,8 @@ MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams 
         fnnls_legacy<T, NP>(AtA, Atb, x, T(prm.eps), prm.max_iter, st, *ls);
     else if (XS == XS_LH)   // PulseChiSqSNNLS.cpp:253-257 (setX, solve, x)
         lh_nnls<T, NP>(AtA, Atb, x, T(prm.eps), prm.max_iter, st);
+    else if (XS == XS_CHOL)  // the incremental-Cholesky twin (PulseChiSqSNNLS.cpp:267)
+        fnnls_inplace_chol<T, NP>(AtA, Atb, x, T(prm.eps), prm.max_iter, st);
     else if (prm.solver == 1)
         fnnls_mask<T, NP>(AtA, Atb, x, T(prm.eps), prm.max_iter, st, prm.pmask);
     else

@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/mf_abi.h"
@@ -127,12 +128,17 @@ __device__ __forceinline__ bool seg_item(const SegArg &a, int &t, int64_t &c)
 
 // The lane's copy of the launch's parameters: its channel's noise scale
 // (ped: per-channel pederr, null for the type's) and its type's pulses.
+// PED: the kernel is built for a per-channel pederr; without it rho stays
+// the constant 1 and costs nothing in the fit loops.
+template <bool PED = true>
 __device__ __forceinline__ FitParams lane_params(const FitParams &prm, const DevType &ty,
                                                  const double *__restrict__ ped, int64_t c)
 {
     FitParams p = prm;
     p.pmask = ty.h.pmask;
-    if (ped) {
+    p.use_pi = prm.use_pi && ty.h.has_pi;   // no pulse covariance: nothing to add
+    p.rho = 1.0;   // a compile-time constant unless PED
+    if (PED && ped) {
         const double e = ped[c];
         p.rho = e * e * ty.inv_pederr2;
     }
@@ -207,7 +213,7 @@ __device__ __forceinline__ uint32_t warm_load(const WarmIn &w, int64_t c, int64_
     return pm;
 }
 
-template <typename TS, int N>
+template <typename TS, int N, bool PED>
 __global__ __launch_bounds__(256)
 void k_fast_p1(const DevType *__restrict__ types, SegArg seg,
                const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
@@ -217,7 +223,7 @@ void k_fast_p1(const DevType *__restrict__ types, SegArg seg,
     int type_id;
     int64_t c;
     if (!seg_item(seg, type_id, c)) return;
-    const FitParams prm = lane_params(prm0, types[type_id], ped, c);
+    const FitParams prm = lane_params<PED>(prm0, types[type_id], ped, c);
     float s[10], x[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) s[i] = i < N ? (float)samples[(int64_t)i * nch + c] : 0.0f;
@@ -256,7 +262,7 @@ template <typename TS, int N> struct SampleReload {
 };
 // Four waves per SIMD: at 128 VGPRs a few values spill (about 56 bytes per
 // lane, outside the solve loops); measured faster than 137 VGPRs at three.
-template <typename TS, int N>
+template <typename TS, int N, bool PED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 void k_fast_fused(const DevType *__restrict__ types, SegArg seg,
                   const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
@@ -266,7 +272,7 @@ void k_fast_fused(const DevType *__restrict__ types, SegArg seg,
     int type_id;
     int64_t c;
     if (!seg_item(seg, type_id, c)) return;
-    const FitParams prm = lane_params(prm0, types[type_id], ped, c);
+    const FitParams prm = lane_params<PED>(prm0, types[type_id], ped, c);
     float s[10], x[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) s[i] = i < N ? (float)samples[(int64_t)i * nch + c] : 0.0f;
@@ -294,7 +300,7 @@ void k_fast_fused(const DevType *__restrict__ types, SegArg seg,
 // pass on MFMA, as k_mlp_warm), then the FP64 phase from that prediction.
 // The prediction never leaves the chip.  Blocks of FAST_WARM_BLOCK threads.
 constexpr int FAST_WARM_BLOCK = 256;
-template <typename TS, int N>
+template <typename TS, int N, bool PED>
 __global__ __launch_bounds__(FAST_WARM_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 void k_fast_warm(const DevType *__restrict__ types, SegArg seg,
                  const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
@@ -322,7 +328,7 @@ void k_fast_warm(const DevType *__restrict__ types, SegArg seg,
     uint32_t pm = mlp::mlp_warm_wave(L, xs[wv], pz[wv], pmw[wv], lane, s, N, xp);
     if (!live) return;
     const DevType &ty = types[type_id];
-    const FitParams prm = lane_params(prm0, ty, ped, c);
+    const FitParams prm = lane_params<PED>(prm0, ty, ped, c);
     pm &= prm.pmask & ((1u << N) - 1u);
     float x[10];
 #pragma unroll
@@ -340,7 +346,7 @@ void k_fast_warm(const DevType *__restrict__ types, SegArg seg,
     ito[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
 }
 
-template <typename TS, int N, bool LIVE>
+template <typename TS, int N, bool LIVE, bool PED>
 __global__ __launch_bounds__(256)
 void k_fast_p2(const DevType *__restrict__ types, SegArg seg,
                const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
@@ -351,7 +357,7 @@ void k_fast_p2(const DevType *__restrict__ types, SegArg seg,
     int type_id;
     int64_t c;
     if (!seg_item(seg, type_id, c)) return;
-    const FitParams prm = lane_params(prm0, types[type_id], ped, c);
+    const FitParams prm = lane_params<PED>(prm0, types[type_id], ped, c);
     float s[10], x[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) {
@@ -458,7 +464,7 @@ __device__ __forceinline__ uint32_t warm_start_state(uint32_t pm, const float (&
 // is worked on); a lane that finishes switches to its next channel at once,
 // and empty prefetch slots are refilled from the queue once at least `need`
 // of them are empty.
-template <typename TS, int N>
+template <typename TS, int N, bool PED>
 __global__ __launch_bounds__(256)
 void k_fast_p1q(const DevType *__restrict__ types, int type_id, Seg seg,
                 const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
@@ -488,7 +494,7 @@ void k_fast_p1q(const DevType *__restrict__ types, int type_id, Seg seg,
 #pragma unroll
                 for (int i = 0; i < 10; ++i) svn[i] = i < N ? (float)samples[(int64_t)i * nch + cn] : 0.0f;
                 warm_fetch<N>(warm, cn, nch, pmn, xn);
-                if (ped) {
+                if (PED && ped) {
                     const double e = ped[cn];
                     rhon = e * e * ty.inv_pederr2;
                 }
@@ -547,7 +553,7 @@ void k_fast_p1q(const DevType *__restrict__ types, int type_id, Seg seg,
 // Phase 2 (zero or unused pulse covariance), persistent, with the same
 // prefetch and refill scheme; the next channel's samples and phase-1 result
 // are loaded while the current one is worked on.
-template <typename TS, int N>
+template <typename TS, int N, bool PED>
 __global__ __launch_bounds__(256)
 void k_fast_p2q(const DevType *__restrict__ types, int type_id, Seg seg,
                 const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
@@ -556,6 +562,7 @@ void k_fast_p2q(const DevType *__restrict__ types, int type_id, Seg seg,
                 int need)
 {
     FitParams prm = prm0;   // rho: the lane's current channel
+    prm.rho = 1.0;
     double rhon = 1.0;
     const DevType &ty = types[type_id];
     const int64_t nitems = seg_count(seg);
@@ -582,7 +589,7 @@ void k_fast_p2q(const DevType *__restrict__ types, int type_id, Seg seg,
                 }
                 fn = flo[cn];
                 itn = ito[cn];
-                if (ped) {
+                if (PED && ped) {
                     const double e = ped[cn];
                     rhon = e * e * ty.inv_pederr2;
                 }
@@ -791,7 +798,7 @@ __global__ __launch_bounds__(256) void k_part_scatter(
 // ---------------------------------------------------------------------------
 // solver-only kernels (inplace_fnnls / fnnls on given A, b)
 // ---------------------------------------------------------------------------
-template <int M>
+template <int M, bool CHOL = false>
 __device__ void solve_one(const double *__restrict__ A, const double *__restrict__ b,
                           double *__restrict__ x, int solver, double eps, unsigned max_iter,
                           int32_t *__restrict__ solved)
@@ -817,6 +824,8 @@ __device__ void solve_one(const double *__restrict__ A, const double *__restrict
     if (solver == 2) {
         const bool ok = lh_nnls<double, M>(AtA, Atb, xx, eps, (int)max_iter, st);
         if (solved) *solved = ok ? 1 : 0;
+    } else if (solver == 3) {
+        if constexpr (CHOL) fnnls_inplace_chol<double, M>(AtA, Atb, xx, eps, (int)max_iter, st);
     } else if (solver == 1) {
         fnnls_mask<double, M>(AtA, Atb, xx, eps, (int)max_iter, st);
     } else {
@@ -848,6 +857,33 @@ __global__ __launch_bounds__(256) void k_nnls(const double *__restrict__ A,
     case 8: solve_one<8>(Ac, bc, xc, solver, eps, max_iter, ok); break;
     case 9: solve_one<9>(Ac, bc, xc, solver, eps, max_iter, ok); break;
     default: solve_one<10>(Ac, bc, xc, solver, eps, max_iter, ok); break;
+    }
+}
+
+// inplace_fnnls_test_cholesky batches: a kernel of their own, so that the
+// persistent factor's registers do not weigh on the other solvers
+__global__ __launch_bounds__(256) void k_nnls_chol(const double *__restrict__ A,
+                                              const double *__restrict__ b, double *__restrict__ x,
+                                              int64_t n, int m, int solver, double eps,
+                                              unsigned max_iter, int32_t *__restrict__ solved)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const double *Ac = A + c * 100;
+    const double *bc = b + c * 10;
+    double *xc = x + c * 10;
+    int32_t *ok = solved ? solved + c : nullptr;
+    switch (m) {
+    case 1: solve_one<1, true>(Ac, bc, xc, solver, eps, max_iter, ok); break;
+    case 2: solve_one<2, true>(Ac, bc, xc, solver, eps, max_iter, ok); break;
+    case 3: solve_one<3, true>(Ac, bc, xc, solver, eps, max_iter, ok); break;
+    case 4: solve_one<4, true>(Ac, bc, xc, solver, eps, max_iter, ok); break;
+    case 5: solve_one<5, true>(Ac, bc, xc, solver, eps, max_iter, ok); break;
+    case 6: solve_one<6, true>(Ac, bc, xc, solver, eps, max_iter, ok); break;
+    case 7: solve_one<7, true>(Ac, bc, xc, solver, eps, max_iter, ok); break;
+    case 8: solve_one<8, true>(Ac, bc, xc, solver, eps, max_iter, ok); break;
+    case 9: solve_one<9, true>(Ac, bc, xc, solver, eps, max_iter, ok); break;
+    default: solve_one<10, true>(Ac, bc, xc, solver, eps, max_iter, ok); break;
     }
 }
 
@@ -1263,7 +1299,8 @@ static mf_status launch_fit(mf_context *ctx, const SegArg &seg, const void *samp
         constexpr bool E = decltype(e)::value;
         return prm.solver == MF_SOLVER_LEGACY ? k_fit_plain<T, TS, N, E, XS_LEGACY>
                : prm.solver == MF_SOLVER_LAWSON_HANSON ? k_fit_plain<T, TS, N, E, XS_LH>
-                                                       : k_fit_plain<T, TS, N, E, XS_NONE>;
+               : prm.solver == MF_SOLVER_INPLACE_FNNLS_CHOL ? k_fit_plain<T, TS, N, E, XS_CHOL>
+                                                            : k_fit_plain<T, TS, N, E, XS_NONE>;
     };
     auto kern = errors ? pick(std::true_type{}) : pick(std::false_type{});
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), lds, st, ctx->d_types, seg,
@@ -1276,7 +1313,7 @@ static mf_status launch_fit(mf_context *ctx, const SegArg &seg, const void *samp
 
 // mlp_here: a warm start whose prediction the fit kernel computes itself
 // (k_fast_warm); warm then holds no prediction.
-template <typename TS, int N, bool LIVE>
+template <typename TS, int N, bool LIVE, bool PED>
 static mf_status launch_fast(mf_context *ctx, const SegArg &seg, const void *samples,
                              const double *ped, int64_t nch, const mf_result *out,
                              const FitParams &prm, int block, WarmIn warm, bool persistent,
@@ -1288,7 +1325,7 @@ static mf_status launch_fast(mf_context *ctx, const SegArg &seg, const void *sam
     if (mlp_here) {
         if (LIVE || persistent) return fail(MF_ERR_INVALID, "in-kernel warm start: not for this launch");
         const int64_t g = (nt + FAST_WARM_BLOCK - 1) / FAST_WARM_BLOCK;
-        hipLaunchKernelGGL((k_fast_warm<TS, N>), dim3((unsigned)g), dim3(FAST_WARM_BLOCK), 0, st,
+        hipLaunchKernelGGL((k_fast_warm<TS, N, PED>), dim3((unsigned)g), dim3(FAST_WARM_BLOCK), 0, st,
                            ctx->d_types, seg, static_cast<const TS *>(samples), ped, nch,
                            static_cast<float *>(out->x), static_cast<float *>(out->chi2),
                            out->flags, out->iters, ctx->mlp_frag, ctx->mlp_pq, prm);
@@ -1302,12 +1339,12 @@ static mf_status launch_fast(mf_context *ctx, const SegArg &seg, const void *sam
         const int64_t g1 = std::max<int64_t>(1, std::min<int64_t>(need_w, (int64_t)ctx->cus * tu.p1_wpc) / wpb);
         const int64_t g2 = std::max<int64_t>(1, std::min<int64_t>(need_w, (int64_t)ctx->cus * tu.p2_wpc) / wpb);
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, queue, 2);
-        hipLaunchKernelGGL((k_fast_p1q<TS, N>), dim3((unsigned)g1), dim3(tu.block), 0, st,
+        hipLaunchKernelGGL((k_fast_p1q<TS, N, PED>), dim3((unsigned)g1), dim3(tu.block), 0, st,
                            ctx->d_types, seg.type_id, seg.list, static_cast<const TS *>(samples),
                            ped, nch, static_cast<float *>(out->x), out->flags, out->iters, warm,
                            prm, queue, tu.p1_need);
         HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL((k_fast_p2q<TS, N>), dim3((unsigned)g2), dim3(tu.block), 0, st,
+        hipLaunchKernelGGL((k_fast_p2q<TS, N, PED>), dim3((unsigned)g2), dim3(tu.block), 0, st,
                            ctx->d_types, seg.type_id, seg.list, static_cast<const TS *>(samples),
                            ped, nch, static_cast<float *>(out->x), static_cast<float *>(out->chi2),
                            out->flags, out->iters, warm.pm ? 1 : 0, prm, queue + 1,
@@ -1317,19 +1354,19 @@ static mf_status launch_fast(mf_context *ctx, const SegArg &seg, const void *sam
     }
     const int64_t grid = (nt + block - 1) / block;
     if (!LIVE) {
-        hipLaunchKernelGGL((k_fast_fused<TS, N>), dim3((unsigned)grid), dim3(block), 0, st,
+        hipLaunchKernelGGL((k_fast_fused<TS, N, PED>), dim3((unsigned)grid), dim3(block), 0, st,
                            ctx->d_types, seg, static_cast<const TS *>(samples), ped, nch,
                            static_cast<float *>(out->x), static_cast<float *>(out->chi2),
                            out->flags, out->iters, warm, prm);
         HIP_TRY(hipGetLastError());
         return MF_OK;
     }
-    hipLaunchKernelGGL((k_fast_p1<TS, N>), dim3((unsigned)grid), dim3(block), 0, st,
+    hipLaunchKernelGGL((k_fast_p1<TS, N, PED>), dim3((unsigned)grid), dim3(block), 0, st,
                        ctx->d_types, seg, static_cast<const TS *>(samples), ped, nch,
                        static_cast<float *>(out->x), out->flags, out->iters, warm, prm);
     HIP_TRY(hipGetLastError());
     const size_t lds = LIVE ? sizeof(double) * 55 * (size_t)block : 0;
-    hipLaunchKernelGGL((k_fast_p2<TS, N, LIVE>), dim3((unsigned)grid), dim3(block), lds, st,
+    hipLaunchKernelGGL((k_fast_p2<TS, N, LIVE, PED>), dim3((unsigned)grid), dim3(block), lds, st,
                        ctx->d_types, seg, static_cast<const TS *>(samples), ped, nch,
                        static_cast<float *>(out->x), static_cast<float *>(out->chi2), out->flags,
                        out->iters, warm.pm ? 1 : 0, prm);
@@ -1354,15 +1391,21 @@ static mf_status dispatch_fast(mf_context *ctx, const SegArg &seg, const mf_batc
 {
     const mf_chtype &t = ctx->types[(size_t)seg_type0(seg)];
     const bool live = type_live(t, prm);
-    if (t.nsamples == 10)
-        return live ? launch_fast<TS, 10, true>(ctx, seg, in->samples, in->pederr, in->nch, out,
-                                                prm, block, warm, persistent, queue, st)
-                    : launch_fast<TS, 10, false>(ctx, seg, in->samples, in->pederr, in->nch, out,
-                                                 prm, block, warm, persistent, queue, st, mlp_here);
-    return live ? launch_fast<TS, 8, true>(ctx, seg, in->samples, in->pederr, in->nch, out, prm,
-                                           block, warm, persistent, queue, st)
-                : launch_fast<TS, 8, false>(ctx, seg, in->samples, in->pederr, in->nch, out, prm,
-                                            block, warm, persistent, queue, st, mlp_here);
+    auto go = [&](auto ped) {
+        constexpr bool P = decltype(ped)::value;
+        if (t.nsamples == 10)
+            return live ? launch_fast<TS, 10, true, P>(ctx, seg, in->samples, in->pederr, in->nch,
+                                                       out, prm, block, warm, persistent, queue, st)
+                        : launch_fast<TS, 10, false, P>(ctx, seg, in->samples, in->pederr,
+                                                        in->nch, out, prm, block, warm,
+                                                        persistent, queue, st, mlp_here);
+        return live ? launch_fast<TS, 8, true, P>(ctx, seg, in->samples, in->pederr, in->nch, out,
+                                                  prm, block, warm, persistent, queue, st)
+                    : launch_fast<TS, 8, false, P>(ctx, seg, in->samples, in->pederr, in->nch, out,
+                                                   prm, block, warm, persistent, queue, st,
+                                                   mlp_here);
+    };
+    return in->pederr ? go(std::true_type{}) : go(std::false_type{});
 }
 
 template <typename T>
@@ -1491,7 +1534,8 @@ static mf_status plan_call(mf_context *ctx, const mf_batch *in, const mf_result 
     if (opts->max_minimize < 0 || opts->max_iter < 0)
         return fail(MF_ERR_INVALID, "max_minimize and max_iter must be >= 0");
     if (opts->solver != MF_SOLVER_INPLACE_FNNLS && opts->solver != MF_SOLVER_FNNLS &&
-        opts->solver != MF_SOLVER_LEGACY && opts->solver != MF_SOLVER_LAWSON_HANSON)
+        opts->solver != MF_SOLVER_LEGACY && opts->solver != MF_SOLVER_LAWSON_HANSON &&
+        opts->solver != MF_SOLVER_INPLACE_FNNLS_CHOL)
         return fail(MF_ERR_INVALID, "unknown solver");
     if (opts->warm_start && opts->precision != MF_PREC_FP32)
         return fail(MF_ERR_UNSUPPORTED, "warm_start needs MF_PREC_FP32");
@@ -1977,7 +2021,8 @@ static mf_status nnls_batch(const double *A, const double *b, double *x, int64_t
     if (!A || !b || !x) return fail(MF_ERR_INVALID, "NULL argument");
     if (m < 1 || m > 10) return fail(MF_ERR_INVALID, "m must be in 1..10");
     const int block = 64;
-    hipLaunchKernelGGL(k_nnls, dim3((unsigned)((n + block - 1) / block)), dim3(block), 0,
+    hipLaunchKernelGGL(solver == 3 ? k_nnls_chol : k_nnls,
+                       dim3((unsigned)((n + block - 1) / block)), dim3(block), 0,
                        static_cast<hipStream_t>(stream), A, b, x, n, m, solver, eps, max_iter,
                        solved);
     HIP_TRY(hipGetLastError());
@@ -1995,6 +2040,13 @@ extern "C" mf_status mf_fnnls_batch(const double *A, const double *b, double *x,
                                     int32_t m, double eps, uint32_t max_iter, void *stream)
 {
     return nnls_batch(A, b, x, n, m, eps, max_iter, 1, stream);
+}
+
+extern "C" mf_status mf_inplace_fnnls_chol_batch(const double *A, const double *b, double *x,
+                                                 int64_t n, int32_t m, double eps,
+                                                 uint32_t max_iter, void *stream)
+{
+    return nnls_batch(A, b, x, n, m, eps, max_iter, 3, stream);
 }
 
 extern "C" mf_status mf_lh_nnls_batch(const double *A, const double *b, double *x, int64_t n,

@@ -287,13 +287,15 @@ class Engine:
 def nnls_batch(A: torch.Tensor, b: torch.Tensor, x: torch.Tensor, m: int = 10,
                eps: float = 1e-11, max_iter: int = 1000, solver: str = "inplace_fnnls",
                stream=None) -> torch.Tensor:
-    """Batched inplace_fnnls / fnnls.  A [n,10,10] column-major per problem
-    (A[c, col, row]), b [n,10], x [n,10] in/out (warm start)."""
+    """Batched inplace_fnnls / inplace_fnnls_test_cholesky ("inplace_fnnls_chol")
+    / fnnls.  A [n,10,10] column-major per problem (A[c, col, row]), b [n,10],
+    x [n,10] in/out (warm start)."""
     L = _abi.lib()
     for t in (A, b, x):
         if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
             raise ValueError("A, b, x must be contiguous float64 device tensors")
-    f = L.mf_inplace_fnnls_batch if solver == "inplace_fnnls" else L.mf_fnnls_batch
+    f = {"inplace_fnnls": L.mf_inplace_fnnls_batch, "fnnls": L.mf_fnnls_batch,
+         "inplace_fnnls_chol": L.mf_inplace_fnnls_chol_batch}[solver]
     _abi.check(f(A.data_ptr(), b.data_ptr(), x.data_ptr(), A.shape[0], m, eps, max_iter,
                  C.c_void_p(_stream_ptr(stream))), solver)
     return x

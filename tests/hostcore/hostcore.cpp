@@ -53,12 +53,16 @@ static void run(const DevType &dt, const double *samples, int64_t nch, int64_t c
         fit_plain<T, N, N, true, XS_LEGACY>(mats<T>(dt), s, prm, park, out, ip);
     else if (g_err && prm.solver == 3)
         fit_plain<T, N, N, true, XS_LH>(mats<T>(dt), s, prm, park, out, ip);
+    else if (g_err && prm.solver == 4)
+        fit_plain<T, N, N, true, XS_CHOL>(mats<T>(dt), s, prm, park, out, ip);
     else if (g_err)
         fit_plain<T, N, N, true>(mats<T>(dt), s, prm, park, out, ip);
     else if (prm.solver == 2)
         fit_plain<T, N, N, false, XS_LEGACY>(mats<T>(dt), s, prm, park, out);
     else if (prm.solver == 3)
         fit_plain<T, N, N, false, XS_LH>(mats<T>(dt), s, prm, park, out);
+    else if (prm.solver == 4)
+        fit_plain<T, N, N, false, XS_CHOL>(mats<T>(dt), s, prm, park, out);
     else
         fit_plain<T, N, N>(mats<T>(dt), s, prm, park, out);
     for (int p = 0; p < 10; ++p) x[p * nch + c] = p < N ? (double)out.x[p] : 0.0;
@@ -129,6 +133,7 @@ extern "C" int hc_fit_batch2(const mf_chtype *types, int ntypes, const uint8_t *
         // the kernels' lane_params
         prm = prm0;
         prm.pmask = d.h.pmask;
+        prm.use_pi = prm0.use_pi && d.h.has_pi;
         if (pederr) prm.rho = pederr[c] * pederr[c] * d.inv_pederr2;
         const bool f64 = o->precision == MF_PREC_FP64;
         const bool live = d.h.has_pi && o->use_pulse_cov;
@@ -155,7 +160,7 @@ extern "C" int hc_fit_batch(const mf_chtype *types, int ntypes, const uint8_t *t
 
 // single NNLS problems (A column-major ld 10, m x m) through fit_core's solvers
 // (0 inplace_fnnls, 1 fnnls, 2 Lawson-Hanson with max_iter as a signed
-// solve count).  Returns 0 when Lawson-Hanson stopped at max_iter, else 1.
+// solve count, 3 inplace_fnnls_test_cholesky).  Returns 0 when Lawson-Hanson stopped at max_iter, else 1.
 extern "C" int hc_nnls(const double *A, const double *b, double *x, int m, int solver,
                        double eps, unsigned max_iter)
 {
@@ -176,6 +181,7 @@ extern "C" int hc_nnls(const double *A, const double *b, double *x, int m, int s
 #define CASE(M)                                                                   \
     case M:                                                                       \
         if (solver == 2) ok = lh_nnls<double, M>(AtA, Atb, xx, eps, (int)max_iter, st); \
+        else if (solver == 3) fnnls_inplace_chol<double, M>(AtA, Atb, xx, eps, (int)max_iter, st); \
         else if (solver == 1) fnnls_mask<double, M>(AtA, Atb, xx, eps, (int)max_iter, st); \
         else fnnls_inplace<double, M>(AtA, Atb, xx, eps, 0.0, (int)max_iter, st); \
         break;

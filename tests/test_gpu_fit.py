@@ -64,6 +64,22 @@ def test_fp64_fit_fnnls_solver(pkg, gpu):
     _gate(ct, s, t, r, oracle_opts(solver=OR_SOLVER_FNNLS), label="fp64 fnnls")
 
 
+@pytest.mark.parametrize("cfg,pulse_cov,n", [("c1", "zero", 10_000), ("c2", "zero", 20_000),
+                                             ("c2", "live", 20_000), ("c3", "live", 20_000),
+                                             ("c5", "zero", 10_000), ("c5", "live", 10_000)])
+def test_fp64_fit_incremental_cholesky_solver(pkg, gpu, cfg, pulse_cov, n):
+    """MF_SOLVER_INPLACE_FNNLS_CHOL (inplace_fnnls_test_cholesky, the factor
+    of the passive block updated from one solve to the next) against the
+    oracle's restatement of the same function, and against the oracle's
+    refactorising inplace_fnnls: the same fits to rounding."""
+    from test_regression_amd import _abi
+    from oracle_binding import OR_SOLVER_INPLACE_CHOL
+    ct, s, t, r = _run(pkg, cfg, pulse_cov, n, solver=_abi.MF_SOLVER_INPLACE_FNNLS_CHOL)
+    _gate(ct, s, t, r, oracle_opts(solver=OR_SOLVER_INPLACE_CHOL),
+          label=f"fp64 incremental cholesky {cfg} {pulse_cov}")
+    _gate(ct, s, t, r, label=f"fp64 incremental cholesky vs inplace_fnnls {cfg} {pulse_cov}")
+
+
 def test_fp64_fit_cpu_max_iter(pkg, gpu):
     """multifit_cpu's FNNLS cap of 10 (PulseChiSqSNNLS.cpp:243) truncates some fits."""
     ct, s, t, r = _run(pkg, "c2", "live", 5000, max_iter=10)

@@ -48,7 +48,7 @@ def _problems(n, m, seed, kind="random"):
     return A, b
 
 
-@pytest.mark.parametrize("solver", ["inplace_fnnls", "fnnls"])
+@pytest.mark.parametrize("solver", ["inplace_fnnls", "fnnls", "inplace_fnnls_chol"])
 @pytest.mark.parametrize("m", [10, 7, 3, 1])
 def test_nnls_batch_matches_oracle_and_scipy(pkg, gpu, solver, m):
     from test_regression_amd import engine
@@ -62,7 +62,8 @@ def test_nnls_batch_matches_oracle_and_scipy(pkg, gpu, solver, m):
     torch.cuda.synchronize()
     xg = x.cpu().numpy()
     L = oracle()
-    f = L.or_inplace_fnnls if solver == "inplace_fnnls" else L.or_fnnls
+    f = {"inplace_fnnls": L.or_inplace_fnnls, "fnnls": L.or_fnnls,
+         "inplace_fnnls_chol": L.or_inplace_fnnls_chol}[solver]
     worst = 0.0
     for c in range(n):
         Ar = np.zeros((10, 10))

@@ -166,3 +166,60 @@ def test_fast_core_tables_match_formed_factors(pkg, cfg, pulse_cov):
     same = ((x0 > 0) == (x1 > 0)).all(axis=0)
     both = ((x0 > 0) == (ref["x"].T > 0)).all(axis=0) & ((x1 > 0) == (ref["x"].T > 0)).all(axis=0)
     assert np.all(same[both])
+
+
+@pytest.mark.parametrize("cfg,pulse_cov,n", [("c2", "zero", 3000), ("c2", "live", 3000),
+                                             ("c3", "live", 3000)])
+def test_incremental_cholesky_core_matches_oracle(pkg, cfg, pulse_cov, n):
+    """The incremental-Cholesky FNNLS (solver 4, inplace_fnnls_test_cholesky)
+    against the oracle's restatement, channel by channel."""
+    from oracle_binding import OR_SOLVER_INPLACE_CHOL
+    from test_regression_amd import workloads, _abi
+    w = workloads.config(cfg, pulse_cov)
+    ct = w.chtypes()
+    s, t, _ = w.generate(n)
+    ref = oracle_fit(ct, s, t, oracle_opts(solver=OR_SOLVER_INPLACE_CHOL), nthreads=4)
+    x, chi2, fl, it = hostcore_fit(ct, s, t, _abi.default_opts(
+        solver=_abi.MF_SOLVER_INPLACE_FNNLS_CHOL))
+    parity.gate(x, chi2, ref, 1e-10, iters=it, flags=fl, compare_inner=True,
+                label=f"incremental cholesky core {cfg} {pulse_cov}")
+
+
+def test_incremental_cholesky_core_single_problems(pkg):
+    """Single NNLS problems through the incremental-Cholesky solver against the
+    oracle's inplace_fnnls_test_cholesky and brute force (m <= 6)."""
+    import ctypes as C
+    import itertools
+    from oracle_binding import OrStats, dptr, hostcore, oracle
+    H, L = hostcore(), oracle()
+    rng = np.random.default_rng(5)
+    for m in (1, 2, 4, 6, 10):
+        for _ in range(200):
+            M = rng.standard_normal((m, m)) + 2 * np.eye(m)
+            b = rng.standard_normal(m) * 10
+            Acm = np.zeros((10, 10))
+            Acm[:m, :m] = M.T            # column-major, ld 10
+            bb = np.zeros(10)
+            bb[:m] = b
+            xg = np.zeros(10)
+            H.hc_nnls(dptr(np.ascontiguousarray(Acm)), dptr(bb), dptr(xg), m, 3, 1e-11, 1000)
+            Ar = np.zeros((10, 10))
+            Ar[:m, :m] = M
+            xr = np.zeros(10)
+            L.or_inplace_fnnls_chol(dptr(Ar), dptr(bb), dptr(xr), m, 1e-11, 1000, C.byref(OrStats()))
+            sc = max(1.0, np.abs(xr).max())
+            assert np.abs(xg - xr).max() <= 1e-10 * sc * max(1.0, np.linalg.cond(M) ** 2 * 1e-12)
+            if m <= 6:
+                best, bx = np.inf, None
+                for k in range(m + 1):
+                    for S in itertools.combinations(range(m), k):
+                        x = np.zeros(m)
+                        if S:
+                            xs, *_ = np.linalg.lstsq(M[:, S], b, rcond=None)
+                            if np.any(xs < 0):
+                                continue
+                            x[list(S)] = xs
+                        rr = np.sum((M @ x - b) ** 2)
+                        if rr < best - 1e-12:
+                            best, bx = rr, x
+                assert np.allclose(xg[:m], bx, rtol=1e-7, atol=1e-7 * max(1, np.abs(bx).max()))
