@@ -888,101 +888,166 @@ __global__ __launch_bounds__(256) void k_nnls_chol(const double *__restrict__ A,
 }
 
 // ---------------------------------------------------------------------------
-// linalgebra primitives, one problem per lane, row-major full x full
+// linalgebra primitives: one problem per lane, 32 problems per block
 // (operation order as linalgebra/interface/*.hpp)
 // ---------------------------------------------------------------------------
+// The caller's layout is a row-major matrix per problem, problem after
+// problem (the reference's raw pointers).  A lane reading its own problem
+// straight from there would stride by full^2 elements and no load would
+// coalesce, so a block first copies its 64 problems' operands into LDS with
+// consecutive threads on consecutive addresses, transposed to
+// [element][problem]; each of the first 32 lanes then works on its column
+// (32 lanes touch 32 consecutive words: no bank conflicts), and the outputs go
+// back the same way.  32 problems per block keep the FP64 tiles (M, L, b, y of
+// 10 x 10) at 56 KB of LDS.
+constexpr int LA_PROBLEMS = 32;
+constexpr int LA_THREADS = 64;
+
 template <typename T>
-__global__ void k_linalg(int op, T *M, T *L, T *b, T *y, int64_t n, int full, int view, int pi,
-                         int pj)
+__device__ __forceinline__ void la_load(T *__restrict__ tile, const T *__restrict__ g, int elems,
+                                        int64_t c0, int64_t n)
 {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
-    const int64_t mo = c * full * full, vo = c * full;
-#define AT(P, r, q) P[mo + (int64_t)(r) * full + (q)]
-    switch (op) {
-    case MF_LA_CHOLESKY:  // cholesky.hpp:8-38
-        for (int i = 0; i < view; ++i) {
-            T sumsq = 0;
-            for (int j = 0; j < i; ++j) {
-                T s2 = 0;
-                for (int k = 0; k < j; ++k) s2 += AT(L, i, k) * AT(L, j, k);
-                AT(L, i, j) = (AT(M, i, j) - s2) / AT(L, j, j);
-                sumsq += AT(L, i, j) * AT(L, i, j);
+    const int64_t np = (n - c0) < LA_PROBLEMS ? (n - c0) : LA_PROBLEMS;
+    const int total = (int)np * elems;
+    const T *src = g + c0 * elems;
+    for (int k = threadIdx.x; k < total; k += blockDim.x) {
+        const int p = k / elems, e = k - p * elems;
+        tile[e * LA_PROBLEMS + p] = src[k];
+    }
+}
+template <typename T>
+__device__ __forceinline__ void la_store(T *__restrict__ g, const T *__restrict__ tile, int elems,
+                                         int64_t c0, int64_t n)
+{
+    const int64_t np = (n - c0) < LA_PROBLEMS ? (n - c0) : LA_PROBLEMS;
+    const int total = (int)np * elems;
+    T *dst = g + c0 * elems;
+    for (int k = threadIdx.x; k < total; k += blockDim.x) {
+        const int p = k / elems, e = k - p * elems;
+        dst[k] = tile[e * LA_PROBLEMS + p];
+    }
+}
+
+// needM/needL/needb/needy: operands the op reads; outM/outL/outy: the ones it writes
+template <typename T>
+__global__ __launch_bounds__(LA_THREADS)
+void k_linalg(int op, T *M, T *L, T *b, T *y, int64_t n, int full, int view, int pi, int pj)
+{
+    extern __shared__ unsigned char la_raw[];
+    const int ff = full * full;
+    T *tM = reinterpret_cast<T *>(la_raw);
+    T *tL = tM + ff * LA_PROBLEMS;
+    T *tb = tL + ff * LA_PROBLEMS;
+    T *ty = tb + full * LA_PROBLEMS;
+    const int64_t c0 = (int64_t)blockIdx.x * LA_PROBLEMS;
+    const bool rM = op == MF_LA_CHOLESKY || op == MF_LA_SWAP || op == MF_LA_RCADDITION ||
+                    op == MF_LA_INBETWEEN;
+    const bool rL = op != MF_LA_CHOLESKY && op != MF_LA_SWAP;
+    const bool rb = op != MF_LA_CHOLESKY && op != MF_LA_SWAP;
+    const bool ry = op == MF_LA_RCADDITION || op == MF_LA_INBETWEEN;
+    if (rM) la_load(tM, M, ff, c0, n);
+    if (rL) la_load(tL, L, ff, c0, n);
+    if (op == MF_LA_CHOLESKY)   // the output's entries the op leaves alone come back as they were
+        la_load(tL, L, ff, c0, n);
+    if (rb) la_load(tb, b, full, c0, n);
+    if (ry) la_load(ty, y, full, c0, n);
+    if (op == MF_LA_FORWARD || op == MF_LA_BACKWARD) la_load(ty, y, full, c0, n);
+    __syncthreads();
+    const int lane = threadIdx.x;
+    if (lane < LA_PROBLEMS && c0 + lane < n) {
+#define AT(P, r, q) P[((r) * full + (q)) * LA_PROBLEMS + lane]
+#define V(P, i) P[(i) * LA_PROBLEMS + lane]
+        switch (op) {
+        case MF_LA_CHOLESKY:  // cholesky.hpp:8-38
+            for (int i = 0; i < view; ++i) {
+                T sumsq = 0;
+                for (int j = 0; j < i; ++j) {
+                    T s2 = 0;
+                    for (int k = 0; k < j; ++k) s2 += AT(tL, i, k) * AT(tL, j, k);
+                    AT(tL, i, j) = (AT(tM, i, j) - s2) / AT(tL, j, j);
+                    sumsq += AT(tL, i, j) * AT(tL, i, j);
+                }
+                AT(tL, i, i) = sqrt(AT(tM, i, i) - sumsq);
             }
-            AT(L, i, i) = sqrt(AT(M, i, i) - sumsq);
-        }
-        break;
-    case MF_LA_FORWARD:  // substitution.hpp:10-26
-        y[vo] = b[vo] / AT(L, 0, 0);
-        for (int i = 1; i < view; ++i) {
-            T t = b[vo + i];
-            for (int j = 0; j < i; ++j) t -= AT(L, i, j) * y[vo + j];
-            y[vo + i] = t / AT(L, i, i);
-        }
-        break;
-    case MF_LA_BACKWARD:  // substitution.hpp:34-50
-        y[vo + view - 1] = b[vo + view - 1] / AT(L, view - 1, view - 1);
-        for (int i = view - 2; i >= 0; --i) {
-            T t = b[vo + i];
-            for (int j = i + 1; j < view; ++j) t -= AT(L, j, i) * y[vo + j];
-            y[vo + i] = t / AT(L, i, i);
-        }
-        break;
-    case MF_LA_SWAP: {  // common.hpp:11-42 (j > i)
-        const int i = pi, j = pj;
-        T t = AT(M, i, i);
-        AT(M, i, i) = AT(M, j, j);
-        AT(M, j, j) = t;
-        for (int e = 0; e < i; ++e) {
-            T tmp = AT(M, i, e);
-            AT(M, i, e) = AT(M, j, e);
-            AT(M, j, e) = tmp;
-            AT(M, e, i) = AT(M, e, j);
-            AT(M, e, j) = tmp;
-        }
-        for (int e = i + 1; e < view; ++e) {
-            T tmp = AT(M, i, e);
-            AT(M, i, e) = AT(M, j, e);
-            AT(M, j, e) = tmp;
-            AT(M, e, i) = AT(M, e, j);
-            AT(M, e, j) = tmp;
-        }
-        break;
-    }
-    case MF_LA_RCADDITION: {  // fused_cholesky_substitution.hpp:18-57
-        const int row = view - 1;
-        T sum = 0, total = b[vo + row];
-        for (int col = 0; col < row; ++col) {
-            T s2 = 0;
-            for (int j = 0; j < col; ++j) s2 += AT(L, row, j) * AT(L, col, j);
-            const T v = (AT(M, row, col) - s2) / AT(L, col, col);
-            total -= v * y[vo + col];
-            sum += v * v;
-            AT(L, row, col) = v;
-        }
-        const T d = sqrt(AT(M, row, row) - sum);
-        AT(L, row, row) = d;
-        y[vo + row] = total / d;
-        break;
-    }
-    case MF_LA_INBETWEEN:  // fused_cholesky_substitution.hpp:62-102
-        for (int i = pi; i < view; ++i) {
-            T sumsq = 0, total = b[vo + i];
-            for (int j = 0; j < i; ++j) {
-                T s2 = 0;
-                for (int k = 0; k < j; ++k) s2 += AT(L, i, k) * AT(L, j, k);
-                const T v = (AT(M, i, j) - s2) / AT(L, j, j);
-                AT(L, i, j) = v;
-                total -= v * y[vo + j];
-                sumsq += v * v;
+            break;
+        case MF_LA_FORWARD:  // substitution.hpp:10-26
+            V(ty, 0) = V(tb, 0) / AT(tL, 0, 0);
+            for (int i = 1; i < view; ++i) {
+                T t = V(tb, i);
+                for (int j = 0; j < i; ++j) t -= AT(tL, i, j) * V(ty, j);
+                V(ty, i) = t / AT(tL, i, i);
             }
-            const T d = sqrt(AT(M, i, i) - sumsq);
-            AT(L, i, i) = d;
-            y[vo + i] = total / d;
+            break;
+        case MF_LA_BACKWARD:  // substitution.hpp:34-50
+            V(ty, view - 1) = V(tb, view - 1) / AT(tL, view - 1, view - 1);
+            for (int i = view - 2; i >= 0; --i) {
+                T t = V(tb, i);
+                for (int j = i + 1; j < view; ++j) t -= AT(tL, j, i) * V(ty, j);
+                V(ty, i) = t / AT(tL, i, i);
+            }
+            break;
+        case MF_LA_SWAP: {  // common.hpp:11-42 (j > i)
+            const int i = pi, j = pj;
+            T t = AT(tM, i, i);
+            AT(tM, i, i) = AT(tM, j, j);
+            AT(tM, j, j) = t;
+            for (int e = 0; e < i; ++e) {
+                T tmp = AT(tM, i, e);
+                AT(tM, i, e) = AT(tM, j, e);
+                AT(tM, j, e) = tmp;
+                AT(tM, e, i) = AT(tM, e, j);
+                AT(tM, e, j) = tmp;
+            }
+            for (int e = i + 1; e < view; ++e) {
+                T tmp = AT(tM, i, e);
+                AT(tM, i, e) = AT(tM, j, e);
+                AT(tM, j, e) = tmp;
+                AT(tM, e, i) = AT(tM, e, j);
+                AT(tM, e, j) = tmp;
+            }
+            break;
         }
-        break;
-    }
+        case MF_LA_RCADDITION: {  // fused_cholesky_substitution.hpp:18-57
+            const int row = view - 1;
+            T sum = 0, total = V(tb, row);
+            for (int col = 0; col < row; ++col) {
+                T s2 = 0;
+                for (int j = 0; j < col; ++j) s2 += AT(tL, row, j) * AT(tL, col, j);
+                const T v = (AT(tM, row, col) - s2) / AT(tL, col, col);
+                total -= v * V(ty, col);
+                sum += v * v;
+                AT(tL, row, col) = v;
+            }
+            const T d = sqrt(AT(tM, row, row) - sum);
+            AT(tL, row, row) = d;
+            V(ty, row) = total / d;
+            break;
+        }
+        case MF_LA_INBETWEEN:  // fused_cholesky_substitution.hpp:62-102
+            for (int i = pi; i < view; ++i) {
+                T sumsq = 0, total = V(tb, i);
+                for (int j = 0; j < i; ++j) {
+                    T s2 = 0;
+                    for (int k = 0; k < j; ++k) s2 += AT(tL, i, k) * AT(tL, j, k);
+                    const T v = (AT(tM, i, j) - s2) / AT(tL, j, j);
+                    AT(tL, i, j) = v;
+                    total -= v * V(ty, j);
+                    sumsq += v * v;
+                }
+                const T d = sqrt(AT(tM, i, i) - sumsq);
+                AT(tL, i, i) = d;
+                V(ty, i) = total / d;
+            }
+            break;
+        }
+#undef V
 #undef AT
+    }
+    __syncthreads();
+    if (op == MF_LA_SWAP) la_store(M, tM, ff, c0, n);
+    if (op == MF_LA_CHOLESKY || op == MF_LA_RCADDITION || op == MF_LA_INBETWEEN)
+        la_store(L, tL, ff, c0, n);
+    if (op != MF_LA_CHOLESKY && op != MF_LA_SWAP) la_store(y, ty, full, c0, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -2076,15 +2141,18 @@ extern "C" mf_status mf_linalg_batch(int32_t op, int32_t dtype, void *M, void *L
     const bool needv = op != MF_LA_CHOLESKY && op != MF_LA_SWAP;
     if ((needM && !M) || (needL && !L) || (needv && (!b || !y)))
         return fail(MF_ERR_INVALID, "missing operand for op %d", op);
-    const int block = 64;
-    const dim3 grid((unsigned)((n + block - 1) / block));
+    const dim3 grid((unsigned)((n + LA_PROBLEMS - 1) / LA_PROBLEMS));
+    const dim3 block(LA_THREADS);
     hipStream_t st = static_cast<hipStream_t>(stream);
+    // LDS tiles: M and L (full^2 each) and b, y (full each) for 32 problems
+    const size_t elems = (size_t)(2 * full * full + 2 * full) * LA_PROBLEMS;
     if (dtype == MF_DTYPE_F64)
-        hipLaunchKernelGGL(k_linalg<double>, grid, dim3(block), 0, st, op,
+        hipLaunchKernelGGL(k_linalg<double>, grid, block, elems * sizeof(double), st, op,
                            static_cast<double *>(M), static_cast<double *>(L),
                            static_cast<double *>(b), static_cast<double *>(y), n, full, view, i, j);
     else if (dtype == MF_DTYPE_F32)
-        hipLaunchKernelGGL(k_linalg<float>, grid, dim3(block), 0, st, op, static_cast<float *>(M),
+        hipLaunchKernelGGL(k_linalg<float>, grid, block, elems * sizeof(float), st, op,
+                           static_cast<float *>(M),
                            static_cast<float *>(L), static_cast<float *>(b),
                            static_cast<float *>(y), n, full, view, i, j);
     else

@@ -172,3 +172,91 @@ def test_linalg_batch(pkg, gpu, dtype):
                        rtol=10 * tol, atol=10 * tol)
     assert np.allclose(y7.double().cpu().numpy()[:, :view], y.double().cpu().numpy()[:, :view],
                        rtol=10 * tol, atol=10 * tol)
+
+
+@pytest.mark.parametrize("sfx,tol", [("f64", 1e-13), ("f32", 2e-5)])
+def test_linalg_batch_golden_every_op(pkg, gpu, sfx, tol):
+    """Every mf_linalg_batch op against the golden vectors written by the
+    reference's own linalgebra headers (tests/golden/make_golden.py): the 3x3
+    literal of test_cholesky_with_solvers.cpp, seeded 10x10 and 7-of-10 SPD
+    factors and solves, the symmetric swap, and the sequence of
+    generate_and_test_fused_impl.cpp:66-119 -- factor 9x9, append row 9
+    (rcaddition), swap rows 6 and 9 and refactor from position 6
+    (inbetween_removal).  The same operation order; agreement to a few ulps
+    (the GPU contracts multiply-adds into FMAs)."""
+    from pathlib import Path
+    from test_regression_amd import engine, _abi
+    g = np.load(Path(__file__).resolve().parent / "golden" / "linalgebra_golden.npz")
+    dt = torch.float64 if sfx == "f64" else torch.float32
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dt).cuda()  # noqa: E731
+    H = lambda t: t.double().cpu().numpy()  # noqa: E731
+
+    def close(got, want, what):
+        sc = max(1.0, np.abs(want).max())
+        err = np.abs(got - want).max() / sc
+        assert err <= tol, (what, err)
+
+    print(f"[gpu-linalg] golden {sfx}", flush=True)
+    # 3x3 literal: factor, forward, backward
+    M3, b3 = T(g[f"lit3_M_{sfx}"][None]), T(g[f"lit3_b_{sfx}"][None])
+    L3, y3, x3 = T(np.zeros((1, 3, 3))), T(np.zeros((1, 3))), T(np.zeros((1, 3)))
+    engine.linalg_batch(_abi.MF_LA_CHOLESKY, M=M3, L=L3, full=3, view=3)
+    engine.linalg_batch(_abi.MF_LA_FORWARD, L=L3, b=b3, y=y3, full=3, view=3)
+    engine.linalg_batch(_abi.MF_LA_BACKWARD, L=L3, b=y3, y=x3, full=3, view=3)
+    close(np.tril(H(L3)[0]), np.tril(g[f"lit3_L_{sfx}"]), "lit3 L")
+    close(H(y3)[0], g[f"lit3_y_{sfx}"], "lit3 y")
+    close(H(x3)[0], g[f"lit3_x_{sfx}"], "lit3 x")
+    # seeded SPD, views 10 and 7 (one call per view)
+    Ms, bs, views = g[f"spd_M_{sfx}"], g[f"spd_b_{sfx}"], g["spd_view"]
+    for v in (10, 7):
+        sel = np.flatnonzero(views == v)
+        M, b = T(Ms[sel]), T(bs[sel])
+        L, y, x = T(np.zeros((sel.size, 10, 10))), T(np.zeros((sel.size, 10))), \
+            T(np.zeros((sel.size, 10)))
+        engine.linalg_batch(_abi.MF_LA_CHOLESKY, M=M, L=L, full=10, view=v)
+        engine.linalg_batch(_abi.MF_LA_FORWARD, L=L, b=b, y=y, full=10, view=v)
+        engine.linalg_batch(_abi.MF_LA_BACKWARD, L=L, b=y, y=x, full=10, view=v)
+        close(np.tril(H(L))[:, :v, :v], np.tril(g[f"spd_L_{sfx}"][sel])[:, :v, :v], f"spd L {v}")
+        close(H(y)[:, :v], g[f"spd_y_{sfx}"][sel][:, :v], f"spd y {v}")
+        close(H(x)[:, :v], g[f"spd_x_{sfx}"][sel][:, :v], f"spd x {v}")
+    # swap (i, j) per problem: one call per pair; bit for bit (moves only)
+    pairs = g["swap_pairs"]
+    for (i, j) in {tuple(p) for p in pairs}:
+        sel = np.flatnonzero((pairs[:, 0] == i) & (pairs[:, 1] == j))
+        M = T(Ms[sel])
+        engine.linalg_batch(_abi.MF_LA_SWAP, M=M, full=10, view=10, i=int(i), j=int(j))
+        assert np.array_equal(H(M), g[f"swap_M_{sfx}"][sel].astype(np.float64)), (i, j)
+    # fused: factor 9, forward 9, append row 9
+    n = Ms.shape[0]
+    M, b = T(Ms), T(bs)
+    L, y = T(np.zeros((n, 10, 10))), T(np.zeros((n, 10)))
+    engine.linalg_batch(_abi.MF_LA_CHOLESKY, M=M, L=L, full=10, view=9)
+    engine.linalg_batch(_abi.MF_LA_FORWARD, L=L, b=b, y=y, full=10, view=9)
+    engine.linalg_batch(_abi.MF_LA_RCADDITION, M=M, L=L, b=b, y=y, full=10, view=10)
+    close(np.tril(H(L)), np.tril(g[f"fused_L_{sfx}"]), "rcaddition L")
+    close(H(y), g[f"fused_y_{sfx}"], "rcaddition y")
+    # removal: swap rows/cols 6 and 9 of M (and of b), refactor from 6 in view 9
+    Mr = T(Ms)
+    engine.linalg_batch(_abi.MF_LA_SWAP, M=Mr, full=10, view=10, i=6, j=9)
+    assert np.array_equal(H(Mr), g[f"removal_M_{sfx}"].astype(np.float64))
+    br = bs.copy()
+    br[:, [6, 9]] = br[:, [9, 6]]
+    assert np.array_equal(br, g[f"removal_b_{sfx}"])
+    engine.linalg_batch(_abi.MF_LA_INBETWEEN, M=Mr, L=L, b=T(br), y=y, full=10, view=9, i=6)
+    close(np.tril(H(L))[:, :9, :9], np.tril(g[f"removal_L_{sfx}"])[:, :9, :9], "removal L")
+    close(H(y)[:, :9], g[f"removal_y_{sfx}"][:, :9], "removal y")
+
+
+def test_linalg_batch_many_problems_and_partial_blocks(pkg, gpu):
+    """Problem counts around the 32-problem blocks (1, 31, 33, 1000): every
+    problem's factor equals numpy's, none is skipped or mixed up."""
+    from test_regression_amd import engine, _abi
+    rng = np.random.default_rng(9)
+    for n in (1, 31, 33, 1000):
+        Ms = np.stack([_spd(rng, 10) for _ in range(n)])
+        M = torch.from_numpy(Ms).cuda()
+        L = torch.zeros_like(M)
+        engine.linalg_batch(_abi.MF_LA_CHOLESKY, M=M, L=L, full=10, view=10)
+        Lg = L.cpu().numpy()
+        Lr = np.linalg.cholesky(Ms)
+        assert np.allclose(np.tril(Lg), Lr, rtol=1e-12, atol=1e-12), n
