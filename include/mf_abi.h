@@ -132,7 +132,11 @@ enum {
     MF_F_WARM_REJECT  = 32,  /* warm start failed its KKT check; the active-set
                                 iterations went on from it                    */
     MF_F_REFINED      = 64,  /* FP32 path: FP64 refinement step applied       */
-    MF_F_BAD_TYPE     = 128  /* type_of[c] >= ntypes: not fitted, x = 0       */
+    MF_F_BAD_TYPE     = 128, /* type_of[c] >= ntypes: not fitted, x = 0       */
+    MF_F_WARM_FALLBACK = 256 /* the iterations from a non-empty prediction
+                                reached max_iter: the prediction was dropped
+                                and the channel fitted from a cold start
+                                (x = 0; an empty prediction is that start)    */
 };
 
 /* Device-resident batch, structure of arrays.  samples[i*nch + c] for sample

@@ -25,7 +25,7 @@ MF_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 FLAGS = {
     "minimize_cap": 1, "llt_fail": 2, "fnnls_cap": 4, "nonfinite": 8,
-    "fp64_pivot": 16, "warm_reject": 32, "refined": 64, "bad_type": 128,
+    "fp64_pivot": 16, "warm_reject": 32, "refined": 64, "bad_type": 128, "warm_fallback": 256,
 }
 
 

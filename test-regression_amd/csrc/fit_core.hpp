@@ -202,7 +202,8 @@ MF_HD bool rho_ok(double r) { return r > 0.0 && r <= 1.7976931348623157e308; }
 
 enum {
     F_MINIMIZE_CAP = 1, F_LLT_FAIL = 2, F_FNNLS_CAP = 4, F_NONFINITE = 8,
-    F_FP64_PIVOT = 16, F_WARM_REJECT = 32, F_REFINED = 64, F_BAD_TYPE = 128
+    F_FP64_PIVOT = 16, F_WARM_REJECT = 32, F_REFINED = 64, F_BAD_TYPE = 128,
+    F_WARM_FALLBACK = 256
 };
 
 struct FnnlsStats {
@@ -1435,6 +1436,18 @@ MF_HD void fast_phase1(const DevType &ty, const float (&s32)[10], const FitParam
     if (ill) flags |= F_FP64_PIVOT;
 }
 
+// The fallback of a warm start: the state of a cold start (x = 0, nothing
+// passive), with the adds counted afresh.
+MF_HD void warm_fallback_reset(ActiveState<double> &S, FnnlsStats &st)
+{
+#pragma unroll
+    for (int i = 0; i < 10; ++i) S.x[i] = 0.0;
+    S.pm = 0;
+    S.solve = false;
+    S.adds = 0;
+    st.hit_max = false;
+}
+
 // Start of phase 2 for one channel: the FP64 right side Atb = G0 s and the
 // FP32 phase's amplitudes as the starting state (passive set = entries > 0).
 // Returns false for a channel that is not fitted (non-finite samples, or
@@ -1517,6 +1530,7 @@ MF_HD void fast_phase2(const DevType &ty, float (&s32)[10], const FitParams &prm
     reload(s32);
     if (fast_p2_begin<NS, NP>(ty, s32, prm, x, Atb64, S, flags)) {
         const int outer0 = st.outer;
+        const bool from_empty = S.pm == 0;   // a start that is the cold start
         const MaskTab *tab = *opaque(&ty.tab);   // wave-uniform: one scalar load
         const double eps = prm.eps * prm.rho;
         while (!fnnls_step<double, NP, true, TAB>(ty.d.AtA0, Atb64, S, eps, prm.eps_rel64,
@@ -1524,6 +1538,15 @@ MF_HD void fast_phase2(const DevType &ty, float (&s32)[10], const FitParams &prm
         }
         flags |= F_REFINED;
         if (warm && st.outer != outer0) flags |= F_WARM_REJECT;
+        if (warm && st.hit_max && !from_empty) {
+            // The cold start is the fallback: a prediction whose iterations
+            // reach max_iter is dropped and the channel fitted from x = 0.
+            warm_fallback_reset(S, st);
+            flags |= F_WARM_FALLBACK;
+            while (!fnnls_step<double, NP, true, TAB>(ty.d.AtA0, Atb64, S, eps, prm.eps_rel64,
+                                                      prm.max_iter, st, 0.0, nullptr, tab)) {
+            }
+        }
         reload(s32);
         chi2 = fast_p2_chi2<NS, NP>(ty, s32, S.x) / prm.rho;
         // With a live pulse covariance the later iterations run the

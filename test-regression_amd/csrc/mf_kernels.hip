@@ -572,6 +572,9 @@ void k_fast_p2q(const DevType *__restrict__ types, int type_id, Seg seg,
     ActiveState<double> S;
     FnnlsStats st;
     int flags = 0, outer0 = 0;
+    // 0: the start is a prediction, 1: it fell back to x = 0, 2: it was the
+    // empty set (the cold start: nothing to fall back to)
+    int restart = 0;
     float svn[10], xn[10];
     int fn = 0, itn = 0;
     QueueCursor q;
@@ -606,7 +609,9 @@ void k_fast_p2q(const DevType *__restrict__ types, int type_id, Seg seg,
             st.llt_fail = (fn & F_LLT_FAIL) != 0;
             outer0 = st.outer;
             flags = fn & F_FP64_PIVOT;
-            if (!fast_p2_begin<N, N>(ty, svn, prm, xn, Atb, S, flags)) {
+            const bool begun = fast_p2_begin<N, N>(ty, svn, prm, xn, Atb, S, flags);
+            restart = S.pm == 0 ? 2 : 0;
+            if (!begun) {
 #pragma unroll
                 for (int p = 0; p < 10; ++p) xo[(int64_t)p * nch + c] = 0.0f;
                 chi2o[c] = 0.0f;
@@ -617,9 +622,11 @@ void k_fast_p2q(const DevType *__restrict__ types, int type_id, Seg seg,
         }
         if (c >= 0 &&
             fnnls_step<double, N, true, true>(ty.d.AtA0, Atb, S, prm.eps * prm.rho, prm.eps_rel64,
-                                              prm.max_iter, st, 0.0, nullptr, *opaque(&ty.tab))) {
+                                              prm.max_iter, st, 0.0, nullptr, *opaque(&ty.tab)) &&
+            !(warm && st.hit_max && restart == 0)) {
             flags |= F_REFINED;
             if (warm && st.outer != outer0) flags |= F_WARM_REJECT;
+            if (restart == 1) flags |= F_WARM_FALLBACK;
             // the samples are read again here rather than held through the steps
             float sv[10];
 #pragma unroll
@@ -634,6 +641,11 @@ void k_fast_p2q(const DevType *__restrict__ types, int type_id, Seg seg,
             flo[c] = flags;
             ito[c] = nmin | (st.outer << 8) | (st.inner << 16);
             c = -1;
+        }
+        // a warm start whose iterations reached max_iter starts over cold
+        if (c >= 0 && warm && st.hit_max && restart == 0) {
+            warm_fallback_reset(S, st);
+            restart = 1;
         }
         if (q.exhausted(nitems) && __ballot(c >= 0 || cn >= 0) == 0) break;
     }

@@ -158,6 +158,55 @@ extern "C" int hc_fit_batch(const mf_chtype *types, int ntypes, const uint8_t *t
                          err);
 }
 
+// The fast path from given starting amplitudes xs[10][nch] (passive set =
+// entries > 0), as a warm start (warm = 1: straight to the FP64 phase) or
+// cold (warm = 0: xs ignored, the FP32 phase first).
+extern "C" int hc_fast_from(const mf_chtype *types, int ntypes, const uint8_t *type_of,
+                            const double *samples, const double *xs, int64_t nch, int warm,
+                            const mf_opts *o, double *x, double *chi2, int32_t *flags,
+                            int32_t *iters)
+{
+    std::vector<DevType> dt((size_t)ntypes);
+    std::vector<MaskTab> tabs((size_t)ntypes);
+    for (int t = 0; t < ntypes; ++t) {
+        if (expand_type(types[t], dt[(size_t)t])) return 1;
+        build_mask_tab(dt[(size_t)t], tabs[(size_t)t]);
+        dt[(size_t)t].tab = &tabs[(size_t)t];
+    }
+    FitParams prm0;
+    prm0.eps = o->eps;
+    prm0.chi2_tol = o->chi2_tol;
+    prm0.max_iter = o->max_iter;
+    prm0.max_minimize = o->max_minimize;
+    prm0.use_pi = o->use_pulse_cov;
+    prm0.solver = o->solver;
+    prm0.eps_rel32 = eps_rel32;
+    prm0.tau32 = tau32;
+    prm0.eps_rel64 = eps_rel64;
+    for (int64_t c = 0; c < nch; ++c) {
+        const DevType &d = dt[type_of ? type_of[c] : 0];
+        if (d.h.ns != 10 || (d.h.has_pi && o->use_pulse_cov)) return 2;
+        FitParams prm = prm0;
+        prm.pmask = d.h.pmask;
+        prm.use_pi = 0;
+        float s[10], x0[10];
+        uint32_t pm = 0;
+        for (int i = 0; i < 10; ++i) {
+            s[i] = (float)samples[i * nch + c];
+            x0[i] = warm && xs[i * nch + c] > 0 ? (float)xs[i * nch + c] : 0.0f;
+            if (x0[i] > 0.0f) pm |= 1u << i;
+        }
+        RegPark<double> park;
+        FitOut<float> out;
+        fit_fast<10, 10, false, true>(d, s, prm, x0, pm, warm != 0, park, out);
+        for (int p = 0; p < 10; ++p) x[p * nch + c] = (double)out.x[p];
+        chi2[c] = (double)out.chi2;
+        flags[c] = out.flags;
+        iters[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
+    }
+    return 0;
+}
+
 // single NNLS problems (A column-major ld 10, m x m) through fit_core's solvers
 // (0 inplace_fnnls, 1 fnnls, 2 Lawson-Hanson with max_iter as a signed
 // solve count, 3 inplace_fnnls_test_cholesky).  Returns 0 when Lawson-Hanson stopped at max_iter, else 1.

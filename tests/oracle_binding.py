@@ -123,6 +123,9 @@ def hostcore():
     L.hc_fit_batch2.restype = C.c_int
     L.hc_nnls.argtypes = [dp, dp, dp, C.c_int, C.c_int, C.c_double, C.c_uint]
     L.hc_nnls.restype = C.c_int
+    L.hc_fast_from.argtypes = [C.c_void_p, C.c_int, C.c_void_p, dp, dp, C.c_int64, C.c_int,
+                               C.c_void_p, dp, dp, C.c_void_p, C.c_void_p]
+    L.hc_fast_from.restype = C.c_int
     L.hc_mask_inverse.argtypes = [C.c_void_p, C.c_int, dp, dp]
     L.hc_mask_inverse.restype = C.c_int
     L.hc_set_use_tables.argtypes = [C.c_int]
@@ -194,4 +197,23 @@ def hostcore_fit(chtypes, samples: np.ndarray, type_of, opts, pederr=None):
     assert rc == 0
     if opts.compute_errors:
         return x, chi2, flags, iters, err
+    return x, chi2, flags, iters
+
+
+def hostcore_fast_from(chtypes, samples: np.ndarray, xstart: np.ndarray, warm: bool, opts):
+    """The fast path of fit_core.hpp's host build from given starting
+    amplitudes (warm start: passive set = xstart > 0)."""
+    L = hostcore()
+    n = samples.shape[1]
+    samples = np.ascontiguousarray(samples, dtype=np.float64)
+    xs = np.ascontiguousarray(xstart, dtype=np.float64)
+    x = np.zeros((10, n))
+    chi2 = np.zeros(n)
+    flags = np.zeros(n, dtype=np.int32)
+    iters = np.zeros(n, dtype=np.int32)
+    rc = L.hc_fast_from(C.cast(chtypes, C.c_void_p), len(chtypes), None, dptr(samples), dptr(xs),
+                        n, 1 if warm else 0, C.cast(C.byref(opts), C.c_void_p), dptr(x),
+                        dptr(chi2), flags.ctypes.data_as(C.c_void_p),
+                        iters.ctypes.data_as(C.c_void_p))
+    assert rc == 0, rc
     return x, chi2, flags, iters
