@@ -49,6 +49,15 @@ MEASURED_TRAFFIC = {("c2", "zero", 1_000_000): {
                              "bench.py --traffic-probe, copy-calibrated"}}
 
 
+# What "achieved" counts: the plain method's operations, counted by the
+# oracle on a sample (the fast path reaches the same optimum with fewer).
+ACHIEVED_COUNTS = ("algorithmic flops of the plain method per channel (about 10.6 solves and "
+                   "two Minimize iterations at zero pulse covariance, counted by the oracle) x "
+                   "channels / kernel time: a rate of useful fit work. The warm kernel reaches "
+                   "the same optimum in about 1.4 solves and needs no second iteration, so it "
+                   "issues far fewer flops (issued-instruction counts are under profiles/)")
+
+
 # Forward pass of the warm-start MLP per channel (10 -> 64 -> 64 -> 20): MFMA
 # (bfloat16) work inside the warm fit kernel, not counted in the VALU
 # roofline's achieved figure.
@@ -166,17 +175,24 @@ def oracle_sample(ct, s, t, n):
                       oracle_opts(count_flops=1), nthreads=min(16, os.cpu_count() or 1))
 
 
-def errors(x, chi2, xr, chi2r):
-    """Relative errors (channel max amplitude floor 1 ADC, chi2 floor 1) and
-    the fraction of channels with identical active sets."""
-    scale = np.maximum(np.abs(xr).max(axis=0), 1.0)
-    e = (np.abs(x - xr) / scale).max(axis=0)
+def errors(x, chi2, xr, chi2r, floor=1.0):
+    """One path against another over the whole batch: the per-amplitude
+    relative error |x - xr| / max(|xr|, floor) (floor 1 ADC), the channel
+    error relative to the channel's largest amplitude (same floor), chi2's
+    relative error (floor 1), and the counts above 1e-4 and of channels whose
+    active sets differ."""
+    d = np.abs(x - xr)
+    a = (d / np.maximum(np.abs(xr), floor)).max(axis=0)
+    e = d.max(axis=0) / np.maximum(np.abs(xr).max(axis=0), floor)
     c = np.abs(chi2 - chi2r) / np.maximum(chi2r, 1.0)
     aset = ((x > 0) == (xr > 0)).all(axis=0)
-    return {"amp_rel_err_max": float(e.max()), "amp_rel_err_p999": float(np.quantile(e, 0.999)),
-            "frac_amp_within_1e-4": float((e <= 1e-4).mean()),
-            "chi2_rel_err_max": float(c.max()), "chi2_rel_err_p999": float(np.quantile(c, 0.999)),
-            "identical_active_sets": float(aset.mean()), "channels": int(x.shape[1])}
+    return {"amp_rel_err_max": float(a.max()), "amp_rel_err_p999": float(np.quantile(a, 0.999)),
+            "amp_above_1e-4": int((a > 1e-4).sum()),
+            "channel_rel_err_max": float(e.max()), "channel_above_1e-4": int((e > 1e-4).sum()),
+            "chi2_rel_err_max": float(c.max()), "chi2_above_1e-4": int((c > 1e-4).sum()),
+            "different_active_sets": int((~aset).sum()),
+            "identical_active_sets": float(aset.mean()), "channels": int(x.shape[1]),
+            "amplitude_floor_adc": floor}
 
 
 def event_stream(args, eng, w, log):
@@ -186,7 +202,7 @@ def event_stream(args, eng, w, log):
     A pool of `event_pool` distinct generated events stays resident in HBM
     and is cycled (event i uses pool slot i mod pool)."""
     import torch
-    from test_regression_amd import _abi, warmstart
+    from test_regression_amd import _abi, engine, warmstart
     P, E = args.event_pool, args.events
     nev = sum(w.event)
     t0 = time.perf_counter()
@@ -222,13 +238,15 @@ def event_stream(args, eng, w, log):
         lat = np.array([x.elapsed_time(y) for x, y in zip(st, en)])
         return a.elapsed_time(b), lat
 
-    def run_inflight(k):
+    def run_inflight(k, gk):
         """Event i replayed on stream i mod k (default priority), so that k
-        events are in flight; k divides the pool size, so a graph (and its
-        output buffers) is only ever replayed on one stream."""
+        events are in flight.  The graphs gk of stream j were captured on a
+        context of their own (a context's scratch serves one call at a
+        time), and k divides the pool size, so a graph and its output buffers
+        are only ever replayed on one stream."""
         streams = [torch.cuda.Stream() for _ in range(k)]
         for i in range(min(P, E)):
-            graphs[i % P].launch(streams[i % k])
+            gk[i % P].launch(streams[i % k])
         torch.cuda.synchronize()
         st = [torch.cuda.Event(enable_timing=True) for _ in range(E)]
         en = [torch.cuda.Event(enable_timing=True) for _ in range(E)]
@@ -239,7 +257,7 @@ def event_stream(args, eng, w, log):
         for i in range(E):
             sm = streams[i % k]
             st[i].record(sm)
-            graphs[i % P].launch(sm)
+            gk[i % P].launch(sm)
             en[i].record(sm)
         for sm in streams:
             d = torch.cuda.Event()
@@ -255,9 +273,26 @@ def event_stream(args, eng, w, log):
     log(f"graph: {total:.1f} ms for {E} events, p50 {np.median(lat):.4f} ms, "
         f"p99 {np.quantile(lat, 0.99):.4f} ms; eager: {total_e:.1f} ms")
     K_INFLIGHT = 4 if P % 4 == 0 else 1
-    total_k, lat_k = run_inflight(K_INFLIGHT)
+    # one context per stream, same types, network and launch shapes
+    engs = [eng] + [engine.Engine(list(w.chtypes())) for _ in range(K_INFLIGHT - 1)]
+    for e in engs[1:]:
+        e.mlp_set_params(eng.mlp_get_params())
+        e.set_tune(**eng.get_tune())
+    outs_k = [eng.alloc_result(nev, _abi.MF_PREC_FP32) for _ in range(P)]
+    graphs_k = [engs[e % K_INFLIGHT].capture(pool[e], None, o, outs_k[e], type_ranges=ranges)
+                for e in range(P)]
+    total_k, lat_k = run_inflight(K_INFLIGHT, graphs_k)
+    # every event's in-flight outputs against its one-at-a-time outputs
+    inflight_equal = all(torch.equal(outs_k[e].x, outs[e].x) and
+                         torch.equal(outs_k[e].chi2, outs[e].chi2) and
+                         torch.equal(outs_k[e].flags, outs[e].flags) for e in range(P))
     log(f"graph, {K_INFLIGHT} events in flight: {total_k:.1f} ms for {E} events, "
-        f"p50 {np.median(lat_k):.4f} ms, p99 {np.quantile(lat_k, 0.99):.4f} ms")
+        f"p50 {np.median(lat_k):.4f} ms, p99 {np.quantile(lat_k, 0.99):.4f} ms; outputs "
+        f"{'identical' if inflight_equal else 'DIFFERENT'} to one at a time")
+    if not inflight_equal:
+        raise SystemExit("in-flight outputs differ from one-at-a-time outputs")
+    for g in graphs_k:
+        g.close()
     # accuracy of one event against the plain FP64 fit
     ref = eng.fit(pool[0].double(), None, _abi.default_opts(precision=_abi.MF_PREC_FP64),
                   type_ranges=ranges)
@@ -292,7 +327,9 @@ def event_stream(args, eng, w, log):
                    "events": E, "channels_per_event": nev, "event_pool": P},
         "latency_ms": {"p50": float(np.median(lat)), "p99": float(np.quantile(lat, 0.99)),
                        "mean": float(lat.mean()), "max": float(lat.max())},
-        "inflight": {"events_in_flight": K_INFLIGHT, "channels_per_s": E * nev / (total_k / 1e3),
+        "inflight": {"events_in_flight": K_INFLIGHT, "contexts": K_INFLIGHT,
+                     "outputs_identical_to_one_at_a_time": inflight_equal,
+                     "channels_per_s": E * nev / (total_k / 1e3),
                      "latency_ms_p50": float(np.median(lat_k)),
                      "latency_ms_p99": float(np.quantile(lat_k, 0.99))},
         "eager": {"channels_per_s": E * nev / (total_e / 1e3),
@@ -341,6 +378,8 @@ def main():
                     help="for counter runs (rocprofv3 --pmc): a 256 MiB device copy, the "
                          "calibration of the byte counters, then --steps steps of each path "
                          "of --probe-paths; no timing, no JSON")
+    ap.add_argument("--no-live", dest="live", action="store_false",
+                    help="skip the live-pulse-covariance block of the default workload")
     ap.add_argument("--probe-paths", default="warm,fast",
                     help="paths of --traffic-probe, in order: any of plain, fast, warm")
     args = ap.parse_args()
@@ -400,63 +439,28 @@ def main():
     auto = dict(block=0, persistent=_abi.MF_PERSISTENT_AUTO)
     paths = {
         "plain": (s64, O(precision=_abi.MF_PREC_FP64, block=0), _abi.MF_PREC_FP64),
+        "plain_chol": (s64, O(precision=_abi.MF_PREC_FP64, block=0,
+                              solver=_abi.MF_SOLVER_INPLACE_FNNLS_CHOL), _abi.MF_PREC_FP64),
         "fast": (s32, O(precision=_abi.MF_PREC_FP32, **auto), _abi.MF_PREC_FP32),
         "warm": (s32, O(precision=_abi.MF_PREC_FP32, warm_start=1, **auto), _abi.MF_PREC_FP32),
     }
-    res, outs, tuned = {}, {}, {}
-    default_tune = eng.get_tune()
-    for name, (x_in, o, prec) in paths.items():
-        out = eng.alloc_result(nch, prec)
-        eng.set_tune(**default_tune)
-        if args.autotune:
-            # launch shape chosen for this path on this batch (mf_autotune)
-            t0 = time.perf_counter()
-            tune, tms = eng.autotune(x_in, tt, o, out)
-            tuned[name] = {**tune, "ms": tms, "seconds": time.perf_counter() - t0}
-            log(f"autotune {name}: {tuned[name]}")
-        ms, wall, kern = time_steps(eng, lambda: eng.fit(x_in, tt, o, out), args.steps,
-                                    args.warmup, stream)
-        res[name] = {"ms": ms, "wall_ms": wall,
-                     "fit_ms": float(np.mean([k["fit_ms"] for k in kern])),
-                     "mlp_ms": float(np.mean([k["mlp_ms"] for k in kern]))}
-        outs[name] = out
-        log(f"{name}: {res[name]['ms']:.3f} ms/step (host clock {wall:.3f}; fit kernels "
-            f"{res[name]['fit_ms']:.3f} ms, mlp {res[name]['mlp_ms']:.3f} ms) -> "
-            f"{nch / (res[name]['ms'] / 1e3):.3e} ch/s")
-    torch.cuda.synchronize()
+    res, outs, tuned, graph_ms = time_paths(eng, paths, tt, nch, args, stream)
 
-    # the warm call captured as a graph (mf_graph_capture runs it once eagerly
-    # first, into the same outputs) and replayed K times, timed as above
-    g = eng.capture(paths["warm"][0], tt, paths["warm"][1], outs["warm"])
-    for _ in range(args.warmup):
-        g.launch()
-    torch.cuda.synchronize()
-    ga, gb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    gate_stream()
-    ga.record(stream)
-    for _ in range(args.steps):
-        g.launch()
-    gb.record(stream)
-    torch.cuda.synchronize()
-    graph_ms = ga.elapsed_time(gb) / args.steps
-    g.close()
-    log(f"warm, graph replays: {graph_ms:.3f} ms/step")
-
-    # accuracy: warm (and cold) fast path vs the plain FP64 fit on the whole batch
-    xp = outs["plain"].x.cpu().numpy()
-    cp = outs["plain"].chi2.cpu().numpy()
-    acc = {k: errors(outs[k].x.double().cpu().numpy(), outs[k].chi2.double().cpu().numpy(), xp, cp)
-           for k in ("warm", "fast")}
-    # ... and vs the FP64 oracle on a sample; the same sample gives the flop count
+    # accuracy, channel by channel, over the whole batch
+    threads = min(16, os.cpu_count() or 1)
+    o_np = {k: np_out(v) for k, v in outs.items()}
+    acc = {k: errors(*o_np[k][:2], *o_np["plain"][:2]) for k in ("warm", "fast")}
+    acc.update(oracle_accounting(ct, s, t, o_np, threads, eng, s64, tt, paths))
+    # the same sample gives the flop count
     orc = oracle_sample(ct, s.astype(np.float32).astype(np.float64), t, args.oracle_sample)
-    n_o = len(orc)
-    acc["warm_vs_oracle"] = errors(outs["warm"].x.double().cpu().numpy()[:, :n_o],
-                                   outs["warm"].chi2.double().cpu().numpy()[:n_o],
-                                   orc["x"].T, orc["chi2"])
     af = float(orc["aflops"].mean())
-    it_w = outs["warm"].iters.cpu().numpy()
-    it_f = outs["fast"].iters.cpu().numpy()
+    it_w = o_np["warm"][3]
+    it_f = o_np["fast"][3]
     mlp_eval = warmstart.evaluate(eng, s32, outs["plain"].x.float())
+    fl_w = o_np["warm"][2]
+    o_chol, o_plain = o_np["plain_chol"], o_np["plain"]
+    del outs, o_np
+    torch.cuda.empty_cache()
 
     ms = res["warm"]["ms"]
     value = nch / (ms / 1e3)
@@ -489,11 +493,18 @@ def main():
                      "kernel": fit_kernel_name(args.pulse_cov),
                      "kernel_ms": kern_ms,
                      "aflops_per_channel": af,
+                     "achieved_counts": ACHIEVED_COUNTS,
                      "mlp_mfma_flops_per_channel_not_counted": MLP_FLOPS_PER_CHANNEL},
         "plain_fp64": {"channels_per_s": nch / (res["plain"]["ms"] / 1e3),
-                       "ms_per_step": res["plain"]["ms"]},
+                       "ms_per_step": res["plain"]["ms"], "kernel_ms": res["plain"]["fit_ms"],
+                       "roofline_frac_fp64": af * nch / (res["plain"]["fit_ms"] / 1e3) / 1e12 /
+                       PEAK_TFLOPS["f64"]},
+        "plain_fp64_incremental_cholesky": {
+            "solver": "MF_SOLVER_INPLACE_FNNLS_CHOL (passive factor kept across solves)",
+            "ms_per_step": res["plain_chol"]["ms"], "kernel_ms": res["plain_chol"]["fit_ms"],
+            "vs_plain": errors(o_chol[0], o_chol[1], o_plain[0], o_plain[1])},
         "fast_cold_fp32": {"channels_per_s": nch / (res["fast"]["ms"] / 1e3),
-                           "ms_per_step": res["fast"]["ms"]},
+                           "ms_per_step": res["fast"]["ms"], "kernel_ms": res["fast"]["fit_ms"]},
         "warm_graph_ms_per_step": graph_ms,
         "speedup_warm_vs_plain_fp64": res["plain"]["ms"] / res["warm"]["ms"],
         "speedup_cold_vs_plain_fp64": res["plain"]["ms"] / res["fast"]["ms"],
@@ -502,11 +513,13 @@ def main():
         "accuracy": acc,
         "solves_per_channel": {"warm": float(((it_w >> 16) & 0xFFFF).mean()),
                                "cold": float(((it_f >> 16) & 0xFFFF).mean())},
-        "warm_start_rejected": float(((outs["warm"].flags.cpu().numpy() & 32) != 0).mean()),
+        "warm_start_rejected": float(((fl_w & _abi.FLAGS["warm_reject"]) != 0).mean()),
         "mlp": {"eval_on_batch": mlp_eval, **{k: v for k, v in mlp_info.items() if k != "report"}},
     }
     if "report" in mlp_info:
         line["mlp"]["train"] = mlp_info["report"]
+    if args.live and args.pulse_cov == "zero" and args.config == "c2":
+        line["live"] = live_block(args, stream, threads)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         cb = cpu_baseline(ct, s, t, args.cpu_seconds, threads)
@@ -516,6 +529,197 @@ def main():
                         f"the first {cb['slice_1t']} channels on one thread")
         line["cpu_baseline"] = cb
     print(json.dumps(line), flush=True)
+
+
+def time_paths(eng, paths, tt, nch, args, stream, graph_path="warm"):
+    """Each path of `paths` (name -> samples, opts, output precision):
+    launch shape chosen by mf_autotune for it on this batch (unless
+    disabled), then timed (time_steps); the graph_path call is also captured
+    as a graph (mf_graph_capture runs it once eagerly first, into the same
+    outputs) and replayed, timed the same way.  Returns the timings, the
+    outputs, the tunes and the graph's ms per step."""
+    import torch
+    res, outs, tuned = {}, {}, {}
+    default_tune = eng.get_tune()
+    nsteps = len(paths)
+    for name, (x_in, o, prec) in paths.items():
+        out = eng.alloc_result(nch, prec)
+        eng.set_tune(**default_tune)
+        if args.autotune:
+            t0 = time.perf_counter()
+            tune, tms = eng.autotune(x_in, tt, o, out)
+            tuned[name] = {**tune, "ms": tms, "seconds": time.perf_counter() - t0}
+            log(f"autotune {name}: {tuned[name]}")
+        ms, wall, kern = time_steps(eng, lambda: eng.fit(x_in, tt, o, out), args.steps,
+                                    args.warmup, stream)
+        res[name] = {"ms": ms, "wall_ms": wall,
+                     "fit_ms": float(np.mean([k["fit_ms"] for k in kern])),
+                     "mlp_ms": float(np.mean([k["mlp_ms"] for k in kern]))}
+        outs[name] = out
+        log(f"{name}: {res[name]['ms']:.3f} ms/step (host clock {wall:.3f}; fit kernels "
+            f"{res[name]['fit_ms']:.3f} ms, mlp {res[name]['mlp_ms']:.3f} ms) -> "
+            f"{nch / (res[name]['ms'] / 1e3):.3e} ch/s")
+    torch.cuda.synchronize()
+    graph_ms = None
+    if graph_path in paths:
+        x_in, o, _ = paths[graph_path]
+        g = eng.capture(x_in, tt, o, outs[graph_path])
+        for _ in range(args.warmup):
+            g.launch()
+        torch.cuda.synchronize()
+        ga, gb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        gate_stream()
+        ga.record(stream)
+        for _ in range(args.steps):
+            g.launch()
+        gb.record(stream)
+        torch.cuda.synchronize()
+        graph_ms = ga.elapsed_time(gb) / args.steps
+        g.close()
+        log(f"{graph_path}, graph replays: {graph_ms:.3f} ms/step")
+    return res, outs, tuned, graph_ms
+
+
+def np_out(out):
+    """x [10, n], chi2, flags, iters of a result, on the host (x, chi2 in FP64)."""
+    return (out.x.double().cpu().numpy(), out.chi2.double().cpu().numpy(),
+            out.flags.cpu().numpy(), out.iters.cpu().numpy())
+
+
+def accounting(o, ref, tol, fp64, mask=None, sets=None):
+    """Per-channel parity accounting (tests/parity.py) of one path's outputs
+    against the oracle's records: counts of channels outside tolerance and
+    how each is explained.  FP64 paths also compare FNNLS counts and flags."""
+    from parity import account
+    x, c, f, it = o
+    rep = account(x, c, ref, tol, iters=it, flags=f if fp64 else None, compare_outer=fp64,
+                  mask=mask, sets=sets)
+    rep = {k: v for k, v in rep.items() if not k.startswith("_")}
+    rep["tol"] = tol
+    rep["amplitude_floor_adc"] = 1.0
+    return rep
+
+
+def oracle_accounting(ct, s, t, o_np, threads, eng, s64, tt, paths):
+    """The oracle (FP64, with its decision margins) over the whole batch,
+    twice: on the FP32-rounded samples the fast path fits and on the FP64
+    samples.  Returns, per path, the channel accounting (fast paths at 1e-4,
+    the plain fit at 1e-10); for the warm path's rejected channels, their
+    accounting alone and chi2 against the plain (cold-start) FP64 fit; and
+    the warm path fed the FP64 samples (rounded to FP32 at load) against the
+    oracle on the unrounded samples, the error the input rounding adds."""
+    from oracle_binding import oracle_fit, oracle_opts
+    from test_regression_amd import _abi
+    t0 = time.perf_counter()
+    s_r = np.ascontiguousarray(s.astype(np.float32).astype(np.float64))
+    ref32 = oracle_fit(ct, s_r, t, oracle_opts(record_margins=1), nthreads=threads)
+    sets = (ct, s_r, t)
+    out = {"vs_oracle": {"warm": accounting(o_np["warm"], ref32, 1e-4, False, sets=sets),
+                         "fast": accounting(o_np["fast"], ref32, 1e-4, False, sets=sets)}}
+    fw = o_np["warm"][2]
+    rej = (fw & _abi.FLAGS["warm_reject"]) != 0
+    cw, cp = o_np["warm"][1], o_np["plain"][1]
+    d = np.abs(cw - cp) / np.maximum(cp, 1.0)
+    out["warm_rejected"] = {
+        "channels": int(rej.sum()), "fraction": float(rej.mean()),
+        "fallback_to_cold_start": int(((fw & _abi.FLAGS["warm_fallback"]) != 0).sum()),
+        "chi2_rel_vs_plain_fp64_max": float(d[rej].max()) if rej.any() else 0.0,
+        "chi2_rel_vs_plain_fp64_above_1e-4": int((d[rej] > 1e-4).sum()),
+        "vs_oracle": accounting(o_np["warm"], ref32, 1e-4, False, mask=rej, sets=sets)}
+    ref64 = oracle_fit(ct, np.ascontiguousarray(s), t, oracle_opts(record_margins=1),
+                       nthreads=threads)
+    out["vs_oracle"]["plain"] = accounting(o_np["plain"], ref64, 1e-10, True)
+    # FP64 samples into the FP32 path: rounded to FP32 at load, so the
+    # outputs are those of the FP32 samples; their distance from the oracle
+    # on the unrounded samples is the input rounding's, which the oracle
+    # itself shows between its two runs
+    x_in, o, prec = paths["warm"]
+    r = np_out(eng.fit(s64, tt, o))
+    out["warm_fp64_input"] = {
+        "outputs_equal_fp32_input": bool(np.array_equal(r[0], o_np["warm"][0]) and
+                                         np.array_equal(r[1], o_np["warm"][1])),
+        "vs_oracle_unrounded": errors(r[0], r[1], ref64["x"].T, ref64["chi2"]),
+        "oracle_rounded_vs_unrounded": errors(ref32["x"].T, ref32["chi2"], ref64["x"].T,
+                                              ref64["chi2"])}
+    del ref32, ref64
+    out["oracle_seconds"] = time.perf_counter() - t0
+    out["oracle_threads"] = threads
+    log(f"oracle accounting: {out['oracle_seconds']:.1f}s; warm "
+        f"{out['vs_oracle']['warm']['outside_tolerance']} outside tolerance "
+        f"({out['vs_oracle']['warm']['unexplained']} unexplained), plain "
+        f"{out['vs_oracle']['plain']['outside_tolerance']} "
+        f"({out['vs_oracle']['plain']['unexplained']} unexplained)")
+    return out
+
+
+def live_block(args, stream, threads):
+    """The same workload with a live pulse covariance (the noise covariance
+    plus the amplitude-weighted pulse covariance, rebuilt per channel at each
+    Minimize iteration): plain, cold and warm paths timed as the headline,
+    with their kernels' rooflines and the channel accounting against the
+    oracle over the whole batch."""
+    import torch
+    from test_regression_amd import _abi, engine, warmstart, workloads
+    w = workloads.config(args.config, "live")
+    nch = args.nch or w.nch
+    ct = w.chtypes()
+    s, t, _ = w.generate(nch, nthreads=threads)
+    eng = engine.Engine(list(ct))
+    warmstart.ensure_trained(eng, warmstart.training_workload(args.config),
+                             path=warmstart.weights_for(args.config), log=log)
+    s64 = torch.from_numpy(s).cuda()
+    s32 = s64.float()
+    tt = torch.from_numpy(t).cuda() if t is not None else None
+    O = _abi.default_opts
+    auto = dict(block=0, persistent=_abi.MF_PERSISTENT_AUTO)
+    paths = {
+        "plain": (s64, O(precision=_abi.MF_PREC_FP64, block=0), _abi.MF_PREC_FP64),
+        "fast": (s32, O(precision=_abi.MF_PREC_FP32, **auto), _abi.MF_PREC_FP32),
+        "warm": (s32, O(precision=_abi.MF_PREC_FP32, warm_start=1, **auto), _abi.MF_PREC_FP32),
+    }
+    log(f"live pulse covariance: {nch} channels")
+    res, outs, tuned, _ = time_paths(eng, paths, tt, nch, args, stream, graph_path=None)
+    o_np = {k: np_out(v) for k, v in outs.items()}
+    del outs
+    acc = {k: errors(*o_np[k][:2], *o_np["plain"][:2]) for k in ("warm", "fast")}
+    acc.update(oracle_accounting(ct, s, t, o_np, threads, eng, s64, tt, paths))
+    orc = oracle_sample(ct, s.astype(np.float32).astype(np.float64), t, args.oracle_sample)
+    af = float(orc["aflops"].mean())
+
+    def roof(name, peak_key):
+        k = res[name]["fit_ms"] + res[name]["mlp_ms"]
+        a = af * nch / (k / 1e3) / 1e12
+        return {"bound": "valu", "achieved": a, "peak": PEAK_TFLOPS[peak_key],
+                "unit": "TFLOP/s", "frac": a / PEAK_TFLOPS[peak_key], "kernel_ms": k,
+                "kernel": fit_kernel_name("live", name == "warm") if name != "plain"
+                else "k_fit_plain"}
+
+    fl = o_np["warm"][2]
+    blk = {
+        "workload": f"{args.config} with a live pulse covariance: {w.description}",
+        "channels": nch,
+        "ms_per_step": {k: v["ms"] for k, v in res.items()},
+        "channels_per_s": {k: nch / (v["ms"] / 1e3) for k, v in res.items()},
+        "kernel_ms": {k: v["fit_ms"] for k, v in res.items()},
+        "mlp_ms": res["warm"]["mlp_ms"],
+        "speedup_warm_vs_plain_fp64": res["plain"]["ms"] / res["warm"]["ms"],
+        "speedup_cold_vs_plain_fp64": res["plain"]["ms"] / res["fast"]["ms"],
+        "roofline": {"warm": roof("warm", "f32"), "fast": roof("fast", "f32"),
+                     "plain": roof("plain", "f64"), "aflops_per_channel": af,
+                     "achieved_counts": ACHIEVED_COUNTS},
+        "accuracy": acc,
+        "solves_per_channel": {k: float(((o_np[k][3] >> 16) & 0xFFFF).mean())
+                               for k in ("warm", "fast", "plain")},
+        "minimize_iterations": {k: float((o_np[k][3] & 255).mean())
+                                for k in ("warm", "fast", "plain")},
+        "warm_start_rejected": float(((fl & _abi.FLAGS["warm_reject"]) != 0).mean()),
+        "autotune": tuned or None,
+    }
+    log(f"live: plain {res['plain']['ms']:.3f} ms, cold {res['fast']['ms']:.3f} ms, warm "
+        f"{res['warm']['ms']:.3f} ms ({blk['speedup_warm_vs_plain_fp64']:.2f}x)")
+    del eng
+    torch.cuda.empty_cache()
+    return blk
 
 
 if __name__ == "__main__":

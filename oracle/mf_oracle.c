@@ -725,6 +725,11 @@ typedef struct {
     double m_chi2;
     double lAtA[LD * LD], lAtb[LD];
     unsigned lpm;
+    /* or_dofit_from_set: the first NNLS call ends at the set force_pm */
+    int force;
+    unsigned force_pm;
+    double force_margin;
+    int force_feasible;
 } fitctx;
 
 /* updateCov, PulseChiSqSNNLS.cpp:169-194 */
@@ -839,6 +844,62 @@ static void legacy_nnls(fitctx *c, const double *A, const double *b, int m, int 
     for (int a = 0; a < n; ++a) c->x[perm[a]] = xp[a];
 }
 
+/* KKT test of passive set pm on normal equations (AtA, Atb): the set's
+ * system solved in long double into z (0 outside the set); *feasible: every
+ * amplitude of the set > 0 (0 also when the block is not positive
+ * definite).  Returns the largest (w_i - eps) / unit_i over the indices
+ * outside the set, w = Atb - AtA z, unit_i = u (p + 1) (|Atb_i| +
+ * sum_j |AtA_ij z_j|), the FP64 rounding of w_i. */
+static double kkt_of_set(const double *AtA, const double *Atb, int np, unsigned pm,
+                         double eps, double *z, int *feasible)
+{
+    int idx[LD], k = 0;
+    for (int i = 0; i < np; ++i)
+        if ((pm >> i) & 1u) idx[k++] = i;
+    long double L[LD][LD], y[LD], zz[LD], zl[LD];
+    for (int i = 0; i < LD; ++i) { zl[i] = 0; z[i] = 0.0; }
+    *feasible = 1;
+    for (int a = 0; a < k; ++a)
+        for (int b2 = 0; b2 <= a; ++b2) {
+            long double v = AtA[IDX(idx[a], idx[b2])];
+            for (int q = 0; q < b2; ++q) v -= L[a][q] * L[b2][q];
+            if (a == b2) {
+                if (!(v > 0)) { *feasible = 0; return INFINITY; }
+                L[a][a] = sqrtl(v);
+            } else {
+                L[a][b2] = v / L[b2][b2];
+            }
+        }
+    for (int a = 0; a < k; ++a) {
+        long double v = Atb[idx[a]];
+        for (int q = 0; q < a; ++q) v -= L[a][q] * y[q];
+        y[a] = v / L[a][a];
+    }
+    for (int a = k - 1; a >= 0; --a) {
+        long double v = y[a];
+        for (int q = a + 1; q < k; ++q) v -= L[q][a] * zz[q];
+        zz[a] = v / L[a][a];
+    }
+    for (int a = 0; a < k; ++a) {
+        zl[idx[a]] = zz[a];
+        z[idx[a]] = (double)zz[a];
+        if (!(zz[a] > 0)) *feasible = 0;
+    }
+    double worst = -INFINITY;
+    for (int i = 0; i < np; ++i) {
+        if ((pm >> i) & 1u) continue;
+        long double w = Atb[i], mag = fabs(Atb[i]);
+        for (int j = 0; j < np; ++j) {
+            w -= (long double)AtA[IDX(i, j)] * zl[j];
+            mag += fabsl((long double)AtA[IDX(i, j)] * zl[j]);
+        }
+        const double unit = OR_U * (np + 1) * (double)mag;
+        const double m = (double)(w - eps) / (unit > 0 ? unit : DBL_MIN);
+        if (m > worst) worst = m;
+    }
+    return worst;
+}
+
 /* NNLS, PulseChiSqSNNLS.cpp:219-274 */
 static void nnls_step(fitctx *c)
 {
@@ -852,6 +913,17 @@ static void nnls_step(fitctx *c)
         for (int i = 0; i < ns; ++i) A[IDX(i, j)] = y[i];
     }
     or_forward_subst(c->L, c->s, b, LD, ns);              /* :236 */
+    if (c->force) {   /* or_dofit_from_set: this call ends at the given set */
+        double AtA[LD * LD], Atb[LD], z[LD];
+        memset(AtA, 0, sizeof(AtA));
+        tmul_mn(A, AtA, ns, np);
+        atb_mn(A, b, Atb, ns, np);
+        c->force_margin = kkt_of_set(AtA, Atb, np, c->force_pm, c->o->eps, z,
+                                     &c->force_feasible);
+        for (int i = 0; i < np; ++i) c->x[i] = z[i];
+        c->force = 0;
+        return;
+    }
     double xx[LD];
     for (int i = 0; i < np; ++i) xx[i] = c->x[i];         /* :258 */
     or_nnls_stats st;
@@ -998,14 +1070,20 @@ static double rounding_estimate(const double *AtA, const double *Atb, unsigned p
     return e / sc;
 }
 
-/* DoFit, PulseChiSqSNNLS.cpp:21-147 */
-int or_dofit(const or_chtype *t, const double *samples, const or_opts *o,
-             or_result *r)
+/* DoFit, PulseChiSqSNNLS.cpp:21-147.  force_pm (or_dofit_from_set): the
+ * first NNLS call ends at that passive set's exact solution instead. */
+static int dofit_impl(const or_chtype *t, const double *samples, const or_opts *o,
+                      or_result *r, const unsigned *force_pm, double *margin, int *feasible)
 {
     fitctx c;
     memset(&c, 0, sizeof(c));
     c.t = t;
     c.o = o;
+    if (force_pm) {
+        c.force = 1;
+        c.force_pm = *force_pm;
+        c.force_margin = INFINITY;
+    }
     if (o->count_flops)
         for (int i = 0; i < OR_FULL * OR_FULL; ++i)
             if (t->fullpulsecov[i] != 0.0) { c.has_pi = 1; break; }
@@ -1024,6 +1102,10 @@ int or_dofit(const or_chtype *t, const double *samples, const or_opts *o,
     c.lpm = 0;
     int nmin = 0;
     int status = minimize(&c, &nmin);                    /* :65 */
+    if (force_pm) {
+        *margin = c.force_margin;
+        *feasible = c.force ? 0 : c.force_feasible;   /* still set: never reached */
+    }
     r->m_eps = c.mg.m_eps;
     r->m_arg = c.mg.m_arg;
     r->m_pos = c.mg.m_pos;
@@ -1106,6 +1188,34 @@ int or_dofit(const or_chtype *t, const double *samples, const or_opts *o,
     return status;
 }
 
+int or_dofit(const or_chtype *t, const double *samples, const or_opts *o, or_result *r)
+{
+    return dofit_impl(t, samples, o, r, NULL, NULL, NULL);
+}
+
+double or_dofit_from_set(const or_chtype *t, const double *samples, const or_opts *o,
+                         unsigned pm, or_result *r, int *feasible)
+{
+    double margin = INFINITY;
+    dofit_impl(t, samples, o, r, &pm, &margin, feasible);
+    return margin;
+}
+
+void or_kkt_batch(const or_chtype *types, const uint8_t *type_of, const double *samples,
+                  const double *pederr, long long nch, const long long *idx, long long k,
+                  const uint32_t *pm, const or_opts *o, or_result *res, double *margin,
+                  int *feasible)
+{
+    for (long long a = 0; a < k; ++a) {
+        const long long c = idx[a];
+        or_chtype tc = types[type_of ? type_of[c] : 0];
+        if (pederr) tc.pederr = pederr[c];
+        double s[LD];
+        for (int i = 0; i < tc.nsamples; ++i) s[i] = samples[i * nch + c];
+        margin[a] = or_dofit_from_set(&tc, s, o, pm[a], res + a, feasible + a);
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* threaded batch (CPU baseline)                                             */
 /* ------------------------------------------------------------------------ */
@@ -1174,3 +1284,4 @@ void or_dofit_batch_pederr(const or_chtype *types, const uint8_t *type_of,
     for (int i = 0; i < nthreads; ++i) pthread_create(&th[i], NULL, batch_worker, &b);
     for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
 }
+

@@ -164,6 +164,25 @@ void or_dofit_batch_pederr(const or_chtype *types, const uint8_t *type_of,
                            const double *samples, const double *pederr, long long nch,
                            const or_opts *o, or_result *res, int nthreads);
 
+/* DoFit with its first NNLS call ending at passive set pm (the set's
+ * system solved in long double on that call's normal equations, the x = 0
+ * problem), the rest of the fit as or_dofit: the reference's algorithm
+ * continued from another end point of its first call.  Returns the first
+ * call's stop-test margin: the largest (w_i - eps) / unit_i over the
+ * indices outside pm, unit_i = u (p + 1) (|Atb_i| + sum_j |AtA_ij z_j|) the
+ * FP64 rounding of w_i (<= 0: the set passes the stop test outright; small
+ * positive values: within rounding); *feasible: the set's amplitudes are
+ * all > 0.  r: the continued fit. */
+double or_dofit_from_set(const or_chtype *t, const double *samples, const or_opts *o,
+                         unsigned pm, or_result *r, int *feasible);
+/* or_dofit_from_set for channels idx[0..k) of a batch (samples SoA,
+ * type_of and pederr as or_dofit_batch_pederr), pm[a] the set of channel
+ * idx[a]; res, margin, feasible [k]. */
+void or_kkt_batch(const or_chtype *types, const uint8_t *type_of, const double *samples,
+                  const double *pederr, long long nch, const long long *idx, long long k,
+                  const uint32_t *pm, const or_opts *o, or_result *res, double *margin,
+                  int *feasible);
+
 #ifdef __cplusplus
 }
 #endif

@@ -96,6 +96,12 @@ def oracle():
                                  C.POINTER(OrOpts), C.POINTER(OrResult), C.c_int]
     L.or_dofit_batch_pederr.argtypes = [C.c_void_p, C.c_void_p, dp, C.c_void_p, C.c_longlong,
                                         C.POINTER(OrOpts), C.POINTER(OrResult), C.c_int]
+    L.or_dofit_from_set.argtypes = [C.c_void_p, dp, C.POINTER(OrOpts), C.c_uint,
+                                    C.POINTER(OrResult), C.POINTER(C.c_int)]
+    L.or_dofit_from_set.restype = C.c_double
+    L.or_kkt_batch.argtypes = [C.c_void_p, C.c_void_p, dp, C.c_void_p, C.c_longlong, C.c_void_p,
+                               C.c_longlong, C.c_void_p, C.POINTER(OrOpts), C.c_void_p, dp,
+                               C.c_void_p]
     _cache["oracle"] = L
     return L
 
@@ -148,6 +154,15 @@ def oracle_opts(eps=1e-11, max_iter=1000, chi2_tol=1e-3, max_minimize=50,
                   record_margins=record_margins)
 
 
+RESULT_DTYPE = np.dtype([
+    ("x", "f8", 10), ("err", "f8", 10), ("chi2", "f8"), ("status", "i4"),
+    ("flags", "i4"), ("n_minimize", "i4"), ("n_outer", "i4"), ("n_inner", "i4"),
+    ("ksum", "i8"), ("k2sum", "i8"), ("k3sum", "i8"), ("m_eps", "f8"), ("m_arg", "f8"),
+    ("m_pos", "f8"), ("m_alpha", "f8"), ("pivot", "f8"), ("m_chi2", "f8"), ("xerr", "f8"),
+    ("aflops", "f8")], align=True)
+assert RESULT_DTYPE.itemsize == C.sizeof(OrResult)
+
+
 def oracle_fit(chtypes, samples: np.ndarray, type_of=None, opts: OrOpts | None = None,
                nthreads: int = 1, pederr: np.ndarray | None = None):
     """Runs the oracle's DoFit on every channel.  samples [10, n] float64;
@@ -164,14 +179,37 @@ def oracle_fit(chtypes, samples: np.ndarray, type_of=None, opts: OrOpts | None =
                             dptr(samples),
                             pederr.ctypes.data_as(C.c_void_p) if pederr is not None else None,
                             n, C.byref(opts or oracle_opts()), res, nthreads)
-    dt = np.dtype([
-        ("x", "f8", 10), ("err", "f8", 10), ("chi2", "f8"), ("status", "i4"),
-        ("flags", "i4"), ("n_minimize", "i4"), ("n_outer", "i4"), ("n_inner", "i4"),
-        ("ksum", "i8"), ("k2sum", "i8"), ("k3sum", "i8"), ("m_eps", "f8"), ("m_arg", "f8"),
-        ("m_pos", "f8"), ("m_alpha", "f8"), ("pivot", "f8"), ("m_chi2", "f8"), ("xerr", "f8"),
-        ("aflops", "f8")], align=True)
-    assert dt.itemsize == C.sizeof(OrResult)
-    return np.frombuffer(res, dtype=dt)
+    return np.frombuffer(res, dtype=RESULT_DTYPE)
+
+
+def oracle_kkt_sets(chtypes, samples: np.ndarray, type_of, idx, pm, opts: OrOpts | None = None,
+                    pederr: np.ndarray | None = None):
+    """The oracle's DoFit of channel idx[a] continued from passive set pm[a]
+    (bit mask) as the end of its first NNLS call (or_kkt_batch).  Returns the
+    continued fits (an oracle_fit record array) with two more fields:
+    margin, the first call's stop-test margin in rounding units of w (<= 0
+    passes outright), and feasible (the set's amplitudes all > 0)."""
+    L = oracle()
+    n = samples.shape[1]
+    samples = np.ascontiguousarray(samples, dtype=np.float64)
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    pm = np.ascontiguousarray(pm, dtype=np.uint32)
+    k = idx.size
+    res = (OrResult * max(k, 1))()
+    margin = np.zeros(max(k, 1))
+    feas = np.zeros(max(k, 1), dtype=np.int32)
+    if pederr is not None:
+        pederr = np.ascontiguousarray(pederr, dtype=np.float64)
+    if k:
+        L.or_kkt_batch(C.cast(chtypes, C.c_void_p),
+                       type_of.ctypes.data_as(C.c_void_p) if type_of is not None else None,
+                       dptr(samples),
+                       pederr.ctypes.data_as(C.c_void_p) if pederr is not None else None,
+                       n, idx.ctypes.data_as(C.c_void_p), k, pm.ctypes.data_as(C.c_void_p),
+                       C.byref(opts or oracle_opts()), res, dptr(margin),
+                       feas.ctypes.data_as(C.c_void_p))
+    r = np.frombuffer(res, dtype=RESULT_DTYPE)[:k]
+    return {"fit": r, "margin": margin[:k], "feasible": feas[:k] != 0}
 
 
 def hostcore_fit(chtypes, samples: np.ndarray, type_of, opts, pederr=None):

@@ -19,6 +19,8 @@ pytestmark = pytest.mark.gpu
 
 TOL_X = 1e-4
 TOL_CHI2 = 1e-4
+# the samples behind each oracle record, for the accounting's end-point test
+ref_sets = {}
 
 
 def _fit_fast(cfg, pulse_cov, n, sample_dtype=np.float32, **kw):
@@ -39,13 +41,14 @@ def _fit_fast(cfg, pulse_cov, n, sample_dtype=np.float32, **kw):
     # the oracle sees exactly the samples the fast path fitted (FP32-rounded)
     s_ref = s_in.astype(np.float32).astype(np.float64)
     ref = oracle_fit(ct, s_ref, t, oracle_opts(), nthreads=8)
+    ref_sets[id(ref)] = (ct, s_ref, t)
     return r, ref
 
 
 def _check(r, ref, label=""):
     return parity.gate(r.x.double().cpu().numpy(), r.chi2.double().cpu().numpy(), ref, TOL_X,
                        tol_chi2=TOL_CHI2, iters=r.iters.cpu().numpy(), compare_outer=False,
-                       label=label)
+                       label=label, sets=ref_sets.get(id(ref)))
 
 
 @pytest.mark.parametrize("cfg,pulse_cov,n", [

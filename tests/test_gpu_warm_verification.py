@@ -76,7 +76,7 @@ def test_rejected_warm_starts_match_the_oracle(pkg, gpu, cfg, pulse_cov, n, pers
     assert rej.sum() > 50
     assert not np.any(fw & _abi.FLAGS["warm_fallback"])
     parity.gate(xw, cw, ref, 1e-4, iters=iw, compare_outer=False, mask=rej,
-                label=f"rejected warm starts {cfg} {pulse_cov}")
+                label=f"rejected warm starts {cfg} {pulse_cov}", sets=(ct, s_ref, t))
 
 
 @pytest.mark.parametrize("passive", [(0,), (0, 1, 2, 3, 4, 6, 7, 8, 9), tuple(range(10))])
@@ -97,7 +97,7 @@ def test_wrong_predictions_end_at_the_oracle_optimum(pkg, gpu, passive, persiste
     assert not np.any(fw & _abi.FLAGS["warm_fallback"])
     ref = oracle_fit(ct, s_ref, t, oracle_opts(), nthreads=8)
     parity.gate(xw, cw, ref, 1e-4, iters=iw, compare_outer=False,
-                label=f"warm from {passive}")
+                label=f"warm from {passive}", sets=(ct, s_ref, t))
 
 
 @pytest.mark.parametrize("persistent", [0, 1])

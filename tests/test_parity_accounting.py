@@ -53,3 +53,57 @@ def test_accounting_catches_planted_mistakes(fits, mistake):
     assert rep["unexplained_channels"] == [c]
     with pytest.raises(AssertionError):
         parity.check(rep)
+
+
+@pytest.fixture(scope="module")
+def near_exact(pkg):
+    """Two c2 channels (zero pulse covariance) fitted almost exactly (chi2 ~
+    1e-11) by nine pulses, where the sets with BX -5 and with BX -4 both pass
+    the reference's stop test: the oracle's cold start ends at one, a warm
+    start from a prediction of the other ends there."""
+    from test_regression_amd import workloads
+    w = workloads.config("c2", "zero")
+    ct = w.chtypes()
+    s, _, _ = w.generate(720_000)
+    s = np.ascontiguousarray(s[:, [119053, 719443]].astype(np.float32).astype(np.float64))
+    ref = oracle_fit(ct, s, None, oracle_opts())
+    return ct, s, ref
+
+
+def _alt(ref, c):
+    pm = int(sum(1 << i for i in range(10) if ref["x"][c][i] > 0))
+    return pm ^ 0x3   # BX -5 and BX -4 exchanged
+
+
+def test_another_end_point_of_the_stop_test_is_explained(near_exact):
+    from oracle_binding import oracle_kkt_sets
+    ct, s, ref = near_exact
+    assert (ref["chi2"] < 1e-9).all()
+    pm = [_alt(ref, c) for c in range(2)]
+    k = oracle_kkt_sets(ct, s, None, [0, 1], pm)
+    assert k["feasible"].all() and (k["margin"] < 0).all()
+    # the fit continued from the other set keeps it
+    assert ((k["fit"]["x"] > 0) == ((np.array(pm)[:, None] >> np.arange(10)) & 1)).all()
+    x = k["fit"]["x"].T.astype(np.float32).astype(np.float64)
+    rep = parity.account(x, k["fit"]["chi2"], ref, 1e-4, sets=(ct, s, None))
+    assert rep["outside_tolerance"] == 2 and rep["by_alternative_end_point"] == 2
+    assert rep["unexplained"] == 0
+    # without the samples the accounting cannot tell: both unexplained
+    assert parity.account(x, k["fit"]["chi2"], ref, 1e-4)["unexplained"] == 2
+
+
+@pytest.mark.parametrize("mistake", ["off_by_1e-3", "in_time_dropped"])
+def test_planted_mistakes_are_not_another_end_point(near_exact, mistake):
+    from oracle_binding import oracle_kkt_sets
+    ct, s, ref = near_exact
+    pm = [_alt(ref, c) for c in range(2)]
+    if mistake == "in_time_dropped":
+        pm = [p & ~(1 << 5) for p in pm]   # the in-time pulse left out: w_5 far above eps
+    k = oracle_kkt_sets(ct, s, None, [0, 1], pm)
+    x = k["fit"]["x"].T.copy()
+    if mistake == "off_by_1e-3":
+        x[5] *= 1 + 1e-3
+    else:
+        assert (k["margin"] > 1e6).all()
+    rep = parity.account(x, k["fit"]["chi2"], ref, 1e-4, sets=(ct, s, None))
+    assert rep["unexplained"] == 2 and rep["by_alternative_end_point"] == 0
