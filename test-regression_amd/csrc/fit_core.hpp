@@ -267,13 +267,22 @@ MF_HD void invert_lower(T (&L)[10][10], const T (&ri)[10])
     }
 }
 
+// A pulse layout known at compile time.  D >= 0: pulse p's window starts at
+// sample first_p >= lay_first<D>(p) = max(0, p - D) (consecutive BXs, as in
+// every reference configuration: BX -5..4 with 10 samples and BX -4..3 with
+// 8 give D = 2), so the rows of P, Pi_p and A = L^-1 P above lay_first<D>(p)
+// are zero and their terms are left out.  Each such term adds an exact zero
+// (an accumulator that starts at +0 stays +0, fma(0, y, a) == a), so the
+// results are identical bit for bit.  D < 0: nothing known, nothing left out.
+template <int D> MF_HD constexpr int lay_first(int p) { return D < 0 ? 0 : (p > D ? p - D : 0); }
+
 // updateCov (PulseChiSqSNNLS.cpp:169-194): lower(C) = pederr^2 R + sum_p x_p^2 Pi_p.
 // A pulse with x_p == 0 adds fma(0, Pi, C) == C exactly, so the reference's
 // `continue` (:178-179) needs no branch.
 // rho: the channel's scale of the type's noise covariance (1: the type's
 // pederr; the type's pederr 1 and rho = pederr^2 give pederr^2 * noisecor
 // exactly as the reference forms it).
-template <typename T, int NS, int NP>
+template <typename T, int NS, int NP, int D = -1>
 MF_HD void build_cov(const TypeMats<T> &tm, bool use_pi, const T (&x)[10], T (&C)[10][10],
                      T rho = T(1))
 {
@@ -291,7 +300,8 @@ MF_HD void build_cov(const TypeMats<T> &tm, bool use_pi, const T (&x)[10], T (&C
 #pragma unroll
             for (int i = 0; i < NS; ++i)
 #pragma unroll
-                for (int j = 0; j <= i; ++j) C[i][j] = mfma(a2, tm.Pi[p][i][j], C[i][j]);
+                for (int j = 0; j <= i; ++j)
+                    if (j >= lay_first<D>(p)) C[i][j] = mfma(a2, tm.Pi[p][i][j], C[i][j]);
         }
     }
 }
@@ -302,7 +312,7 @@ MF_HD void build_cov(const TypeMats<T> &tm, bool use_pi, const T (&x)[10], T (&C
 //
 // zc >= 0: column zc of the pulse matrix is taken as zero (DoFit's error
 // computation zeroes the in-time column, PulseChiSqSNNLS.cpp:104-105).
-template <typename T, int NS, int NP>
+template <typename T, int NS, int NP, int D = -1>
 MF_HD void normal_equations(const TypeMats<T> &tm, const T (&Li)[10][10], const T (&s)[10],
                             T (&AtA)[10][10], T (&Atb)[10], int zc = -1)
 {
@@ -317,16 +327,19 @@ MF_HD void normal_equations(const TypeMats<T> &tm, const T (&Li)[10][10], const 
         T b = T(0);
 #pragma unroll
         for (int k = 0; k <= r; ++k) b = mfma(Li[r][k], s[k], b);
+        // a[p] = row r of A: zero above column p's first sample
         T a[10];
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
             T t = T(0);
 #pragma unroll
-            for (int k = 0; k <= r; ++k) t = mfma(Li[r][k], tm.P[k][p], t);
+            for (int k = 0; k <= r; ++k)
+                if (k >= lay_first<D>(p)) t = mfma(Li[r][k], tm.P[k][p], t);
             a[p] = (p == zc) ? T(0) : t;
         }
 #pragma unroll
         for (int i = 0; i < NP; ++i) {
+            if (r < lay_first<D>(i)) continue;   // a[i] = 0: adds exactly 0
             Atb[i] = mfma(a[i], b, Atb[i]);
 #pragma unroll
             for (int j = 0; j <= i; ++j) AtA[i][j] = mfma(a[i], a[j], AtA[i][j]);
@@ -335,7 +348,7 @@ MF_HD void normal_equations(const TypeMats<T> &tm, const T (&Li)[10][10], const 
 }
 
 // chi2 = |L^{-1} (P x - s)|^2 (ComputeChiSq, PulseChiSqSNNLS.cpp:196-200)
-template <typename T, int NS, int NP>
+template <typename T, int NS, int NP, int D = -1>
 MF_HD T chisq(const TypeMats<T> &tm, const T (&Li)[10][10], const T (&s)[10], const T (&x)[10],
               int zc = -1)
 {
@@ -344,7 +357,8 @@ MF_HD T chisq(const TypeMats<T> &tm, const T (&Li)[10][10], const T (&s)[10], co
     for (int i = 0; i < NS; ++i) {
         T t = T(0);
 #pragma unroll
-        for (int p = 0; p < NP; ++p) t = mfma(p == zc ? T(0) : tm.P[i][p], x[p], t);
+        for (int p = 0; p < NP; ++p)
+            if (i >= lay_first<D>(p)) t = mfma(p == zc ? T(0) : tm.P[i][p], x[p], t);
         r[i] = t - s[i];
     }
     T q = T(0);
@@ -1051,22 +1065,31 @@ template <typename T> struct FitOut {
 // by prm.solver): XS_LEGACY (with ls), XS_LH or XS_CHOL.
 enum { XS_NONE = 0, XS_LEGACY = 2, XS_LH = 3, XS_CHOL = 4 };
 
-template <typename T, int NS, int NP, int XS = XS_NONE, typename Park>
-MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams &prm,
-                      Park &park, T (&x)[10], FnnlsStats &st, int &flags, int zc = -1,
-                      LegacyState *ls = nullptr)
+// The whitened normal equations of one Minimize iteration (updateCov + LLT,
+// PulseChiSqSNNLS.cpp:155-157, 235-236), with L^{-1} parked in `park`.
+template <typename T, int NS, int NP, int D = -1, typename Park>
+MF_HD void iteration_problem(const TypeMats<T> &tm, const T (&s)[10], const FitParams &prm,
+                             Park &park, const T (&x)[10], T (&AtA)[10][10], T (&Atb)[10],
+                             int &flags, int zc)
 {
-    const TypeMats<T> &tm = *opaque(&tm0);
     T Li[10][10], ri[10];
-    build_cov<T, NS, NP>(tm, prm.use_pi != 0, x, Li, T(prm.rho));
+    build_cov<T, NS, NP, D>(tm, prm.use_pi != 0, x, Li, T(prm.rho));
     if (!cholesky_inplace<T, NS>(Li, ri)) flags |= F_LLT_FAIL;
     invert_lower<T, NS>(Li, ri);
-    T AtA[10][10], Atb[10];
-    normal_equations<T, NS, NP>(tm, Li, s, AtA, Atb, zc);
+    normal_equations<T, NS, NP, D>(tm, Li, s, AtA, Atb, zc);
 #pragma unroll
     for (int i = 0; i < NS; ++i)
 #pragma unroll
         for (int j = 0; j <= i; ++j) park.store(i * (i + 1) / 2 + j, Li[i][j]);
+}
+
+// The NNLS call and chi2 of one Minimize iteration on its normal equations
+// (PulseChiSqSNNLS.cpp:158-160); L^{-1} is read back from `park`.
+template <typename T, int NS, int NP, int XS = XS_NONE, int D = -1, typename Park>
+MF_HD T iteration_solve(const TypeMats<T> &tm, const T (&s)[10], const FitParams &prm,
+                        Park &park, const T (&AtA)[10][10], const T (&Atb)[10], T (&x)[10],
+                        FnnlsStats &st, int zc, LegacyState *ls)
+{
     if (XS == XS_LEGACY && ls)
         fnnls_legacy<T, NP>(AtA, Atb, x, T(prm.eps), prm.max_iter, st, *ls);
     else if (XS == XS_LH)   // PulseChiSqSNNLS.cpp:253-257 (setX, solve, x)
@@ -1077,23 +1100,56 @@ MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams 
         fnnls_mask<T, NP>(AtA, Atb, x, T(prm.eps), prm.max_iter, st, prm.pmask);
     else
         fnnls_inplace<T, NP>(AtA, Atb, x, T(prm.eps), T(0), prm.max_iter, st);
+    T Li[10][10];
 #pragma unroll
     for (int i = 0; i < NS; ++i)
 #pragma unroll
         for (int j = 0; j <= i; ++j) Li[i][j] = park.load(i * (i + 1) / 2 + j);
-    return chisq<T, NS, NP>(tm, Li, s, x, zc);
+    return chisq<T, NS, NP, D>(tm, Li, s, x, zc);
+}
+
+template <typename T, int NS, int NP, int XS = XS_NONE, int D = -1, typename Park>
+MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams &prm,
+                      Park &park, T (&x)[10], FnnlsStats &st, int &flags, int zc = -1,
+                      LegacyState *ls = nullptr)
+{
+    const TypeMats<T> &tm = *opaque(&tm0);
+    T AtA[10][10], Atb[10];
+    iteration_problem<T, NS, NP, D>(tm, s, prm, park, x, AtA, Atb, flags, zc);
+    return iteration_solve<T, NS, NP, XS, D>(tm, s, prm, park, AtA, Atb, x, st, zc, ls);
 }
 
 // Minimize (PulseChiSqSNNLS.cpp:149-166) from iteration `it0` on, with the
 // previous chi2 `chi2` and amplitudes x.  Returns the iteration count.
-template <typename T, int NS, int NP, int XS = XS_NONE, typename Park>
+// KEEP: form the normal equations once when the covariance does not depend
+// on x (off for the error refits, whose extra register state costs more
+// than the repeated set-up there).
+template <typename T, int NS, int NP, int XS = XS_NONE, int D = -1, bool KEEP = true,
+          typename Park>
 MF_HD int minimize_from(const TypeMats<T> &tm, const T (&s)[10], const FitParams &prm,
                         Park &park, T (&x)[10], T &chi2, int it0, FnnlsStats &st, int &flags,
                         int zc = -1, LegacyState *ls = nullptr)
 {
     int it = it0;
+    if (KEEP && !prm.use_pi && it < prm.max_minimize) {
+        // The covariance does not depend on x: every iteration forms the
+        // same normal equations and L^{-1}, bit for bit, so they are formed
+        // once and kept (the solves and chi2 are the reference's).
+        const TypeMats<T> &tmo = *opaque(&tm);
+        T AtA[10][10], Atb[10];
+        iteration_problem<T, NS, NP, D>(tmo, s, prm, park, x, AtA, Atb, flags, zc);
+        for (; it < prm.max_minimize; ++it) {                   // :154
+            const T now =
+                iteration_solve<T, NS, NP, XS, D>(tmo, s, prm, park, AtA, Atb, x, st, zc, ls);
+            const T d = now - chi2;
+            chi2 = now;
+            if (mabs(d) < T(prm.chi2_tol)) return it + 1;       // :162
+        }
+        flags |= F_MINIMIZE_CAP;
+        return it;
+    }
     for (; it < prm.max_minimize; ++it) {                       // :154
-        const T now = minimize_step<T, NS, NP, XS>(tm, s, prm, park, x, st, flags, zc, ls);
+        const T now = minimize_step<T, NS, NP, XS, D>(tm, s, prm, park, x, st, flags, zc, ls);
         const T d = now - chi2;
         chi2 = now;
         if (mabs(d) < T(prm.chi2_tol)) return it + 1;           // :162
@@ -1124,7 +1180,7 @@ MF_HD bool all_finite(const T (&s)[10])
 // pulse is passive it leaves the passive set first (:151-157 -- a branch the
 // current reference never takes, its _nP staying 0), and the two Minimize
 // calls continue from that state.
-template <typename T, int NS, int NP, int XS = XS_NONE, typename Park>
+template <typename T, int NS, int NP, int XS = XS_NONE, int D = -1, typename Park>
 MF_HD T minos_error(const TypeMats<T> &tm, const T (&s)[10], const FitParams &prm, Park &park,
                     const T (&xfit)[10], T chi2_0, int ip, FnnlsStats &st, int &flags,
                     LegacyState *ls = nullptr)
@@ -1155,13 +1211,13 @@ MF_HD T minos_error(const TypeMats<T> &tm, const T (&s)[10], const FitParams &pr
     x[ip] = xplus;
 #pragma unroll
     for (int i = 0; i < 10; ++i) sr[i] = i < NS ? s[i] - xplus * tm.P[i][ip] : T(0);
-    minimize_from<T, NS, NP, XS>(tm, sr, prm, park, x, chi2, 0, st, flags, ip, ls);
+    minimize_from<T, NS, NP, XS, D, false>(tm, sr, prm, park, x, chi2, 0, st, flags, ip, ls);
     T Li[10][10];
 #pragma unroll
     for (int i = 0; i < NS; ++i)
 #pragma unroll
         for (int j = 0; j <= i; ++j) Li[i][j] = park.load(i * (i + 1) / 2 + j);
-    const T chiplus = chisq<T, NS, NP>(tm, Li, sr, x, ip);      // :117
+    const T chiplus = chisq<T, NS, NP, D>(tm, Li, sr, x, ip);      // :117
     const T sigplus = mabs(xplus - x0) / sqrt(chiplus - chi2_0);
     if (!(x0 / sigplus > T(0.5))) return sigplus;              // :122, :141-142
     T xminus = x0 - approxerr;                                  // :130
@@ -1169,12 +1225,12 @@ MF_HD T minos_error(const TypeMats<T> &tm, const T (&s)[10], const FitParams &pr
     x[ip] = xminus;
 #pragma unroll
     for (int i = 0; i < 10; ++i) sr[i] = i < NS ? s[i] - xminus * tm.P[i][ip] : T(0);
-    minimize_from<T, NS, NP, XS>(tm, sr, prm, park, x, chi2, 0, st, flags, ip, ls);
+    minimize_from<T, NS, NP, XS, D, false>(tm, sr, prm, park, x, chi2, 0, st, flags, ip, ls);
 #pragma unroll
     for (int i = 0; i < NS; ++i)
 #pragma unroll
         for (int j = 0; j <= i; ++j) Li[i][j] = park.load(i * (i + 1) / 2 + j);
-    const T chiminus = chisq<T, NS, NP>(tm, Li, sr, x, ip);
+    const T chiminus = chisq<T, NS, NP, D>(tm, Li, sr, x, ip);
     const T sigminus = mabs(xminus - x0) / sqrt(chiminus - chi2_0);
     return T(0.5) * (sigplus + sigminus);
 }
@@ -1186,7 +1242,8 @@ MF_HD T minos_error(const TypeMats<T> &tm, const T (&s)[10], const FitParams &pr
 // separate instantiations, so that the default solvers do not carry their
 // state and code.
 // Non-finite samples are not fitted (x = 0, F_NONFINITE).
-template <typename T, int NS, int NP, bool ERRORS = false, int XS = XS_NONE, typename Park>
+template <typename T, int NS, int NP, bool ERRORS = false, int XS = XS_NONE, int D = -1,
+          typename Park>
 MF_HD void fit_plain(const TypeMats<T> &tm, const T (&s)[10], const FitParams &prm,
                      Park &park, FitOut<T> &out, int ip = -1)
 {
@@ -1203,13 +1260,13 @@ MF_HD void fit_plain(const TypeMats<T> &tm, const T (&s)[10], const FitParams &p
     ls.reset();
     LegacyState *lsp = XS == XS_LEGACY ? &ls : nullptr;
     if (all_finite<T, NS>(s) && rho_ok(prm.rho))
-        nmin = minimize_from<T, NS, NP, XS>(tm, s, prm, park, x, chi2, 0, st, flags, -1, lsp);
+        nmin = minimize_from<T, NS, NP, XS, D>(tm, s, prm, park, x, chi2, 0, st, flags, -1, lsp);
     else
         flags |= F_NONFINITE;
     if (ERRORS && ip >= 0 && !(flags & F_NONFINITE) && prm.max_minimize > 0) {
         FnnlsStats st2;
         int fl2 = 0;
-        const T e = minos_error<T, NS, NP, XS>(tm, s, prm, park, x, chi2, ip, st2, fl2, lsp);
+        const T e = minos_error<T, NS, NP, XS, D>(tm, s, prm, park, x, chi2, ip, st2, fl2, lsp);
 #pragma unroll
         for (int i = 0; i < 10; ++i)
             if (i == ip) out.err[i] = e;
@@ -1517,7 +1574,7 @@ MF_HD int fixed_cov_nmin(double chi2, const FitParams &prm, int &flags)
 struct KeepSamples {
     MF_HD void operator()(float (&)[10]) const {}
 };
-template <int NS, int NP, bool LIVE, bool TAB = true, typename Park,
+template <int NS, int NP, bool LIVE, bool TAB = true, int D = -1, typename Park,
           typename Reload = KeepSamples>
 MF_HD void fast_phase2(const DevType &ty, float (&s32)[10], const FitParams &prm,
                        const float (&x)[10], bool warm, FnnlsStats &st, int flags, Park &park,
@@ -1560,8 +1617,8 @@ MF_HD void fast_phase2(const DevType &ty, float (&s32)[10], const FitParams &prm
                 double s64[10];
 #pragma unroll
                 for (int i = 0; i < 10; ++i) s64[i] = (double)s32[i];
-                nmin = minimize_from<double, NS, NP>(ty.d, s64, prm, park, S.x, chi2, 1, st,
-                                                     flags);
+                nmin = minimize_from<double, NS, NP, XS_NONE, D>(ty.d, s64, prm, park, S.x,
+                                                                 chi2, 1, st, flags);
             }
         } else {
             nmin = fixed_cov_nmin(chi2, prm, flags);
@@ -1583,7 +1640,7 @@ MF_HD void fast_phase2(const DevType &ty, float (&s32)[10], const FitParams &prm
 
 // The fast fit of one channel in one call: phase 1 from a cold start, then
 // phase 2 (a warm start begins at phase 2).
-template <int NS, int NP, bool LIVE, bool TAB = true, typename Park>
+template <int NS, int NP, bool LIVE, bool TAB = true, int D = -1, typename Park>
 MF_HD void fit_fast(const DevType &ty, const float (&s32_in)[10], const FitParams &prm,
                     float (&x)[10], uint32_t pm, bool warm, Park &park, FitOut<float> &out)
 {
@@ -1594,7 +1651,7 @@ MF_HD void fit_fast(const DevType &ty, const float (&s32_in)[10], const FitParam
     for (int i = 0; i < 10; ++i) s32[i] = s32_in[i];
     if (!warm && all_finite<float, NS>(s32) && rho_ok(prm.rho) && prm.max_minimize > 0)
         fast_phase1<NS, NP>(ty, s32, prm, x, pm, st, flags);
-    fast_phase2<NS, NP, LIVE, TAB>(ty, s32, prm, x, warm, st, flags, park, out);
+    fast_phase2<NS, NP, LIVE, TAB, D>(ty, s32, prm, x, warm, st, flags, park, out);
 }
 
 // Register-array park for host builds and for kernels that keep L^{-1} live.

@@ -145,7 +145,8 @@ __device__ __forceinline__ FitParams lane_params(const FitParams &prm, const Dev
     return p;
 }
 
-template <typename T, typename TS, int N, bool ERRORS, int XS>
+// D: the pulse layout the kernel is specialised for (lay_first, fit_core.hpp).
+template <typename T, typename TS, int N, bool ERRORS, int XS, int D>
 __global__ __launch_bounds__(256)
 void k_fit_plain(const DevType *__restrict__ types, SegArg seg,
                  const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
@@ -163,7 +164,7 @@ void k_fit_plain(const DevType *__restrict__ types, SegArg seg,
     for (int i = 0; i < 10; ++i) s[i] = i < N ? (T)samples[(int64_t)i * nch + c] : T(0);
     LdsPark<T> park{reinterpret_cast<T *>(lds_raw) + threadIdx.x, (int)blockDim.x};
     FitOut<T> out;
-    fit_plain<T, N, N, ERRORS, XS>(tm, s, prm, park, out, types[type_id].h.intime);
+    fit_plain<T, N, N, ERRORS, XS, D>(tm, s, prm, park, out, types[type_id].h.intime);
 #pragma unroll
     for (int p = 0; p < 10; ++p) xo[(int64_t)p * nch + c] = out.x[p];
     if (ERRORS) {
@@ -346,7 +347,7 @@ void k_fast_warm(const DevType *__restrict__ types, SegArg seg,
     ito[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
 }
 
-template <typename TS, int N, bool LIVE, bool PED>
+template <typename TS, int N, bool LIVE, bool PED, int D>
 __global__ __launch_bounds__(256)
 void k_fast_p2(const DevType *__restrict__ types, SegArg seg,
                const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
@@ -372,8 +373,8 @@ void k_fast_p2(const DevType *__restrict__ types, SegArg seg,
     st.llt_fail = (f1 & F_LLT_FAIL) != 0;
     LdsPark<double> park{reinterpret_cast<double *>(lds_raw) + threadIdx.x, (int)blockDim.x};
     FitOut<float> out;
-    fast_phase2<N, N, LIVE>(types[type_id], s, prm, x, warm != 0, st, f1 & F_FP64_PIVOT, park,
-                            out);
+    fast_phase2<N, N, LIVE, true, D>(types[type_id], s, prm, x, warm != 0, st, f1 & F_FP64_PIVOT,
+                                     park, out);
 #pragma unroll
     for (int p = 0; p < 10; ++p) xo[(int64_t)p * nch + c] = out.x[p];
     chi2o[c] = out.chi2;
@@ -1363,6 +1364,23 @@ extern "C" mf_status mf_set_types(mf_context *ctx, const mf_chtype *types, int32
 static int64_t seg_threads(const SegArg &a) { return a.nrange ? a.wave0[a.nrange] * 64 : a.list.n; }
 static int seg_type0(const SegArg &a) { return a.nrange ? a.rtype[0] : a.type_id; }
 
+// Whether the kernels' layout specialisation D = 2 holds for a type (see
+// lay_first): every pulse window starts at or after sample max(0, p - 2).
+static bool layout2(const mf_chtype &t)
+{
+    for (int p = 0; p < t.npulses; ++p)
+        if (std::max(0, t.bx[p] + t.first_off) < std::max(0, p - 2)) return false;
+    return true;
+}
+// ... for every type of a launch
+static bool seg_layout2(const mf_context *ctx, const SegArg &seg)
+{
+    if (seg.nrange == 0) return layout2(ctx->types[(size_t)seg.type_id]);
+    for (int r = 0; r < seg.nrange; ++r)
+        if (!layout2(ctx->types[(size_t)seg.rtype[r]])) return false;
+    return true;
+}
+
 template <typename T, typename TS, int N>
 static mf_status launch_fit(mf_context *ctx, const SegArg &seg, const void *samples,
                             const double *ped, int64_t nch, const mf_result *out,
@@ -1372,12 +1390,14 @@ static mf_status launch_fit(mf_context *ctx, const SegArg &seg, const void *samp
     if (nt <= 0) return MF_OK;
     const int64_t grid = (nt + block - 1) / block;
     const size_t lds = sizeof(T) * 55 * (size_t)block;
+    const bool l2 = seg_layout2(ctx, seg);
     auto pick = [&](auto e) {
         constexpr bool E = decltype(e)::value;
-        return prm.solver == MF_SOLVER_LEGACY ? k_fit_plain<T, TS, N, E, XS_LEGACY>
-               : prm.solver == MF_SOLVER_LAWSON_HANSON ? k_fit_plain<T, TS, N, E, XS_LH>
-               : prm.solver == MF_SOLVER_INPLACE_FNNLS_CHOL ? k_fit_plain<T, TS, N, E, XS_CHOL>
-                                                            : k_fit_plain<T, TS, N, E, XS_NONE>;
+        return prm.solver == MF_SOLVER_LEGACY ? k_fit_plain<T, TS, N, E, XS_LEGACY, -1>
+               : prm.solver == MF_SOLVER_LAWSON_HANSON ? k_fit_plain<T, TS, N, E, XS_LH, -1>
+               : prm.solver == MF_SOLVER_INPLACE_FNNLS_CHOL
+                   ? (l2 ? k_fit_plain<T, TS, N, E, XS_CHOL, 2> : k_fit_plain<T, TS, N, E, XS_CHOL, -1>)
+                   : (l2 ? k_fit_plain<T, TS, N, E, XS_NONE, 2> : k_fit_plain<T, TS, N, E, XS_NONE, -1>);
     };
     auto kern = errors ? pick(std::true_type{}) : pick(std::false_type{});
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), lds, st, ctx->d_types, seg,
@@ -1443,7 +1463,9 @@ static mf_status launch_fast(mf_context *ctx, const SegArg &seg, const void *sam
                        static_cast<float *>(out->x), out->flags, out->iters, warm, prm);
     HIP_TRY(hipGetLastError());
     const size_t lds = LIVE ? sizeof(double) * 55 * (size_t)block : 0;
-    hipLaunchKernelGGL((k_fast_p2<TS, N, LIVE, PED>), dim3((unsigned)grid), dim3(block), lds, st,
+    auto p2 = LIVE && seg_layout2(ctx, seg) ? k_fast_p2<TS, N, LIVE, PED, 2>
+                                            : k_fast_p2<TS, N, LIVE, PED, -1>;
+    hipLaunchKernelGGL(p2, dim3((unsigned)grid), dim3(block), lds, st,
                        ctx->d_types, seg, static_cast<const TS *>(samples), ped, nch,
                        static_cast<float *>(out->x), static_cast<float *>(out->chi2), out->flags,
                        out->iters, warm.pm ? 1 : 0, prm);

@@ -15,6 +15,10 @@
 using namespace mf;
 
 static bool use_tables = true;
+// 2: the per-channel iterations built for the pulse layout D = 2 (lay_first),
+// as the kernels run types of that layout; -1: the generic build
+static int g_layout = -1;
+extern "C" void hc_set_layout(int d) { g_layout = d == 2 ? 2 : -1; }
 
 template <int N, bool LIVE>
 static void run_fast(const DevType &dt, const double *samples, int64_t nch, int64_t c,
@@ -28,7 +32,9 @@ static void run_fast(const DevType &dt, const double *samples, int64_t nch, int6
     }
     RegPark<double> park;   // the live-covariance iterations park L^{-1} in FP64, as the kernel's LDS
     FitOut<float> out;
-    if (use_tables)
+    if (use_tables && g_layout == 2)
+        fit_fast<N, N, LIVE, true, 2>(dt, s, prm, x0, 0u, false, park, out);
+    else if (use_tables)
         fit_fast<N, N, LIVE, true>(dt, s, prm, x0, 0u, false, park, out);
     else
         fit_fast<N, N, LIVE, false>(dt, s, prm, x0, 0u, false, park, out);
@@ -40,6 +46,16 @@ static void run_fast(const DevType &dt, const double *samples, int64_t nch, int6
 
 static double *g_err = nullptr;
 
+template <typename T, int N, bool E, int XS>
+static void plain_d(const DevType &dt, const T (&s)[10], const FitParams &prm, RegPark<T> &park,
+                    FitOut<T> &out, int ip)
+{
+    if (g_layout == 2)
+        fit_plain<T, N, N, E, XS, 2>(mats<T>(dt), s, prm, park, out, ip);
+    else
+        fit_plain<T, N, N, E, XS, -1>(mats<T>(dt), s, prm, park, out, ip);
+}
+
 template <typename T, int N>
 static void run(const DevType &dt, const double *samples, int64_t nch, int64_t c,
                 const FitParams &prm, double *x, double *chi2, int32_t *flags, int32_t *iters)
@@ -50,21 +66,21 @@ static void run(const DevType &dt, const double *samples, int64_t nch, int64_t c
     FitOut<T> out;
     const int ip = g_err ? dt.h.intime : -1;
     if (g_err && prm.solver == 2)
-        fit_plain<T, N, N, true, XS_LEGACY>(mats<T>(dt), s, prm, park, out, ip);
+        plain_d<T, N, true, XS_LEGACY>(dt, s, prm, park, out, ip);
     else if (g_err && prm.solver == 3)
-        fit_plain<T, N, N, true, XS_LH>(mats<T>(dt), s, prm, park, out, ip);
+        plain_d<T, N, true, XS_LH>(dt, s, prm, park, out, ip);
     else if (g_err && prm.solver == 4)
-        fit_plain<T, N, N, true, XS_CHOL>(mats<T>(dt), s, prm, park, out, ip);
+        plain_d<T, N, true, XS_CHOL>(dt, s, prm, park, out, ip);
     else if (g_err)
-        fit_plain<T, N, N, true>(mats<T>(dt), s, prm, park, out, ip);
+        plain_d<T, N, true, XS_NONE>(dt, s, prm, park, out, ip);
     else if (prm.solver == 2)
-        fit_plain<T, N, N, false, XS_LEGACY>(mats<T>(dt), s, prm, park, out);
+        plain_d<T, N, false, XS_LEGACY>(dt, s, prm, park, out, -1);
     else if (prm.solver == 3)
-        fit_plain<T, N, N, false, XS_LH>(mats<T>(dt), s, prm, park, out);
+        plain_d<T, N, false, XS_LH>(dt, s, prm, park, out, -1);
     else if (prm.solver == 4)
-        fit_plain<T, N, N, false, XS_CHOL>(mats<T>(dt), s, prm, park, out);
+        plain_d<T, N, false, XS_CHOL>(dt, s, prm, park, out, -1);
     else
-        fit_plain<T, N, N>(mats<T>(dt), s, prm, park, out);
+        plain_d<T, N, false, XS_NONE>(dt, s, prm, park, out, -1);
     for (int p = 0; p < 10; ++p) x[p * nch + c] = p < N ? (double)out.x[p] : 0.0;
     if (g_err)
         for (int p = 0; p < 10; ++p) g_err[p * nch + c] = p < N ? (double)out.err[p] : 0.0;

@@ -135,6 +135,7 @@ def hostcore():
     L.hc_mask_inverse.argtypes = [C.c_void_p, C.c_int, dp, dp]
     L.hc_mask_inverse.restype = C.c_int
     L.hc_set_use_tables.argtypes = [C.c_int]
+    L.hc_set_layout.argtypes = [C.c_int]
     _cache["hostcore"] = L
     return L
 

@@ -223,3 +223,34 @@ def test_incremental_cholesky_core_single_problems(pkg):
                         if rr < best - 1e-12:
                             best, bx = rr, x
                 assert np.allclose(xg[:m], bx, rtol=1e-7, atol=1e-7 * max(1, np.abs(bx).max()))
+
+
+@pytest.mark.parametrize("cfg,pulse_cov", [("c1", "zero"), ("c2", "zero"), ("c2", "live"),
+                                           ("c3", "live"), ("c5", "live")])
+@pytest.mark.parametrize("mode", ["plain", "plain_errors", "plain_chol", "fast"])
+def test_layout_specialisation_is_bit_identical(pkg, cfg, pulse_cov, mode):
+    """The per-channel iterations built for the pulse layout D = 2 (the terms
+    of structurally zero rows of P, Pi and A left out) against the generic
+    build: every output identical bit for bit (an omitted term adds an exact
+    zero)."""
+    from oracle_binding import hostcore
+    from test_regression_amd import workloads, _abi
+    w = workloads.config(cfg, pulse_cov)
+    ct = w.chtypes()
+    s, t, _ = w.generate(3000)
+    if mode == "fast":
+        s = s.astype(np.float32).astype(np.float64)
+        o = _abi.default_opts(precision=_abi.MF_PREC_FP32)
+    else:
+        o = _abi.default_opts(compute_errors=int(mode == "plain_errors"),
+                              solver=_abi.MF_SOLVER_INPLACE_FNNLS_CHOL if mode == "plain_chol"
+                              else _abi.MF_SOLVER_INPLACE_FNNLS)
+    H = hostcore()
+    a = hostcore_fit(ct, s, t, o)
+    try:
+        H.hc_set_layout(2)
+        b = hostcore_fit(ct, s, t, o)
+    finally:
+        H.hc_set_layout(-1)
+    for u, v in zip(a, b):
+        assert np.array_equal(u, v, equal_nan=True)
