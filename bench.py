@@ -45,7 +45,7 @@ PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
 # factors that make the counters read a 256 MiB device copy as 256 MiB each
 # way.  Other workloads report null.
 MEASURED_TRAFFIC = {("c2", "zero", 1_000_000): {
-    "bytes": 105.4e6, "how": "rocprofv3 --pmc FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00), "
+    "bytes": 111.3e6, "how": "rocprofv3 --pmc FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00), "
                              "bench.py --traffic-probe, copy-calibrated"}}
 
 
@@ -381,7 +381,8 @@ def main():
     ap.add_argument("--no-live", dest="live", action="store_false",
                     help="skip the live-pulse-covariance block of the default workload")
     ap.add_argument("--probe-paths", default="warm,fast",
-                    help="paths of --traffic-probe, in order: any of plain, fast, warm")
+                    help="paths of --traffic-probe, in order: any of plain, plain_chol, fast, "
+                         "warm")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the multifit path has no cross-GPU exchange "
@@ -427,7 +428,10 @@ def main():
         torch.cuda.synchronize()
         probe = {"warm": (s32, O(precision=_abi.MF_PREC_FP32, warm_start=1), _abi.MF_PREC_FP32),
                  "fast": (s32, O(precision=_abi.MF_PREC_FP32), _abi.MF_PREC_FP32),
-                 "plain": (s64, O(precision=_abi.MF_PREC_FP64), _abi.MF_PREC_FP64)}
+                 "plain": (s64, O(precision=_abi.MF_PREC_FP64), _abi.MF_PREC_FP64),
+                 "plain_chol": (s64, O(precision=_abi.MF_PREC_FP64,
+                                       solver=_abi.MF_SOLVER_INPLACE_FNNLS_CHOL),
+                                _abi.MF_PREC_FP64)}
         for name in args.probe_paths.split(","):
             x_in, o, prec = probe[name]
             out = eng.alloc_result(nch, prec)
