@@ -1567,6 +1567,79 @@ MF_HD int fixed_cov_nmin(double chi2, const FitParams &prm, int &flags)
     return nmin;
 }
 
+// Phase 2's first Minimize iteration: the NNLS of the x = 0 normal equations
+// from the start x (phase 1's amplitudes or the prediction), with the cold
+// fallback of a warm start, and its chi2.  Returns false for a channel that
+// is not fitted (fast_p2_begin; S.x stays 0, chi2 0).
+template <int NS, int NP, bool TAB, typename Reload>
+MF_HD bool fast_p2_first(const DevType &ty, float (&s32)[10], const FitParams &prm,
+                         const float (&x)[10], bool warm, FnnlsStats &st, int &flags,
+                         ActiveState<double> &S, double &chi2, Reload &reload)
+{
+    double Atb64[10];
+    chi2 = 0.0;
+    reload(s32);
+    if (!fast_p2_begin<NS, NP>(ty, s32, prm, x, Atb64, S, flags)) return false;
+    const int outer0 = st.outer;
+    const bool from_empty = S.pm == 0;   // a start that is the cold start
+    const MaskTab *tab = *opaque(&ty.tab);   // wave-uniform: one scalar load
+    const double eps = prm.eps * prm.rho;
+    while (!fnnls_step<double, NP, true, TAB>(ty.d.AtA0, Atb64, S, eps, prm.eps_rel64,
+                                              prm.max_iter, st, 0.0, nullptr, tab)) {
+    }
+    flags |= F_REFINED;
+    if (warm && st.outer != outer0) flags |= F_WARM_REJECT;
+    if (warm && st.hit_max && !from_empty) {
+        // The cold start is the fallback: a prediction whose iterations
+        // reach max_iter is dropped and the channel fitted from x = 0.
+        warm_fallback_reset(S, st);
+        flags |= F_WARM_FALLBACK;
+        while (!fnnls_step<double, NP, true, TAB>(ty.d.AtA0, Atb64, S, eps, prm.eps_rel64,
+                                                  prm.max_iter, st, 0.0, nullptr, tab)) {
+        }
+    }
+    reload(s32);
+    chi2 = fast_p2_chi2<NS, NP>(ty, s32, S.x) / prm.rho;
+    return true;
+}
+
+// The fit after its first iteration: the Minimize count; with a live pulse
+// covariance the later iterations run the reference's FP64 iteration body
+// (per-channel covariance, inplace_fnnls with warm x and reset passive set).
+// They are FP64 because the reference decides them on gradients near its
+// 1e-11 threshold.
+template <int NS, int NP, bool LIVE, int D, typename Park>
+MF_HD int fast_p2_rest(const DevType &ty, const float (&s32)[10], const FitParams &prm,
+                       double (&x)[10], double &chi2, FnnlsStats &st, int &flags, Park &park)
+{
+    if (LIVE && prm.use_pi) {
+        if (fabs(chi2) < prm.chi2_tol) return 1;
+        double s64[10];
+#pragma unroll
+        for (int i = 0; i < 10; ++i) s64[i] = (double)s32[i];
+        return minimize_from<double, NS, NP, XS_NONE, D>(ty.d, s64, prm, park, x, chi2, 1, st,
+                                                         flags);
+    }
+    return fixed_cov_nmin(chi2, prm, flags);
+}
+
+MF_HD void fast_p2_out(const double (&x)[10], double chi2, int flags, int nmin,
+                       const FnnlsStats &st, FitOut<float> &out)
+{
+    if (st.hit_max) flags |= F_FNNLS_CAP;
+    if (st.llt_fail) flags |= F_LLT_FAIL;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        out.x[i] = (float)x[i];
+        out.err[i] = 0.0f;
+    }
+    out.chi2 = (float)chi2;
+    out.flags = flags;
+    out.n_min = nmin;
+    out.n_outer = st.outer;
+    out.n_inner = st.inner;
+}
+
 // Phase 2 and the rest of the fit (see fit_fast).  x: phase 1's amplitudes
 // (passive set = entries > 0).  Fills out.  reload(s32) is called before
 // each use of the samples (the start and chi2): a kernel can read them again
@@ -1580,62 +1653,12 @@ MF_HD void fast_phase2(const DevType &ty, float (&s32)[10], const FitParams &prm
                        const float (&x)[10], bool warm, FnnlsStats &st, int flags, Park &park,
                        FitOut<float> &out, Reload &&reload = KeepSamples{})
 {
-    double Atb64[10];
     ActiveState<double> S;
-    int nmin = 0;
     double chi2 = 0.0;
-    reload(s32);
-    if (fast_p2_begin<NS, NP>(ty, s32, prm, x, Atb64, S, flags)) {
-        const int outer0 = st.outer;
-        const bool from_empty = S.pm == 0;   // a start that is the cold start
-        const MaskTab *tab = *opaque(&ty.tab);   // wave-uniform: one scalar load
-        const double eps = prm.eps * prm.rho;
-        while (!fnnls_step<double, NP, true, TAB>(ty.d.AtA0, Atb64, S, eps, prm.eps_rel64,
-                                                  prm.max_iter, st, 0.0, nullptr, tab)) {
-        }
-        flags |= F_REFINED;
-        if (warm && st.outer != outer0) flags |= F_WARM_REJECT;
-        if (warm && st.hit_max && !from_empty) {
-            // The cold start is the fallback: a prediction whose iterations
-            // reach max_iter is dropped and the channel fitted from x = 0.
-            warm_fallback_reset(S, st);
-            flags |= F_WARM_FALLBACK;
-            while (!fnnls_step<double, NP, true, TAB>(ty.d.AtA0, Atb64, S, eps, prm.eps_rel64,
-                                                      prm.max_iter, st, 0.0, nullptr, tab)) {
-            }
-        }
-        reload(s32);
-        chi2 = fast_p2_chi2<NS, NP>(ty, s32, S.x) / prm.rho;
-        // With a live pulse covariance the later iterations run the
-        // reference's FP64 iteration body (per-channel covariance,
-        // inplace_fnnls with warm x and reset passive set); they are FP64
-        // because the reference decides them on gradients near its 1e-11
-        // threshold.
-        if (LIVE && prm.use_pi) {
-            nmin = 1;
-            if (!(fabs(chi2) < prm.chi2_tol)) {
-                double s64[10];
-#pragma unroll
-                for (int i = 0; i < 10; ++i) s64[i] = (double)s32[i];
-                nmin = minimize_from<double, NS, NP, XS_NONE, D>(ty.d, s64, prm, park, S.x,
-                                                                 chi2, 1, st, flags);
-            }
-        } else {
-            nmin = fixed_cov_nmin(chi2, prm, flags);
-        }
-    }
-    if (st.hit_max) flags |= F_FNNLS_CAP;
-    if (st.llt_fail) flags |= F_LLT_FAIL;
-#pragma unroll
-    for (int i = 0; i < 10; ++i) {
-        out.x[i] = (float)S.x[i];
-        out.err[i] = 0.0f;
-    }
-    out.chi2 = (float)chi2;
-    out.flags = flags;
-    out.n_min = nmin;
-    out.n_outer = st.outer;
-    out.n_inner = st.inner;
+    int nmin = 0;
+    if (fast_p2_first<NS, NP, TAB>(ty, s32, prm, x, warm, st, flags, S, chi2, reload))
+        nmin = fast_p2_rest<NS, NP, LIVE, D>(ty, s32, prm, S.x, chi2, st, flags, park);
+    fast_p2_out(S.x, chi2, flags, nmin, st, out);
 }
 
 // The fast fit of one channel in one call: phase 1 from a cold start, then

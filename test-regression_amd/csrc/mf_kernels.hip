@@ -261,14 +261,43 @@ template <typename TS, int N> struct SampleReload {
         for (int i = 0; i < 10; ++i) v[i] = i < N ? (float)q[(int64_t)i * nch + c] : 0.0f;
     }
 };
+// Hand-over of a channel after phase 2's first iteration, for types whose
+// pulse covariance is live: that iteration runs in a light kernel (four
+// waves per SIMD), the later iterations, which need about twice the
+// registers, in k_fast_p3.  `hand` holds the FP64 amplitudes [10][nch] and
+// chi2 [nch] (after them); flags and the FNNLS counters go to the outputs, as
+// between k_fast_p1 and k_fast_p2.  A channel that is not fitted gets its
+// final outputs here (its flags say so: F_NONFINITE or F_MINIMIZE_CAP, which
+// no fitted channel carries at this point).
+__device__ __forceinline__ void hand_over(bool begun, const ActiveState<double> &S, double chi2,
+                                          int flags, const FnnlsStats &st, int64_t c,
+                                          int64_t nch, double *__restrict__ hand,
+                                          float *__restrict__ xo, float *__restrict__ chi2o,
+                                          int32_t *__restrict__ flo, int32_t *__restrict__ ito)
+{
+    if (begun) {
+#pragma unroll
+        for (int p = 0; p < 10; ++p) hand[(int64_t)p * nch + c] = S.x[p];
+        hand[10 * nch + c] = chi2;
+    } else {
+#pragma unroll
+        for (int p = 0; p < 10; ++p) xo[(int64_t)p * nch + c] = 0.0f;
+        chi2o[c] = 0.0f;
+    }
+    flo[c] = flags | (st.hit_max ? F_FNNLS_CAP : 0) | (st.llt_fail ? F_LLT_FAIL : 0);
+    ito[c] = (st.outer << 8) | (st.inner << 16);
+}
+
 // Four waves per SIMD: at 128 VGPRs a few values spill (about 56 bytes per
 // lane, outside the solve loops); measured faster than 137 VGPRs at three.
-template <typename TS, int N, bool PED>
+// HO: live pulse covariance, the first iteration only (hand_over).
+template <typename TS, int N, bool PED, bool HO = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 void k_fast_fused(const DevType *__restrict__ types, SegArg seg,
                   const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
                   float *__restrict__ xo, float *__restrict__ chi2o, int32_t *__restrict__ flo,
-                  int32_t *__restrict__ ito, WarmIn warm, FitParams prm0)
+                  int32_t *__restrict__ ito, WarmIn warm, FitParams prm0,
+                  double *__restrict__ hand = nullptr)
 {
     int type_id;
     int64_t c;
@@ -285,6 +314,16 @@ void k_fast_fused(const DevType *__restrict__ types, SegArg seg,
         fast_phase1<N, N>(ty, s, prm, x, pm, st, flags);
 #pragma unroll
     for (int p = 0; p < 10; ++p) x[p] = (p < N && ((pm >> p) & 1u)) ? x[p] : 0.0f;
+    if (HO) {
+        ActiveState<double> S;
+        double chi2;
+        int fl = flags & F_FP64_PIVOT;
+        SampleReload<TS, N> rl{samples, nch, c};
+        const bool begun = fast_p2_first<N, N, true>(ty, s, prm, x, warm.pm != nullptr, st, fl,
+                                                     S, chi2, rl);
+        hand_over(begun, S, chi2, fl, st, c, nch, hand, xo, chi2o, flo, ito);
+        return;
+    }
     NoPark park;
     FitOut<float> out;
     fast_phase2<N, N, false>(ty, s, prm, x, warm.pm != nullptr, st, flags & F_FP64_PIVOT, park,
@@ -301,13 +340,13 @@ void k_fast_fused(const DevType *__restrict__ types, SegArg seg,
 // pass on MFMA, as k_mlp_warm), then the FP64 phase from that prediction.
 // The prediction never leaves the chip.  Blocks of FAST_WARM_BLOCK threads.
 constexpr int FAST_WARM_BLOCK = 256;
-template <typename TS, int N, bool PED>
+template <typename TS, int N, bool PED, bool HO = false>
 __global__ __launch_bounds__(FAST_WARM_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 void k_fast_warm(const DevType *__restrict__ types, SegArg seg,
                  const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
                  float *__restrict__ xo, float *__restrict__ chi2o, int32_t *__restrict__ flo,
                  int32_t *__restrict__ ito, const uint16_t *__restrict__ frag,
-                 const float *__restrict__ pq, FitParams prm0)
+                 const float *__restrict__ pq, FitParams prm0, double *__restrict__ hand = nullptr)
 {
     __shared__ mlp::MlpLds L;
     __shared__ __bf16 xs[FAST_WARM_BLOCK / 64][64][16];
@@ -336,6 +375,15 @@ void k_fast_warm(const DevType *__restrict__ types, SegArg seg,
     for (int i = 0; i < 10; ++i) x[i] = (i < N && ((pm >> i) & 1u)) ? fmaxf(xp[i], 1e-6f) : 0.0f;
     // a warm start goes straight to the FP64 phase (see k_fast_p1)
     FnnlsStats st;
+    if (HO) {
+        ActiveState<double> S;
+        double chi2;
+        int fl = 0;
+        SampleReload<TS, N> rl{samples, nch, c};
+        const bool begun = fast_p2_first<N, N, true>(ty, s, prm, x, true, st, fl, S, chi2, rl);
+        hand_over(begun, S, chi2, fl, st, c, nch, hand, xo, chi2o, flo, ito);
+        return;
+    }
     NoPark park;
     FitOut<float> out;
     fast_phase2<N, N, false>(ty, s, prm, x, true, st, 0, park, out,
@@ -375,6 +423,49 @@ void k_fast_p2(const DevType *__restrict__ types, SegArg seg,
     FitOut<float> out;
     fast_phase2<N, N, LIVE, true, D>(types[type_id], s, prm, x, warm != 0, st, f1 & F_FP64_PIVOT,
                                      park, out);
+#pragma unroll
+    for (int p = 0; p < 10; ++p) xo[(int64_t)p * nch + c] = out.x[p];
+    chi2o[c] = out.chi2;
+    flo[c] = out.flags;
+    ito[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
+}
+
+// The later Minimize iterations of a live pulse covariance after a hand-over
+// (k_fast_warm / k_fast_fused with HO): the channel's FP64 amplitudes and
+// chi2 from `hand`, its flags and counters from the outputs; the rest of the
+// fit as k_fast_p2 runs it after its first iteration (fast_p2_rest).
+template <typename TS, int N, bool PED, int D>
+__global__ __launch_bounds__(256)
+void k_fast_p3(const DevType *__restrict__ types, SegArg seg,
+               const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
+               float *__restrict__ xo, float *__restrict__ chi2o, int32_t *__restrict__ flo,
+               int32_t *__restrict__ ito, const double *__restrict__ hand, FitParams prm0)
+{
+    extern __shared__ unsigned char lds_raw[];
+    int type_id;
+    int64_t c;
+    if (!seg_item(seg, type_id, c)) return;
+    const int f1 = flo[c], it1 = ito[c];
+    if (f1 & (F_NONFINITE | F_MINIMIZE_CAP)) return;   // not fitted: outputs written
+    const FitParams prm = lane_params<PED>(prm0, types[type_id], ped, c);
+    float s[10];
+    double x[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        s[i] = i < N ? (float)samples[(int64_t)i * nch + c] : 0.0f;
+        x[i] = hand[(int64_t)i * nch + c];
+    }
+    double chi2 = hand[10 * nch + c];
+    FnnlsStats st;
+    st.outer = (it1 >> 8) & 255;
+    st.inner = (it1 >> 16) & 0xFFFF;
+    st.hit_max = (f1 & F_FNNLS_CAP) != 0;
+    st.llt_fail = (f1 & F_LLT_FAIL) != 0;
+    int flags = f1 & ~(F_FNNLS_CAP | F_LLT_FAIL);
+    LdsPark<double> park{reinterpret_cast<double *>(lds_raw) + threadIdx.x, (int)blockDim.x};
+    const int nmin = fast_p2_rest<N, N, true, D>(types[type_id], s, prm, x, chi2, st, flags, park);
+    FitOut<float> out;
+    fast_p2_out(x, chi2, flags, nmin, st, out);
 #pragma unroll
     for (int p = 0; p < 10; ++p) xo[(int64_t)p * nch + c] = out.x[p];
     chi2o[c] = out.chi2;
@@ -1104,9 +1195,12 @@ struct mf_context {
     mf::mlp::Scratch mlp_w;
     float *pred = nullptr;  // warm-start prediction: x [10][nch], then pm [nch] (uint32)
     int64_t pred_cap = 0;
+    double *hand = nullptr;  // live-covariance hand-over: x [10][nch] and chi2 [nch] in FP64
+    int64_t hand_cap = 0;
     // live mf_graph objects whose kernels point into the prediction / the
-    // partition scratch (that scratch must not move while they exist)
-    int graphs_pred = 0, graphs_part = 0;
+    // partition / the hand-over scratch (that scratch must not move while
+    // they exist)
+    int graphs_pred = 0, graphs_part = 0, graphs_hand = 0;
     // every live graph of the context: their kernels also point into the
     // type tables, so mf_set_types is refused while any exists, and
     // mf_context_destroy detaches them (a detached graph refuses to launch)
@@ -1130,7 +1224,7 @@ struct mf_graph {
     hipGraphExec_t exec = nullptr;
     int device = 0;
     mf_context *ctx = nullptr;
-    bool pred = false, part = false;  // scratch its kernels use
+    bool pred = false, part = false, hand = false;  // scratch its kernels use
 };
 
 static void time_mark(mf_context *ctx, int slot, hipStream_t st)
@@ -1321,7 +1415,8 @@ extern "C" mf_status mf_context_destroy(mf_context *ctx)
                     (void *)ctx->mlp_m, (void *)ctx->mlp_v, (void *)ctx->mlp_bf,
                     (void *)ctx->mlp_w.X, (void *)ctx->mlp_w.m, (void *)ctx->mlp_w.Z1,
                     (void *)ctx->mlp_w.H1, (void *)ctx->mlp_w.Z2, (void *)ctx->mlp_w.H2,
-                    (void *)ctx->mlp_w.Z3, (void *)ctx->mlp_w.lossdummy, (void *)ctx->pred})
+                    (void *)ctx->mlp_w.Z3, (void *)ctx->mlp_w.lossdummy, (void *)ctx->pred,
+                    (void *)ctx->hand})
         if (p) (void)hipFree(p);
     delete ctx;
     return MF_OK;
@@ -1414,11 +1509,37 @@ template <typename TS, int N, bool LIVE, bool PED>
 static mf_status launch_fast(mf_context *ctx, const SegArg &seg, const void *samples,
                              const double *ped, int64_t nch, const mf_result *out,
                              const FitParams &prm, int block, WarmIn warm, bool persistent,
-                             unsigned *queue, hipStream_t st, bool mlp_here = false)
+                             unsigned *queue, hipStream_t st, bool mlp_here = false,
+                             double *hand = nullptr)
 {
     const int64_t nt = seg_threads(seg);
     if (nt <= 0) return MF_OK;
     if (!out->iters) return fail(MF_ERR_INVALID, "the FP32 path needs mf_result.iters");
+    if (LIVE && hand) {
+        // first iteration in a light kernel, the later ones in k_fast_p3
+        float *xo = static_cast<float *>(out->x), *c2 = static_cast<float *>(out->chi2);
+        if (mlp_here) {
+            const int64_t g = (nt + FAST_WARM_BLOCK - 1) / FAST_WARM_BLOCK;
+            hipLaunchKernelGGL((k_fast_warm<TS, N, PED, true>), dim3((unsigned)g),
+                               dim3(FAST_WARM_BLOCK), 0, st, ctx->d_types, seg,
+                               static_cast<const TS *>(samples), ped, nch, xo, c2, out->flags,
+                               out->iters, ctx->mlp_frag, ctx->mlp_pq, prm, hand);
+        } else {
+            const int64_t g = (nt + block - 1) / block;
+            hipLaunchKernelGGL((k_fast_fused<TS, N, PED, true>), dim3((unsigned)g), dim3(block), 0,
+                               st, ctx->d_types, seg, static_cast<const TS *>(samples), ped, nch,
+                               xo, c2, out->flags, out->iters, warm, prm, hand);
+        }
+        HIP_TRY(hipGetLastError());
+        const int64_t g = (nt + block - 1) / block;
+        const size_t lds = sizeof(double) * 55 * (size_t)block;
+        auto p3 = seg_layout2(ctx, seg) ? k_fast_p3<TS, N, PED, 2> : k_fast_p3<TS, N, PED, -1>;
+        hipLaunchKernelGGL(p3, dim3((unsigned)g), dim3(block), lds, st, ctx->d_types, seg,
+                           static_cast<const TS *>(samples), ped, nch, xo, c2, out->flags,
+                           out->iters, static_cast<const double *>(hand), prm);
+        HIP_TRY(hipGetLastError());
+        return MF_OK;
+    }
     if (mlp_here) {
         if (LIVE || persistent) return fail(MF_ERR_INVALID, "in-kernel warm start: not for this launch");
         const int64_t g = (nt + FAST_WARM_BLOCK - 1) / FAST_WARM_BLOCK;
@@ -1486,7 +1607,7 @@ template <typename TS>
 static mf_status dispatch_fast(mf_context *ctx, const SegArg &seg, const mf_batch *in,
                                const mf_result *out, const FitParams &prm, int block,
                                WarmIn warm, bool persistent, unsigned *queue, hipStream_t st,
-                               bool mlp_here)
+                               bool mlp_here, double *hand)
 {
     const mf_chtype &t = ctx->types[(size_t)seg_type0(seg)];
     const bool live = type_live(t, prm);
@@ -1494,12 +1615,14 @@ static mf_status dispatch_fast(mf_context *ctx, const SegArg &seg, const mf_batc
         constexpr bool P = decltype(ped)::value;
         if (t.nsamples == 10)
             return live ? launch_fast<TS, 10, true, P>(ctx, seg, in->samples, in->pederr, in->nch,
-                                                       out, prm, block, warm, persistent, queue, st)
+                                                       out, prm, block, warm, persistent, queue, st,
+                                                       mlp_here && hand, hand)
                         : launch_fast<TS, 10, false, P>(ctx, seg, in->samples, in->pederr,
                                                         in->nch, out, prm, block, warm,
                                                         persistent, queue, st, mlp_here);
         return live ? launch_fast<TS, 8, true, P>(ctx, seg, in->samples, in->pederr, in->nch, out,
-                                                  prm, block, warm, persistent, queue, st)
+                                                  prm, block, warm, persistent, queue, st,
+                                                  mlp_here && hand, hand)
                     : launch_fast<TS, 8, false, P>(ctx, seg, in->samples, in->pederr, in->nch, out,
                                                    prm, block, warm, persistent, queue, st,
                                                    mlp_here);
@@ -1541,16 +1664,19 @@ struct CallScratch {
     int32_t *idx = nullptr;     // nch entries: the types' index lists
     float *pred = nullptr;      // 11 nch floats: warm-start prediction, x [10][nch] then pm [nch]
     unsigned *queue = nullptr;  // 2 heads: the persistent kernels' work queues
+    double *hand = nullptr;     // 11 nch doubles: live-covariance hand-over (hand_over)
 };
 // What a call needs of it.
 struct ScratchNeed {
-    bool part = false, pred = false, queue = false;
+    bool part = false, pred = false, queue = false, hand = false;
     int64_t nch = 0;
 };
+// hand-over of the live-covariance fast path: FP64 amplitudes and chi2
+constexpr size_t HAND_BYTES_PER_CH = sizeof(double) * 11;
 // Offsets of the parts in one contiguous region (256-byte aligned): queue
 // heads, partition counts, index lists, prediction.
 static size_t align256(size_t v) { return (v + 255) / 256 * 256; }
-static size_t scratch_layout(const ScratchNeed &n, size_t off[4])
+static size_t scratch_layout(const ScratchNeed &n, size_t off[5])
 {
     size_t k = 0;
     off[0] = k;
@@ -1561,6 +1687,8 @@ static size_t scratch_layout(const ScratchNeed &n, size_t off[4])
     if (n.part) k += align256(sizeof(int32_t) * (size_t)n.nch);
     off[3] = k;
     if (n.pred) k += align256(sizeof(float) * 11 * (size_t)n.nch);
+    off[4] = k;
+    if (n.hand) k += align256(HAND_BYTES_PER_CH * (size_t)n.nch);
     return k;
 }
 
@@ -1591,10 +1719,21 @@ static mf_status ctx_scratch(mf_context *ctx, const ScratchNeed &n, CallScratch 
             ctx->idx_cap = n.nch;
         }
     }
+    if (n.hand && ctx->hand_cap < n.nch) {
+        if (ctx->graphs_hand)
+            return fail(MF_ERR_INVALID, "the scratch would grow under a captured graph; "
+                                        "destroy the context's graphs first");
+        if (ctx->hand) HIP_TRY(hipFree(ctx->hand));
+        ctx->hand = nullptr;
+        ctx->hand_cap = 0;
+        HIP_TRY(hipMalloc(&ctx->hand, HAND_BYTES_PER_CH * (size_t)n.nch));
+        ctx->hand_cap = n.nch;
+    }
     sc.part = ctx->d_part;
     sc.idx = ctx->d_idx;
     sc.pred = ctx->pred;
     sc.queue = ctx->d_queue;
+    sc.hand = n.hand ? ctx->hand : nullptr;
     return MF_OK;
 }
 
@@ -1605,8 +1744,13 @@ struct CallPlan {
     bool pers = false, mlp_here = false;
     ScratchNeed need;
 };
+// split: types with a live pulse covariance run the fast path's first
+// iteration and the later ones in two kernels (hand_over), with FP64 state
+// between them in scratch; without it (the allocation-free entry point,
+// whose fixed scratch would not hold 88 B per channel of a large batch) both
+// run in k_fast_p2, with the same results.
 static mf_status plan_call(mf_context *ctx, const mf_batch *in, const mf_result *out,
-                           const mf_opts *opts, CallPlan &pl)
+                           const mf_opts *opts, CallPlan &pl, bool split = true)
 {
     if (!ctx || !in || !out || !opts) return fail(MF_ERR_INVALID, "NULL argument");
     if (ctx->types.empty()) return fail(MF_ERR_INVALID, "no channel types set (mf_set_types)");
@@ -1661,12 +1805,16 @@ static mf_status plan_call(mf_context *ctx, const mf_batch *in, const mf_result 
         return fail(MF_ERR_INVALID, "compute_errors needs mf_result.err");
     pl.pers = opts->persistent == 1 ||
               (opts->persistent == MF_PERSISTENT_AUTO && ctx->tune.use_persistent);
-    // A warm start without a live pulse covariance in any type runs the MLP
-    // inside the fit kernel; otherwise the MLP kernel writes the prediction
-    // for all channels first.
-    pl.mlp_here = opts->warm_start && !pl.pers;
-    for (const mf_chtype &t : ctx->types) pl.mlp_here = pl.mlp_here && !type_live(t, prm);
+    // A warm start runs the MLP inside the fit kernel (k_fast_warm) unless
+    // the persistent kernels run, or a type's live pulse covariance runs in
+    // k_fast_p2 (no split): then the MLP kernel writes the prediction for
+    // all channels first.
+    bool any_live = false;
+    for (const mf_chtype &t : ctx->types) any_live = any_live || type_live(t, prm);
+    const bool fast = opts->precision == MF_PREC_FP32;
+    pl.mlp_here = opts->warm_start && !pl.pers && (!any_live || split);
     pl.need.nch = in->nch;
+    pl.need.hand = fast && any_live && split;
     pl.need.pred = opts->warm_start && !pl.mlp_here;
     pl.need.part = in->type_of != nullptr && in->nranges == 0;
     pl.need.queue = opts->precision == MF_PREC_FP32 && pl.pers;
@@ -1728,9 +1876,9 @@ static mf_status run_call(mf_context *ctx, const mf_batch *in, const mf_result *
                                         st);
         return in->sample_dtype == MF_DTYPE_F64
                    ? dispatch_fast<double>(ctx, seg, in, out, prm, block, warm, pers, sc.queue, st,
-                                           mlp_here)
+                                           mlp_here, sc.hand)
                    : dispatch_fast<float>(ctx, seg, in, out, prm, block, warm, pers, sc.queue, st,
-                                          mlp_here);
+                                          mlp_here, sc.hand);
     };
     ctx->n_fit_ev = 0;
     auto go = [&](const SegArg &seg) {
@@ -1830,9 +1978,9 @@ extern "C" mf_status mf_fit_batch_noalloc(mf_context *ctx, const mf_batch *in,
                                           const mf_result *out, const mf_opts *opts, void *stream)
 {
     CallPlan pl;
-    mf_status r = plan_call(ctx, in, out, opts, pl);
+    mf_status r = plan_call(ctx, in, out, opts, pl, false);
     if (r != MF_OK || in->nch == 0) return r;
-    size_t off[4];
+    size_t off[5];
     const size_t need = scratch_layout(pl.need, off);
     if (need > (size_t)MF_NOALLOC_SCRATCH_BYTES)
         return fail(MF_ERR_INVALID,
@@ -1897,10 +2045,14 @@ extern "C" mf_status mf_graph_capture(mf_context *ctx, const mf_batch *in, const
     h->exec = x;
     h->device = ctx->device;
     h->ctx = ctx;
+    CallPlan pl;
+    (void)plan_call(ctx, in, out, opts, pl);   // as the captured call: checked by its eager run
     h->pred = opts->warm_start != 0;
     h->part = in->type_of != nullptr && in->nranges == 0;
+    h->hand = pl.need.hand;
     ctx->graphs_pred += h->pred;
     ctx->graphs_part += h->part;
+    ctx->graphs_hand += h->hand;
     ctx->graphs.push_back(h);
     *graph = h;
     return MF_OK;
@@ -1927,6 +2079,7 @@ extern "C" mf_status mf_graph_destroy(mf_graph *graph)
         mf_context *c = graph->ctx;
         c->graphs_pred -= graph->pred;
         c->graphs_part -= graph->part;
+        c->graphs_hand -= graph->hand;
         for (size_t k = 0; k < c->graphs.size(); ++k)
             if (c->graphs[k] == graph) {
                 c->graphs.erase(c->graphs.begin() + (long)k);

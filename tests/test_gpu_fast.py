@@ -201,3 +201,35 @@ def test_persistent_edge_cases(pkg, gpu):
         assert torch.equal(r.chi2, q.chi2) and torch.equal(r.iters, q.iters)
         if n >= 5:
             assert r.flags[1].item() & _abi.FLAGS["nonfinite"]
+
+
+@pytest.mark.parametrize("cfg,warm,persistent", [("c2", 0, 0), ("c2", 1, 0), ("c3", 1, 0),
+                                                 ("c3", 1, 1), ("c5", 1, 0)])
+def test_live_split_equals_single_kernel(pkg, gpu, cfg, warm, persistent):
+    """With a live pulse covariance mf_fit_batch runs the first iteration in
+    a light kernel and hands the FP64 state over to the later iterations'
+    kernel (k_fast_p3); mf_fit_batch_noalloc keeps both in k_fast_p2.  The
+    same arithmetic: outputs identical bit for bit (warm starts through the
+    in-kernel MLP, or through the prediction kernel when persistent)."""
+    from test_regression_amd import engine, workloads, _abi, warmstart
+    w = workloads.config(cfg, "live")
+    eng = engine.Engine(list(w.chtypes()))
+    if warm:
+        warmstart.ensure_trained(eng, path=warmstart.weights_for(cfg))
+    kw = {}
+    if w.event:
+        s, _, _ = w.generate(events=1)
+        kw["type_ranges"] = [0, w.event[0], sum(w.event)]
+        t = None
+    else:
+        s, t, _ = w.generate(60_000)
+        t = torch.from_numpy(t).cuda() if t is not None else None
+    x = torch.from_numpy(s.astype(np.float32)).cuda()
+    print(f"[gpu-fast] live split cfg={cfg} warm={warm} persistent={persistent}", flush=True)
+    o = _abi.default_opts(precision=_abi.MF_PREC_FP32, warm_start=warm, persistent=persistent)
+    a = eng.fit(x, t, o, **kw)
+    b = eng.fit_noalloc(x, t, o, **kw)
+    torch.cuda.synchronize()
+    assert torch.equal(a.x, b.x) and torch.equal(a.chi2, b.chi2)
+    assert torch.equal(a.flags, b.flags) and torch.equal(a.iters, b.iters)
+    assert ((a.iters.cpu().numpy() & 255) > 1).mean() > 0.5   # later iterations ran
