@@ -1108,13 +1108,184 @@ MF_HD T iteration_solve(const TypeMats<T> &tm, const T (&s)[10], const FitParams
     return chisq<T, NS, NP, D>(tm, Li, s, x, zc);
 }
 
-template <typename T, int NS, int NP, int XS = XS_NONE, int D = -1, typename Park>
+// The stop test that opens an inplace_fnnls call whose passive set starts
+// empty (nnls/src/inplace_fnnls.cu:140-147: max_i w_i < eps), at x, with
+// w = A^T (b - A x) formed from the residual instead of the normal
+// equations: w = P^T C^-1 (s - P x) = -P^T L^-T u, u = L^-1 (P x - s).  Also
+// gives |u|^2, the chi2 at x (ComputeChiSq).  It is the reference's
+// Atb - AtA x in exact arithmetic; in rounding it differs, which can move
+// only a decision taken within rounding of eps.  L: the covariance's
+// Cholesky factor (cholesky_inplace: ri the reciprocal diagonal).
+template <typename T, int NS, int NP, int D>
+MF_HD void forward_subst(const T (&L)[10][10], const T (&ri)[10], T (&y)[10])
+{
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        T v = y[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) v = mfma(-L[i][k], y[k], v);
+        y[i] = v * ri[i];
+    }
+}
+template <typename T, int NS>
+MF_HD void backward_subst(const T (&L)[10][10], const T (&ri)[10], T (&y)[10])
+{
+#pragma unroll
+    for (int i = NS - 1; i >= 0; --i) {
+        T v = y[i];
+#pragma unroll
+        for (int k = i + 1; k < NS; ++k) v = mfma(-L[k][i], y[k], v);
+        y[i] = v * ri[i];
+    }
+}
+// y_p = (P^T v)_p
+template <typename T, int NS, int NP, int D>
+MF_HD void pt_times(const TypeMats<T> &tm, const T (&v)[10], T (&y)[10])
+{
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        T t = T(0);
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+            if (i >= lay_first<D>(p)) t = mfma(tm.P[i][p], v[i], t);
+        y[p] = t;
+    }
+}
+
+// u = L^-1 (P x - s), w = -P^T L^-T u (see above), chi2 = |u|^2; returns
+// whether max_p w_p < eps.
+template <typename T, int NS, int NP, int D>
+MF_HD bool stop_at_start(const TypeMats<T> &tm, const T (&s)[10], const T (&x)[10],
+                         const T (&L)[10][10], const T (&ri)[10], T eps, T &chi2, T (&u)[10],
+                         T (&w)[10])
+{
+#pragma unroll
+    for (int i = 0; i < 10; ++i) u[i] = T(0);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        T t = T(0);
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+            if (i >= lay_first<D>(p)) t = mfma(tm.P[i][p], x[p], t);
+        u[i] = t - s[i];
+    }
+    forward_subst<T, NS, NP, D>(L, ri, u);
+    chi2 = T(0);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) chi2 = mfma(u[i], u[i], chi2);
+    T v[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) v[i] = u[i];
+    backward_subst<T, NS>(L, ri, v);
+    pt_times<T, NS, NP, D>(tm, v, w);
+    bool stop = true;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        w[p] = -w[p];
+        stop = stop && (w[p] < eps);
+    }
+    return stop;
+}
+
+// One add and one solve, what the later iterations of the fast path do
+// when the stop test fails: the reference's inplace_fnnls from the warm x
+// with its passive set reset adds the index q of the largest w (the first
+// of equal ones), solves the passive system of {q} alone (x_q = Atb_q /
+// AtA_qq by the masked factorisation's steps; the other amplitudes kept)
+// and stops when no free w reaches eps (nnls/src/inplace_fnnls.cu:140-233).
+// Formed from the factor L and the u, w of stop_at_start: a_q = L^-1 P_q,
+// Atb_q = a_q . L^-1 s, AtA_qq = a_q . a_q, the new w = w - (P^T L^-T a_q)
+// dx_q, the new chi2 = |u + dx_q a_q|^2.  Returns false when the channel
+// leaves that case (the solve is not positive, a second index is added,
+// max_iter < 2); x is then unchanged and the caller runs the iteration in
+// full.
+template <typename T, int NS, int NP, int D>
+MF_HD bool single_add(const TypeMats<T> &tm, const T (&s)[10], T (&x)[10], const T (&L)[10][10],
+                      const T (&ri)[10], const T (&u)[10], const T (&w)[10], T eps,
+                      int max_iter, T &chi2)
+{
+    if (max_iter < 2) return false;
+    int q = 0;
+    T wq = w[0];
+#pragma unroll
+    for (int p = 1; p < NP; ++p)
+        if (w[p] > wq) {
+            wq = w[p];
+            q = p;
+        }
+    T a[10], b[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        T c = T(0);
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+            if (i < NS && i >= lay_first<D>(p)) c = (p == q) ? tm.P[i][p] : c;
+        a[i] = c;
+        b[i] = i < NS ? s[i] : T(0);
+    }
+    forward_subst<T, NS, NP, D>(L, ri, a);
+    forward_subst<T, NS, NP, D>(L, ri, b);
+    T att = T(0), atb = T(0);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        att = mfma(a[i], a[i], att);
+        atb = mfma(a[i], b[i], atb);
+    }
+    if (!(att > T(0))) return false;
+    const T r = mrsqrt(att);
+    const T sq = (atb * r) * r;
+    if (!(sq > T(0))) return false;
+    const T dx = sq - x[q];
+    T g[10], c[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) g[i] = a[i];
+    backward_subst<T, NS>(L, ri, g);
+    pt_times<T, NS, NP, D>(tm, g, c);   // column q of AtA
+    bool stop = true;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) stop = stop && (p == q || mfma(-c[p], dx, w[p]) < eps);
+    if (!stop) return false;
+    x[q] = sq;
+    chi2 = T(0);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        const T e = mfma(dx, a[i], u[i]);
+        chi2 = mfma(e, e, chi2);
+    }
+    return true;
+}
+
+// LAZY (the fast path's later iterations, inplace_fnnls): the stop test is
+// taken first from the residual (stop_at_start); a channel that passes it
+// keeps x and gets its chi2 without the normal equations, and one that adds
+// a single index and stops (single_add) needs only that index's column.
+// Only the channels left form the normal equations.
+template <typename T, int NS, int NP, int XS = XS_NONE, int D = -1, bool LAZY = false,
+          typename Park>
 MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams &prm,
                       Park &park, T (&x)[10], FnnlsStats &st, int &flags, int zc = -1,
                       LegacyState *ls = nullptr)
 {
     const TypeMats<T> &tm = *opaque(&tm0);
     T AtA[10][10], Atb[10];
+    if (LAZY && XS == XS_NONE && zc < 0 && prm.solver == 0) {
+        T L[10][10], ri[10], chi2, u[10], w[10];
+        build_cov<T, NS, NP, D>(tm, prm.use_pi != 0, x, L, T(prm.rho));
+        if (!cholesky_inplace<T, NS>(L, ri)) flags |= F_LLT_FAIL;
+        if (stop_at_start<T, NS, NP, D>(tm, s, x, L, ri, T(prm.eps), chi2, u, w)) return chi2;
+        if (single_add<T, NS, NP, D>(tm, s, x, L, ri, u, w, T(prm.eps), prm.max_iter, chi2)) {
+            st.outer++;
+            st.inner++;
+            return chi2;
+        }
+        invert_lower<T, NS>(L, ri);
+        normal_equations<T, NS, NP, D>(tm, L, s, AtA, Atb, zc);
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+#pragma unroll
+            for (int j = 0; j <= i; ++j) park.store(i * (i + 1) / 2 + j, L[i][j]);
+        return iteration_solve<T, NS, NP, XS, D>(tm, s, prm, park, AtA, Atb, x, st, zc, ls);
+    }
     iteration_problem<T, NS, NP, D>(tm, s, prm, park, x, AtA, Atb, flags, zc);
     return iteration_solve<T, NS, NP, XS, D>(tm, s, prm, park, AtA, Atb, x, st, zc, ls);
 }
@@ -1125,7 +1296,7 @@ MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams 
 // on x (off for the error refits, whose extra register state costs more
 // than the repeated set-up there).
 template <typename T, int NS, int NP, int XS = XS_NONE, int D = -1, bool KEEP = true,
-          typename Park>
+          bool LAZY = false, typename Park>
 MF_HD int minimize_from(const TypeMats<T> &tm, const T (&s)[10], const FitParams &prm,
                         Park &park, T (&x)[10], T &chi2, int it0, FnnlsStats &st, int &flags,
                         int zc = -1, LegacyState *ls = nullptr)
@@ -1149,7 +1320,8 @@ MF_HD int minimize_from(const TypeMats<T> &tm, const T (&s)[10], const FitParams
         return it;
     }
     for (; it < prm.max_minimize; ++it) {                       // :154
-        const T now = minimize_step<T, NS, NP, XS, D>(tm, s, prm, park, x, st, flags, zc, ls);
+        const T now =
+            minimize_step<T, NS, NP, XS, D, LAZY>(tm, s, prm, park, x, st, flags, zc, ls);
         const T d = now - chi2;
         chi2 = now;
         if (mabs(d) < T(prm.chi2_tol)) return it + 1;           // :162
@@ -1617,8 +1789,8 @@ MF_HD int fast_p2_rest(const DevType &ty, const float (&s32)[10], const FitParam
         double s64[10];
 #pragma unroll
         for (int i = 0; i < 10; ++i) s64[i] = (double)s32[i];
-        return minimize_from<double, NS, NP, XS_NONE, D>(ty.d, s64, prm, park, x, chi2, 1, st,
-                                                         flags);
+        return minimize_from<double, NS, NP, XS_NONE, D, false, true>(ty.d, s64, prm, park, x,
+                                                                      chi2, 1, st, flags);
     }
     return fixed_cov_nmin(chi2, prm, flags);
 }
