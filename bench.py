@@ -67,7 +67,10 @@ MLP_FLOPS_PER_CHANNEL = 2 * (10 * 64 + 64 * 64 + 64 * 20)
 def fit_kernel_name(pulse_cov: str, warm: bool = True) -> str:
     """The fast path's fit kernels for a pulse-covariance mode."""
     if pulse_cov == "live":
-        return ("k_mlp_warm + " if warm else "") + "k_fast_p1 + k_fast_p2 (the step's kernels)"
+        return (("k_fast_warm (MLP and first iteration)" if warm else
+                 "k_fast_fused (first iteration)") +
+                " + k_fast_p3 (later iterations) + k_fast_p4 (channels needing the normal "
+                "equations): the step's kernels")
     if warm:
         return "k_fast_warm (MLP prediction and FP64 active-set phase in one launch)"
     return "k_fast_fused (both phases in one launch)"

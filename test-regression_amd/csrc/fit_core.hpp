@@ -1259,35 +1259,53 @@ MF_HD bool single_add(const TypeMats<T> &tm, const T (&s)[10], T (&x)[10], const
 // taken first from the residual (stop_at_start); a channel that passes it
 // keeps x and gets its chi2 without the normal equations, and one that adds
 // a single index and stops (single_add) needs only that index's column.
-// Only the channels left form the normal equations.
+// Only the channels left form the normal equations.  *kept: set when the
+// stop test passed at the start (x unchanged).
+// ONLY (with LAZY): the iteration is not run in full; a channel that needs
+// the normal equations sets *pend and keeps x, for a kernel that holds them
+// (so that this one needs neither their registers nor L^{-1}'s park).
 template <typename T, int NS, int NP, int XS = XS_NONE, int D = -1, bool LAZY = false,
-          typename Park>
+          bool ONLY = false, typename Park>
 MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams &prm,
                       Park &park, T (&x)[10], FnnlsStats &st, int &flags, int zc = -1,
-                      LegacyState *ls = nullptr)
+                      LegacyState *ls = nullptr, bool *kept = nullptr, bool *pend = nullptr)
 {
     const TypeMats<T> &tm = *opaque(&tm0);
-    T AtA[10][10], Atb[10];
     if (LAZY && XS == XS_NONE && zc < 0 && prm.solver == 0) {
         T L[10][10], ri[10], chi2, u[10], w[10];
         build_cov<T, NS, NP, D>(tm, prm.use_pi != 0, x, L, T(prm.rho));
         if (!cholesky_inplace<T, NS>(L, ri)) flags |= F_LLT_FAIL;
-        if (stop_at_start<T, NS, NP, D>(tm, s, x, L, ri, T(prm.eps), chi2, u, w)) return chi2;
+        if (stop_at_start<T, NS, NP, D>(tm, s, x, L, ri, T(prm.eps), chi2, u, w)) {
+            if (kept) *kept = true;
+            return chi2;
+        }
         if (single_add<T, NS, NP, D>(tm, s, x, L, ri, u, w, T(prm.eps), prm.max_iter, chi2)) {
             st.outer++;
             st.inner++;
             return chi2;
         }
-        invert_lower<T, NS>(L, ri);
-        normal_equations<T, NS, NP, D>(tm, L, s, AtA, Atb, zc);
+        if constexpr (ONLY) {
+            *pend = true;
+            return chi2;
+        } else {
+            T AtA[10][10], Atb[10];
+            invert_lower<T, NS>(L, ri);
+            normal_equations<T, NS, NP, D>(tm, L, s, AtA, Atb, zc);
 #pragma unroll
-        for (int i = 0; i < NS; ++i)
+            for (int i = 0; i < NS; ++i)
 #pragma unroll
-            for (int j = 0; j <= i; ++j) park.store(i * (i + 1) / 2 + j, L[i][j]);
+                for (int j = 0; j <= i; ++j) park.store(i * (i + 1) / 2 + j, L[i][j]);
+            return iteration_solve<T, NS, NP, XS, D>(tm, s, prm, park, AtA, Atb, x, st, zc, ls);
+        }
+    }
+    if constexpr (ONLY) {
+        *pend = true;
+        return T(0);
+    } else {
+        T AtA[10][10], Atb[10];
+        iteration_problem<T, NS, NP, D>(tm, s, prm, park, x, AtA, Atb, flags, zc);
         return iteration_solve<T, NS, NP, XS, D>(tm, s, prm, park, AtA, Atb, x, st, zc, ls);
     }
-    iteration_problem<T, NS, NP, D>(tm, s, prm, park, x, AtA, Atb, flags, zc);
-    return iteration_solve<T, NS, NP, XS, D>(tm, s, prm, park, AtA, Atb, x, st, zc, ls);
 }
 
 // Minimize (PulseChiSqSNNLS.cpp:149-166) from iteration `it0` on, with the
@@ -1295,11 +1313,14 @@ MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams 
 // KEEP: form the normal equations once when the covariance does not depend
 // on x (off for the error refits, whose extra register state costs more
 // than the repeated set-up there).
+// ONLY (with LAZY, see minimize_step): when an iteration needs the normal
+// equations, *pend is set and the index of that iteration returned, with x
+// and chi2 as they were at its start.
 template <typename T, int NS, int NP, int XS = XS_NONE, int D = -1, bool KEEP = true,
-          bool LAZY = false, typename Park>
+          bool LAZY = false, bool ONLY = false, typename Park>
 MF_HD int minimize_from(const TypeMats<T> &tm, const T (&s)[10], const FitParams &prm,
                         Park &park, T (&x)[10], T &chi2, int it0, FnnlsStats &st, int &flags,
-                        int zc = -1, LegacyState *ls = nullptr)
+                        int zc = -1, LegacyState *ls = nullptr, bool *pend = nullptr)
 {
     int it = it0;
     if (KEEP && !prm.use_pi && it < prm.max_minimize) {
@@ -1320,11 +1341,24 @@ MF_HD int minimize_from(const TypeMats<T> &tm, const T (&s)[10], const FitParams
         return it;
     }
     for (; it < prm.max_minimize; ++it) {                       // :154
-        const T now =
-            minimize_step<T, NS, NP, XS, D, LAZY>(tm, s, prm, park, x, st, flags, zc, ls);
+        bool kept = false, pd = false;
+        const T now = minimize_step<T, NS, NP, XS, D, LAZY, ONLY>(tm, s, prm, park, x, st, flags,
+                                                                  zc, ls, &kept, &pd);
+        if (ONLY && pd) {
+            *pend = true;
+            return it;
+        }
         const T d = now - chi2;
         chi2 = now;
         if (mabs(d) < T(prm.chi2_tol)) return it + 1;           // :162
+        // LAZY: an iteration that kept x is followed by one that rebuilds the
+        // same covariance from the same x, passes the same stop test and
+        // returns the same chi2 bit for bit; its difference is 0, so Minimize
+        // ends after it (a finite chi2 and a positive tolerance).  It is
+        // counted, not run.
+        if (LAZY && kept && now - now == T(0) && T(prm.chi2_tol) > T(0) &&
+            it + 1 < prm.max_minimize)
+            return it + 2;
     }
     flags |= F_MINIMIZE_CAP;
     return it;
@@ -1793,6 +1827,43 @@ MF_HD int fast_p2_rest(const DevType &ty, const float (&s32)[10], const FitParam
                                                                       chi2, 1, st, flags);
     }
     return fixed_cov_nmin(chi2, prm, flags);
+}
+
+// fast_p2_rest in two parts, for kernels that cannot afford the registers of
+// the normal equations: fast_p2_rest_lazy runs the later iterations while
+// each one stops at its start or adds a single index (minimize_step, LAZY);
+// at the first that needs the normal equations it sets pend and returns that
+// iteration's index, x and chi2 as at its start, and fast_p2_resume goes on
+// from there with the whole iteration body.  Together they give
+// fast_p2_rest's results bit for bit.
+struct NullPark {
+    MF_HD void store(int, double) {}
+    MF_HD double load(int) const { return 0.0; }
+};
+template <int NS, int NP, int D>
+MF_HD int fast_p2_rest_lazy(const DevType &ty, const float (&s32)[10], const FitParams &prm,
+                            double (&x)[10], double &chi2, FnnlsStats &st, int &flags, bool &pend)
+{
+    pend = false;
+    if (!prm.use_pi) return fixed_cov_nmin(chi2, prm, flags);
+    if (fabs(chi2) < prm.chi2_tol) return 1;
+    double s64[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) s64[i] = (double)s32[i];
+    NullPark park;
+    return minimize_from<double, NS, NP, XS_NONE, D, false, true, true>(
+        ty.d, s64, prm, park, x, chi2, 1, st, flags, -1, nullptr, &pend);
+}
+template <int NS, int NP, int D, typename Park>
+MF_HD int fast_p2_resume(const DevType &ty, const float (&s32)[10], const FitParams &prm,
+                         double (&x)[10], double &chi2, int it, FnnlsStats &st, int &flags,
+                         Park &park)
+{
+    double s64[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) s64[i] = (double)s32[i];
+    return minimize_from<double, NS, NP, XS_NONE, D, false, true>(ty.d, s64, prm, park, x, chi2,
+                                                                  it, st, flags);
 }
 
 MF_HD void fast_p2_out(const double (&x)[10], double chi2, int flags, int nmin,

@@ -103,9 +103,12 @@ struct SegArg {
     int64_t rfirst[MAX_RANGES], rcount[MAX_RANGES], wave0[MAX_RANGES + 1];
 };
 // This thread's channel c and type t; false: no channel.  t is wave-uniform.
-__device__ __forceinline__ bool seg_item(const SegArg &a, int &t, int64_t &c)
+// *range (optional): the range's index within the launch (0 for a list).
+__device__ __forceinline__ bool seg_item(const SegArg &a, int &t, int64_t &c,
+                                         int *range = nullptr)
 {
     const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (range) *range = 0;
     if (a.nrange == 0) {
         t = a.type_id;
         if (gt >= seg_count(a.list)) return false;
@@ -119,6 +122,7 @@ __device__ __forceinline__ bool seg_item(const SegArg &a, int &t, int64_t &c)
     for (int k = 1; k < MAX_RANGES; ++k)
         if (k < a.nrange && w >= a.wave0[k]) r = k;
     r = __builtin_amdgcn_readfirstlane(r);
+    if (range) *range = r;
     t = __builtin_amdgcn_readfirstlane(a.rtype[r]);
     const int64_t g = (w - a.wave0[r]) * 64 + (gt & 63);
     if (w >= a.wave0[a.nrange] || g >= a.rcount[r]) return false;
@@ -431,20 +435,46 @@ void k_fast_p2(const DevType *__restrict__ types, SegArg seg,
 }
 
 // The later Minimize iterations of a live pulse covariance after a hand-over
-// (k_fast_warm / k_fast_fused with HO): the channel's FP64 amplitudes and
-// chi2 from `hand`, its flags and counters from the outputs; the rest of the
-// fit as k_fast_p2 runs it after its first iteration (fast_p2_rest).
+// (k_fast_warm / k_fast_fused with HO), in two kernels.  k_fast_p3 runs them
+// while each one stops at its start or adds a single index
+// (fast_p2_rest_lazy): no normal equations, so it needs half the registers
+// of the whole iteration body and no park for L^{-1}.  A channel that reaches
+// an iteration needing the normal equations leaves its state in `hand`, its
+// flags with F_PENDING and its iteration index in the outputs, and its item
+// in the pending list; k_fast_p4 finishes those channels with the whole body
+// (fast_p2_resume).  Results are those of fast_p2_rest bit for bit.
+//
+// Pending lists (after the hand-over's FP64 values): one count per range of
+// the launch (MAX_RANGES), cleared by a kernel before the first-iteration
+// launch, then the items (the index seg_item reads).  Range r files its
+// items from slot wave0[r] * 64 on (a single list from slot 0), so that the
+// lists need no shared count and k_fast_p4 takes one range per grid row.
+constexpr int F_PENDING = 1 << 30;   // between k_fast_p3 and k_fast_p4 only
+__host__ __device__ inline size_t hand_fp64_bytes(int64_t nch)
+{
+    return ((size_t)nch * 11 * sizeof(double) + 255) / 256 * 256;
+}
+// slots: at most nch items plus each range's last partial wavefront
+__host__ __device__ inline size_t hand_bytes(int64_t nch)
+{
+    return hand_fp64_bytes(nch) + 256 + sizeof(int32_t) * ((size_t)nch + 64 * MAX_RANGES);
+}
+__host__ __device__ inline unsigned *hand_pending(double *hand, int64_t nch)
+{
+    return reinterpret_cast<unsigned *>(reinterpret_cast<unsigned char *>(hand) +
+                                        hand_fp64_bytes(nch));
+}
+
 template <typename TS, int N, bool PED, int D>
 __global__ __launch_bounds__(256)
 void k_fast_p3(const DevType *__restrict__ types, SegArg seg,
                const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
                float *__restrict__ xo, float *__restrict__ chi2o, int32_t *__restrict__ flo,
-               int32_t *__restrict__ ito, const double *__restrict__ hand, FitParams prm0)
+               int32_t *__restrict__ ito, double *__restrict__ hand, FitParams prm0)
 {
-    extern __shared__ unsigned char lds_raw[];
-    int type_id;
+    int type_id, r;
     int64_t c;
-    if (!seg_item(seg, type_id, c)) return;
+    if (!seg_item(seg, type_id, c, &r)) return;
     const int f1 = flo[c], it1 = ito[c];
     if (f1 & (F_NONFINITE | F_MINIMIZE_CAP)) return;   // not fitted: outputs written
     const FitParams prm = lane_params<PED>(prm0, types[type_id], ped, c);
@@ -462,8 +492,22 @@ void k_fast_p3(const DevType *__restrict__ types, SegArg seg,
     st.hit_max = (f1 & F_FNNLS_CAP) != 0;
     st.llt_fail = (f1 & F_LLT_FAIL) != 0;
     int flags = f1 & ~(F_FNNLS_CAP | F_LLT_FAIL);
-    LdsPark<double> park{reinterpret_cast<double *>(lds_raw) + threadIdx.x, (int)blockDim.x};
-    const int nmin = fast_p2_rest<N, N, true, D>(types[type_id], s, prm, x, chi2, st, flags, park);
+    bool pend;
+    const int nmin = fast_p2_rest_lazy<N, N, D>(types[type_id], s, prm, x, chi2, st, flags, pend);
+    if (pend) {
+#pragma unroll
+        for (int p = 0; p < 10; ++p) hand[(int64_t)p * nch + c] = x[p];
+        hand[10 * nch + c] = chi2;
+        flo[c] = flags | F_PENDING | (st.hit_max ? F_FNNLS_CAP : 0) |
+                 (st.llt_fail ? F_LLT_FAIL : 0);
+        ito[c] = nmin | (st.outer << 8) | (st.inner << 16);
+        unsigned *pq = hand_pending(hand, nch);
+        const unsigned k = atomicAdd(pq + r, 1u);
+        const int64_t base = seg.nrange ? seg.wave0[r] * 64 : 0;
+        reinterpret_cast<int32_t *>(pq + 64)[base + k] =
+            (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+        return;
+    }
     FitOut<float> out;
     fast_p2_out(x, chi2, flags, nmin, st, out);
 #pragma unroll
@@ -471,6 +515,67 @@ void k_fast_p3(const DevType *__restrict__ types, SegArg seg,
     chi2o[c] = out.chi2;
     flo[c] = out.flags;
     ito[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
+}
+
+// The pending channels of k_fast_p3: grid row blockIdx.y takes range
+// blockIdx.y's list (one type: its tables are read with uniform addresses),
+// one wavefront per block, grid-stride over the list.  L^{-1} is parked in
+// LDS (55 doubles per lane).
+constexpr int FAST_P4_BLOCK = 64;
+template <typename TS, int N, bool PED, int D>
+__global__ __launch_bounds__(FAST_P4_BLOCK)
+void k_fast_p4(const DevType *__restrict__ types, SegArg seg,
+               const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
+               float *__restrict__ xo, float *__restrict__ chi2o, int32_t *__restrict__ flo,
+               int32_t *__restrict__ ito, double *__restrict__ hand, FitParams prm0)
+{
+    extern __shared__ unsigned char lds_raw[];
+    const int r = blockIdx.y;
+    const unsigned *pq = hand_pending(hand, nch);
+    const int64_t np = (int64_t)pq[r];
+    const int32_t *list =
+        reinterpret_cast<const int32_t *>(pq + 64) + (seg.nrange ? seg.wave0[r] * 64 : 0);
+    const int type_id = seg.nrange ? seg.rtype[r] : seg.type_id;
+    const DevType &ty = types[type_id];
+    for (int64_t k0 = (int64_t)blockIdx.x * FAST_P4_BLOCK; k0 < np;
+         k0 += (int64_t)gridDim.x * FAST_P4_BLOCK) {
+        const int64_t k = k0 + threadIdx.x;
+        if (k >= np) continue;
+        // the item's channel: as seg_item maps the item's thread index
+        const int64_t gt = list[k];
+        int64_t c;
+        if (seg.nrange == 0) {
+            const int32_t *idx = seg_list(seg.list);
+            c = idx ? (int64_t)idx[gt] : gt;
+        } else {
+            c = seg.rfirst[r] + (gt - seg.wave0[r] * 64);
+        }
+        const int f1 = flo[c], it1 = ito[c];
+        const FitParams prm = lane_params<PED>(prm0, ty, ped, c);
+        float s[10];
+        double x[10];
+#pragma unroll
+        for (int i = 0; i < 10; ++i) {
+            s[i] = i < N ? (float)samples[(int64_t)i * nch + c] : 0.0f;
+            x[i] = hand[(int64_t)i * nch + c];
+        }
+        double chi2 = hand[10 * nch + c];
+        FnnlsStats st;
+        st.outer = (it1 >> 8) & 255;
+        st.inner = (it1 >> 16) & 0xFFFF;
+        st.hit_max = (f1 & F_FNNLS_CAP) != 0;
+        st.llt_fail = (f1 & F_LLT_FAIL) != 0;
+        int flags = f1 & ~(F_PENDING | F_FNNLS_CAP | F_LLT_FAIL);
+        LdsPark<double> park{reinterpret_cast<double *>(lds_raw) + threadIdx.x, FAST_P4_BLOCK};
+        const int nmin = fast_p2_resume<N, N, D>(ty, s, prm, x, chi2, it1 & 255, st, flags, park);
+        FitOut<float> out;
+        fast_p2_out(x, chi2, flags, nmin, st, out);
+#pragma unroll
+        for (int p = 0; p < 10; ++p) xo[(int64_t)p * nch + c] = out.x[p];
+        chi2o[c] = out.chi2;
+        flo[c] = out.flags;
+        ito[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1195,7 +1300,8 @@ struct mf_context {
     mf::mlp::Scratch mlp_w;
     float *pred = nullptr;  // warm-start prediction: x [10][nch], then pm [nch] (uint32)
     int64_t pred_cap = 0;
-    double *hand = nullptr;  // live-covariance hand-over: x [10][nch] and chi2 [nch] in FP64
+    double *hand = nullptr;  // live-covariance hand-over: x [10][nch] and chi2 [nch] in FP64,
+                             // then the pending list (hand_bytes)
     int64_t hand_cap = 0;
     // live mf_graph objects whose kernels point into the prediction / the
     // partition / the hand-over scratch (that scratch must not move while
@@ -1516,8 +1622,11 @@ static mf_status launch_fast(mf_context *ctx, const SegArg &seg, const void *sam
     if (nt <= 0) return MF_OK;
     if (!out->iters) return fail(MF_ERR_INVALID, "the FP32 path needs mf_result.iters");
     if (LIVE && hand) {
-        // first iteration in a light kernel, the later ones in k_fast_p3
+        // first iteration in a light kernel, the later ones in k_fast_p3 and,
+        // for the channels that need the normal equations, k_fast_p4
         float *xo = static_cast<float *>(out->x), *c2 = static_cast<float *>(out->chi2);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, hand_pending(hand, nch),
+                           MAX_RANGES);
         if (mlp_here) {
             const int64_t g = (nt + FAST_WARM_BLOCK - 1) / FAST_WARM_BLOCK;
             hipLaunchKernelGGL((k_fast_warm<TS, N, PED, true>), dim3((unsigned)g),
@@ -1532,11 +1641,23 @@ static mf_status launch_fast(mf_context *ctx, const SegArg &seg, const void *sam
         }
         HIP_TRY(hipGetLastError());
         const int64_t g = (nt + block - 1) / block;
-        const size_t lds = sizeof(double) * 55 * (size_t)block;
-        auto p3 = seg_layout2(ctx, seg) ? k_fast_p3<TS, N, PED, 2> : k_fast_p3<TS, N, PED, -1>;
-        hipLaunchKernelGGL(p3, dim3((unsigned)g), dim3(block), lds, st, ctx->d_types, seg,
+        const bool l2 = seg_layout2(ctx, seg);
+        auto p3 = l2 ? k_fast_p3<TS, N, PED, 2> : k_fast_p3<TS, N, PED, -1>;
+        hipLaunchKernelGGL(p3, dim3((unsigned)g), dim3(block), 0, st, ctx->d_types, seg,
                            static_cast<const TS *>(samples), ped, nch, xo, c2, out->flags,
-                           out->iters, static_cast<const double *>(hand), prm);
+                           out->iters, hand, prm);
+        HIP_TRY(hipGetLastError());
+        // the pending channels: one grid row per range, a few wavefronts per
+        // CU in all
+        const int nr = seg.nrange ? seg.nrange : 1;
+        const int64_t g4 = std::max<int64_t>(
+            1, std::min<int64_t>((nt + FAST_P4_BLOCK - 1) / FAST_P4_BLOCK,
+                                 4 * (int64_t)ctx->cus / nr));
+        auto p4 = l2 ? k_fast_p4<TS, N, PED, 2> : k_fast_p4<TS, N, PED, -1>;
+        hipLaunchKernelGGL(p4, dim3((unsigned)g4, (unsigned)nr), dim3(FAST_P4_BLOCK),
+                           sizeof(double) * 55 * FAST_P4_BLOCK, st, ctx->d_types, seg,
+                           static_cast<const TS *>(samples), ped, nch, xo, c2, out->flags,
+                           out->iters, hand, prm);
         HIP_TRY(hipGetLastError());
         return MF_OK;
     }
@@ -1664,15 +1785,13 @@ struct CallScratch {
     int32_t *idx = nullptr;     // nch entries: the types' index lists
     float *pred = nullptr;      // 11 nch floats: warm-start prediction, x [10][nch] then pm [nch]
     unsigned *queue = nullptr;  // 2 heads: the persistent kernels' work queues
-    double *hand = nullptr;     // 11 nch doubles: live-covariance hand-over (hand_over)
+    double *hand = nullptr;     // hand_bytes(nch): live-covariance hand-over (hand_over), pending list
 };
 // What a call needs of it.
 struct ScratchNeed {
     bool part = false, pred = false, queue = false, hand = false;
     int64_t nch = 0;
 };
-// hand-over of the live-covariance fast path: FP64 amplitudes and chi2
-constexpr size_t HAND_BYTES_PER_CH = sizeof(double) * 11;
 // Offsets of the parts in one contiguous region (256-byte aligned): queue
 // heads, partition counts, index lists, prediction.
 static size_t align256(size_t v) { return (v + 255) / 256 * 256; }
@@ -1688,7 +1807,7 @@ static size_t scratch_layout(const ScratchNeed &n, size_t off[5])
     off[3] = k;
     if (n.pred) k += align256(sizeof(float) * 11 * (size_t)n.nch);
     off[4] = k;
-    if (n.hand) k += align256(HAND_BYTES_PER_CH * (size_t)n.nch);
+    if (n.hand) k += align256(hand_bytes(n.nch));
     return k;
 }
 
@@ -1726,7 +1845,7 @@ static mf_status ctx_scratch(mf_context *ctx, const ScratchNeed &n, CallScratch 
         if (ctx->hand) HIP_TRY(hipFree(ctx->hand));
         ctx->hand = nullptr;
         ctx->hand_cap = 0;
-        HIP_TRY(hipMalloc(&ctx->hand, HAND_BYTES_PER_CH * (size_t)n.nch));
+        HIP_TRY(hipMalloc(&ctx->hand, hand_bytes(n.nch)));
         ctx->hand_cap = n.nch;
     }
     sc.part = ctx->d_part;

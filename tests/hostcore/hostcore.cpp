@@ -20,6 +20,61 @@ static bool use_tables = true;
 static int g_layout = -1;
 extern "C" void hc_set_layout(int d) { g_layout = d == 2 ? 2 : -1; }
 
+// How the later iterations of a live pulse covariance run: 0 in one body
+// (fast_p2_rest, as k_fast_p2); 1 split as k_fast_p3 + k_fast_p4 (the lazy
+// iterations, then the whole body from the first iteration that needs the
+// normal equations); 2 the whole body from iteration 1 (fast_p2_resume alone).
+static int g_split = 0;
+static long long g_pending = 0;
+extern "C" void hc_set_split(int mode) { g_split = mode; }
+extern "C" long long hc_pending_count(void)
+{
+    const long long n = g_pending;
+    g_pending = 0;
+    return n;
+}
+
+template <int N, int D>
+static void fit_fast_split(const DevType &dt, const float (&s_in)[10], const FitParams &prm,
+                           FitOut<float> &out)
+{
+    int flags = 0;
+    FnnlsStats st;
+    float s[10], x[10];
+    uint32_t pm = 0;
+    for (int i = 0; i < 10; ++i) {
+        s[i] = s_in[i];
+        x[i] = 0.0f;
+    }
+    if (all_finite<float, N>(s) && rho_ok(prm.rho) && prm.max_minimize > 0)
+        fast_phase1<N, N>(dt, s, prm, x, pm, st, flags);
+    ActiveState<double> S;
+    double chi2 = 0.0;
+    int nmin = 0;
+    KeepSamples keep;
+    flags &= F_FP64_PIVOT;
+    if (fast_p2_first<N, N, true>(dt, s, prm, x, false, st, flags, S, chi2, keep)) {
+        bool pend = true;
+        int it = 1;
+        if (g_split == 1) {
+            nmin = fast_p2_rest_lazy<N, N, D>(dt, s, prm, S.x, chi2, st, flags, pend);
+            it = nmin;
+        } else if (prm.use_pi && fabs(chi2) < prm.chi2_tol) {
+            nmin = 1;
+            pend = false;
+        } else if (!prm.use_pi) {
+            nmin = fixed_cov_nmin(chi2, prm, flags);
+            pend = false;
+        }
+        if (pend) {
+            ++g_pending;
+            RegPark<double> park;
+            nmin = fast_p2_resume<N, N, D>(dt, s, prm, S.x, chi2, it, st, flags, park);
+        }
+    }
+    fast_p2_out(S.x, chi2, flags, nmin, st, out);
+}
+
 template <int N, bool LIVE>
 static void run_fast(const DevType &dt, const double *samples, int64_t nch, int64_t c,
                      const FitParams &prm, double *x, double *chi2, int32_t *flags,
@@ -32,7 +87,11 @@ static void run_fast(const DevType &dt, const double *samples, int64_t nch, int6
     }
     RegPark<double> park;   // the live-covariance iterations park L^{-1} in FP64, as the kernel's LDS
     FitOut<float> out;
-    if (use_tables && g_layout == 2)
+    if (LIVE && g_split && use_tables && g_layout == 2)
+        fit_fast_split<N, 2>(dt, s, prm, out);
+    else if (LIVE && g_split && use_tables)
+        fit_fast_split<N, -1>(dt, s, prm, out);
+    else if (use_tables && g_layout == 2)
         fit_fast<N, N, LIVE, true, 2>(dt, s, prm, x0, 0u, false, park, out);
     else if (use_tables)
         fit_fast<N, N, LIVE, true>(dt, s, prm, x0, 0u, false, park, out);

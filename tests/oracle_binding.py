@@ -136,6 +136,8 @@ def hostcore():
     L.hc_mask_inverse.restype = C.c_int
     L.hc_set_use_tables.argtypes = [C.c_int]
     L.hc_set_layout.argtypes = [C.c_int]
+    L.hc_set_split.argtypes = [C.c_int]
+    L.hc_pending_count.restype = C.c_longlong
     _cache["hostcore"] = L
     return L
 

@@ -203,14 +203,20 @@ def test_persistent_edge_cases(pkg, gpu):
             assert r.flags[1].item() & _abi.FLAGS["nonfinite"]
 
 
-@pytest.mark.parametrize("cfg,warm,persistent", [("c2", 0, 0), ("c2", 1, 0), ("c3", 1, 0),
-                                                 ("c3", 1, 1), ("c5", 1, 0)])
-def test_live_split_equals_single_kernel(pkg, gpu, cfg, warm, persistent):
+@pytest.mark.parametrize("cfg,warm,persistent,max_iter", [
+    ("c2", 0, 0, 1000), ("c2", 1, 0, 1000), ("c3", 1, 0, 1000), ("c3", 1, 1, 1000),
+    ("c5", 1, 0, 1000), ("c2", 1, 0, 2), ("c3", 0, 0, 1), ("c3", 1, 0, 3), ("c5", 1, 0, 2)])
+def test_live_split_equals_single_kernel(pkg, gpu, cfg, warm, persistent, max_iter):
     """With a live pulse covariance mf_fit_batch runs the first iteration in
     a light kernel and hands the FP64 state over to the later iterations'
-    kernel (k_fast_p3); mf_fit_batch_noalloc keeps both in k_fast_p2.  The
-    same arithmetic: outputs identical bit for bit (warm starts through the
-    in-kernel MLP, or through the prediction kernel when persistent)."""
+    kernels: k_fast_p3 runs the iterations that stop at their start or add a
+    single index, k_fast_p4 finishes the channels that need the normal
+    equations; mf_fit_batch_noalloc keeps all of it in k_fast_p2.  The same
+    arithmetic: outputs identical bit for bit (warm starts through the
+    in-kernel MLP, or through the prediction kernel when persistent).  A
+    small max_iter sends channels to k_fast_p4 (the single-index step needs
+    two solves' allowance): from a list, type-partitioned lists and type
+    ranges."""
     from test_regression_amd import engine, workloads, _abi, warmstart
     w = workloads.config(cfg, "live")
     eng = engine.Engine(list(w.chtypes()))
@@ -225,8 +231,10 @@ def test_live_split_equals_single_kernel(pkg, gpu, cfg, warm, persistent):
         s, t, _ = w.generate(60_000)
         t = torch.from_numpy(t).cuda() if t is not None else None
     x = torch.from_numpy(s.astype(np.float32)).cuda()
-    print(f"[gpu-fast] live split cfg={cfg} warm={warm} persistent={persistent}", flush=True)
-    o = _abi.default_opts(precision=_abi.MF_PREC_FP32, warm_start=warm, persistent=persistent)
+    print(f"[gpu-fast] live split cfg={cfg} warm={warm} persistent={persistent} "
+          f"max_iter={max_iter}", flush=True)
+    o = _abi.default_opts(precision=_abi.MF_PREC_FP32, warm_start=warm, persistent=persistent,
+                          max_iter=max_iter)
     a = eng.fit(x, t, o, **kw)
     b = eng.fit_noalloc(x, t, o, **kw)
     torch.cuda.synchronize()

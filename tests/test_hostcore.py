@@ -254,3 +254,42 @@ def test_layout_specialisation_is_bit_identical(pkg, cfg, pulse_cov, mode):
         H.hc_set_layout(-1)
     for u, v in zip(a, b):
         assert np.array_equal(u, v, equal_nan=True)
+
+
+@pytest.mark.parametrize("cfg", ["c2", "c3", "c5"])
+@pytest.mark.parametrize("max_iter", [1000, 3, 1])
+def test_lazy_split_is_bit_identical(pkg, cfg, max_iter):
+    """The later iterations of a live pulse covariance as k_fast_p3 + k_fast_p4
+    run them (the iterations that stop at their start or add a single index,
+    then the whole iteration body from the first that needs the normal
+    equations; hc_set_split 1), and the whole body from iteration 1
+    (hc_set_split 2), against one body (k_fast_p2): identical bit for bit.
+    A small max_iter sends channels to the whole body (the single-index step
+    needs two solves' allowance); at the default none goes there."""
+    from oracle_binding import hostcore
+    from test_regression_amd import workloads, _abi
+    w = workloads.config(cfg, "live")
+    ct = w.chtypes()
+    s, t, _ = w.generate(3000)
+    s = s.astype(np.float32).astype(np.float64)
+    o = _abi.default_opts(precision=_abi.MF_PREC_FP32, max_iter=max_iter)
+    H = hostcore()
+    H.hc_set_layout(2)
+    try:
+        a = hostcore_fit(ct, s, t, o)
+        H.hc_set_split(1)
+        H.hc_pending_count()
+        b = hostcore_fit(ct, s, t, o)
+        pend = H.hc_pending_count()
+        H.hc_set_split(2)
+        c = hostcore_fit(ct, s, t, o)
+    finally:
+        H.hc_set_split(0)
+        H.hc_set_layout(-1)
+    for u, v, z in zip(a, b, c):
+        assert np.array_equal(u, v, equal_nan=True)
+        assert np.array_equal(u, z, equal_nan=True)
+    if max_iter == 1:
+        assert pend > 0.9 * s.shape[1]
+    if max_iter == 1000:
+        assert pend == 0
