@@ -33,6 +33,8 @@
 
 #include <stdint.h>
 
+#include <type_traits>
+
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define MF_HD __host__ __device__ __forceinline__
@@ -1154,8 +1156,8 @@ MF_HD void pt_times(const TypeMats<T> &tm, const T (&v)[10], T (&y)[10])
 
 // u = L^-1 (P x - s), w = -P^T L^-T u (see above), chi2 = |u|^2; returns
 // whether max_p w_p < eps.
-template <typename T, int NS, int NP, int D>
-MF_HD bool stop_at_start(const TypeMats<T> &tm, const T (&s)[10], const T (&x)[10],
+template <typename T, int NS, int NP, int D, typename TS>
+MF_HD bool stop_at_start(const TypeMats<T> &tm, const TS (&s)[10], const T (&x)[10],
                          const T (&L)[10][10], const T (&ri)[10], T eps, T &chi2, T (&u)[10],
                          T (&w)[10])
 {
@@ -1167,7 +1169,7 @@ MF_HD bool stop_at_start(const TypeMats<T> &tm, const T (&s)[10], const T (&x)[1
 #pragma unroll
         for (int p = 0; p < NP; ++p)
             if (i >= lay_first<D>(p)) t = mfma(tm.P[i][p], x[p], t);
-        u[i] = t - s[i];
+        u[i] = t - T(s[i]);
     }
     forward_subst<T, NS, NP, D>(L, ri, u);
     chi2 = T(0);
@@ -1194,13 +1196,13 @@ MF_HD bool stop_at_start(const TypeMats<T> &tm, const T (&s)[10], const T (&x)[1
 // AtA_qq by the masked factorisation's steps; the other amplitudes kept)
 // and stops when no free w reaches eps (nnls/src/inplace_fnnls.cu:140-233).
 // Formed from the factor L and the u, w of stop_at_start: a_q = L^-1 P_q,
-// Atb_q = a_q . L^-1 s, AtA_qq = a_q . a_q, the new w = w - (P^T L^-T a_q)
-// dx_q, the new chi2 = |u + dx_q a_q|^2.  Returns false when the channel
-// leaves that case (the solve is not positive, a second index is added,
-// max_iter < 2); x is then unchanged and the caller runs the iteration in
-// full.
-template <typename T, int NS, int NP, int D>
-MF_HD bool single_add(const TypeMats<T> &tm, const T (&s)[10], T (&x)[10], const T (&L)[10][10],
+// AtA_qq = a_q . a_q, g = L^-T a_q = C^-1 P_q, Atb_q = a_q . L^-1 s = g . s,
+// column q of AtA = P^T g, the new w = w - (P^T g) dx_q, the new chi2 =
+// |u + dx_q a_q|^2.  Returns false when the channel leaves that case (the
+// solve is not positive, a second index is added, max_iter < 2); x is then
+// unchanged and the caller runs the iteration in full.
+template <typename T, int NS, int NP, int D, typename TS>
+MF_HD bool single_add(const TypeMats<T> &tm, const TS (&s)[10], T (&x)[10], const T (&L)[10][10],
                       const T (&ri)[10], const T (&u)[10], const T (&w)[10], T eps,
                       int max_iter, T &chi2)
 {
@@ -1213,7 +1215,7 @@ MF_HD bool single_add(const TypeMats<T> &tm, const T (&s)[10], T (&x)[10], const
             wq = w[p];
             q = p;
         }
-    T a[10], b[10];
+    T a[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) {
         T c = T(0);
@@ -1221,25 +1223,23 @@ MF_HD bool single_add(const TypeMats<T> &tm, const T (&s)[10], T (&x)[10], const
         for (int p = 0; p < NP; ++p)
             if (i < NS && i >= lay_first<D>(p)) c = (p == q) ? tm.P[i][p] : c;
         a[i] = c;
-        b[i] = i < NS ? s[i] : T(0);
     }
     forward_subst<T, NS, NP, D>(L, ri, a);
-    forward_subst<T, NS, NP, D>(L, ri, b);
-    T att = T(0), atb = T(0);
+    T att = T(0);
 #pragma unroll
-    for (int i = 0; i < NS; ++i) {
-        att = mfma(a[i], a[i], att);
-        atb = mfma(a[i], b[i], atb);
-    }
+    for (int i = 0; i < NS; ++i) att = mfma(a[i], a[i], att);
     if (!(att > T(0))) return false;
-    const T r = mrsqrt(att);
-    const T sq = (atb * r) * r;
-    if (!(sq > T(0))) return false;
-    const T dx = sq - x[q];
     T g[10], c[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) g[i] = a[i];
     backward_subst<T, NS>(L, ri, g);
+    T atb = T(0);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) atb = mfma(g[i], T(s[i]), atb);
+    const T r = mrsqrt(att);
+    const T sq = (atb * r) * r;
+    if (!(sq > T(0))) return false;
+    const T dx = sq - x[q];
     pt_times<T, NS, NP, D>(tm, g, c);   // column q of AtA
     bool stop = true;
 #pragma unroll
@@ -1265,12 +1265,15 @@ MF_HD bool single_add(const TypeMats<T> &tm, const T (&s)[10], T (&x)[10], const
 // the normal equations sets *pend and keeps x, for a kernel that holds them
 // (so that this one needs neither their registers nor L^{-1}'s park).
 template <typename T, int NS, int NP, int XS = XS_NONE, int D = -1, bool LAZY = false,
-          bool ONLY = false, typename Park>
-MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams &prm,
+          bool ONLY = false, typename Park, typename TS>
+MF_HD T minimize_step(const TypeMats<T> &tm0, const TS (&s)[10], const FitParams &prm,
                       Park &park, T (&x)[10], FnnlsStats &st, int &flags, int zc = -1,
                       LegacyState *ls = nullptr, bool *kept = nullptr, bool *pend = nullptr)
 {
     const TypeMats<T> &tm = *opaque(&tm0);
+    // TS: the samples' type; other than T only for ONLY (the lazy steps
+    // convert each sample where they use it)
+    static_assert(std::is_same<TS, T>::value || ONLY, "samples in T");
     if (LAZY && XS == XS_NONE && zc < 0 && prm.solver == 0) {
         T L[10][10], ri[10], chi2, u[10], w[10];
         build_cov<T, NS, NP, D>(tm, prm.use_pi != 0, x, L, T(prm.rho));
@@ -1317,13 +1320,13 @@ MF_HD T minimize_step(const TypeMats<T> &tm0, const T (&s)[10], const FitParams 
 // equations, *pend is set and the index of that iteration returned, with x
 // and chi2 as they were at its start.
 template <typename T, int NS, int NP, int XS = XS_NONE, int D = -1, bool KEEP = true,
-          bool LAZY = false, bool ONLY = false, typename Park>
-MF_HD int minimize_from(const TypeMats<T> &tm, const T (&s)[10], const FitParams &prm,
+          bool LAZY = false, bool ONLY = false, typename Park, typename TS>
+MF_HD int minimize_from(const TypeMats<T> &tm, const TS (&s)[10], const FitParams &prm,
                         Park &park, T (&x)[10], T &chi2, int it0, FnnlsStats &st, int &flags,
                         int zc = -1, LegacyState *ls = nullptr, bool *pend = nullptr)
 {
     int it = it0;
-    if (KEEP && !prm.use_pi && it < prm.max_minimize) {
+    if constexpr (KEEP) if (!prm.use_pi && it < prm.max_minimize) {
         // The covariance does not depend on x: every iteration forms the
         // same normal equations and L^{-1}, bit for bit, so they are formed
         // once and kept (the solves and chi2 are the reference's).
@@ -1847,12 +1850,9 @@ MF_HD int fast_p2_rest_lazy(const DevType &ty, const float (&s32)[10], const Fit
     pend = false;
     if (!prm.use_pi) return fixed_cov_nmin(chi2, prm, flags);
     if (fabs(chi2) < prm.chi2_tol) return 1;
-    double s64[10];
-#pragma unroll
-    for (int i = 0; i < 10; ++i) s64[i] = (double)s32[i];
     NullPark park;
     return minimize_from<double, NS, NP, XS_NONE, D, false, true, true>(
-        ty.d, s64, prm, park, x, chi2, 1, st, flags, -1, nullptr, &pend);
+        ty.d, s32, prm, park, x, chi2, 1, st, flags, -1, nullptr, &pend);
 }
 template <int NS, int NP, int D, typename Park>
 MF_HD int fast_p2_resume(const DevType &ty, const float (&s32)[10], const FitParams &prm,

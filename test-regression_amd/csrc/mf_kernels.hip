@@ -265,6 +265,27 @@ template <typename TS, int N> struct SampleReload {
         for (int i = 0; i < 10; ++i) v[i] = i < N ? (float)q[(int64_t)i * nch + c] : 0.0f;
     }
 };
+// Pending lists (after the hand-over's FP64 values): one count per range of
+// the launch (MAX_RANGES), cleared by the first-iteration launch, then the
+// items (the index seg_item reads).  Range r files its
+// items from slot wave0[r] * 64 on (a single list from slot 0), so that the
+// lists need no shared count and k_fast_p4 takes one range per grid row.
+constexpr int F_PENDING = 1 << 30;   // between k_fast_p3 and k_fast_p4 only
+__host__ __device__ inline size_t hand_fp64_bytes(int64_t nch)
+{
+    return ((size_t)nch * 11 * sizeof(double) + 255) / 256 * 256;
+}
+// slots: at most nch items plus each range's last partial wavefront
+__host__ __device__ inline size_t hand_bytes(int64_t nch)
+{
+    return hand_fp64_bytes(nch) + 256 + sizeof(int32_t) * ((size_t)nch + 64 * MAX_RANGES);
+}
+__host__ __device__ inline unsigned *hand_pending(double *hand, int64_t nch)
+{
+    return reinterpret_cast<unsigned *>(reinterpret_cast<unsigned char *>(hand) +
+                                        hand_fp64_bytes(nch));
+}
+
 // Hand-over of a channel after phase 2's first iteration, for types whose
 // pulse covariance is live: that iteration runs in a light kernel (four
 // waves per SIMD), the later iterations, which need about twice the
@@ -303,6 +324,9 @@ void k_fast_fused(const DevType *__restrict__ types, SegArg seg,
                   int32_t *__restrict__ ito, WarmIn warm, FitParams prm0,
                   double *__restrict__ hand = nullptr)
 {
+    // HO: the first block clears the pending counts of k_fast_p3 (this
+    // launch precedes it on the stream)
+    if (HO && blockIdx.x == 0 && threadIdx.x < MAX_RANGES) hand_pending(hand, nch)[threadIdx.x] = 0u;
     int type_id;
     int64_t c;
     if (!seg_item(seg, type_id, c)) return;
@@ -356,6 +380,9 @@ void k_fast_warm(const DevType *__restrict__ types, SegArg seg,
     __shared__ __bf16 xs[FAST_WARM_BLOCK / 64][64][16];
     __shared__ float pz[FAST_WARM_BLOCK / 64][10][64];
     __shared__ uint32_t pmw[FAST_WARM_BLOCK / 64][64];
+    // HO: the first block clears the pending counts of k_fast_p3 (this
+    // launch precedes it on the stream)
+    if (HO && blockIdx.x == 0 && threadIdx.x < MAX_RANGES) hand_pending(hand, nch)[threadIdx.x] = 0u;
     // a block past the end of its type's list (the grid of a type's launch is
     // sized for the whole batch) stages nothing
     if (seg.nrange == 0 && (int64_t)blockIdx.x * FAST_WARM_BLOCK >= seg_count(seg.list)) return;
@@ -444,29 +471,11 @@ void k_fast_p2(const DevType *__restrict__ types, SegArg seg,
 // in the pending list; k_fast_p4 finishes those channels with the whole body
 // (fast_p2_resume).  Results are those of fast_p2_rest bit for bit.
 //
-// Pending lists (after the hand-over's FP64 values): one count per range of
-// the launch (MAX_RANGES), cleared by a kernel before the first-iteration
-// launch, then the items (the index seg_item reads).  Range r files its
-// items from slot wave0[r] * 64 on (a single list from slot 0), so that the
-// lists need no shared count and k_fast_p4 takes one range per grid row.
-constexpr int F_PENDING = 1 << 30;   // between k_fast_p3 and k_fast_p4 only
-__host__ __device__ inline size_t hand_fp64_bytes(int64_t nch)
-{
-    return ((size_t)nch * 11 * sizeof(double) + 255) / 256 * 256;
-}
-// slots: at most nch items plus each range's last partial wavefront
-__host__ __device__ inline size_t hand_bytes(int64_t nch)
-{
-    return hand_fp64_bytes(nch) + 256 + sizeof(int32_t) * ((size_t)nch + 64 * MAX_RANGES);
-}
-__host__ __device__ inline unsigned *hand_pending(double *hand, int64_t nch)
-{
-    return reinterpret_cast<unsigned *>(reinterpret_cast<unsigned char *>(hand) +
-                                        hand_fp64_bytes(nch));
-}
-
+// Two waves per SIMD: the compiler then spills a few values (84 bytes per
+// lane); measured faster than one wave at 276 registers (0.316 against
+// 0.374 ms for the warm step of 1M live channels).
 template <typename TS, int N, bool PED, int D>
-__global__ __launch_bounds__(256)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_fast_p3(const DevType *__restrict__ types, SegArg seg,
                const TS *__restrict__ samples, const double *__restrict__ ped, int64_t nch,
                float *__restrict__ xo, float *__restrict__ chi2o, int32_t *__restrict__ flo,
@@ -494,6 +503,9 @@ void k_fast_p3(const DevType *__restrict__ types, SegArg seg,
     int flags = f1 & ~(F_FNNLS_CAP | F_LLT_FAIL);
     bool pend;
     const int nmin = fast_p2_rest_lazy<N, N, D>(types[type_id], s, prm, x, chi2, st, flags, pend);
+    // the stores' addresses are formed here again from laundered bases, not
+    // kept in registers through the iterations
+    asm volatile("" : "+s"(hand), "+s"(xo), "+s"(chi2o), "+s"(flo), "+s"(ito));
     if (pend) {
 #pragma unroll
         for (int p = 0; p < 10; ++p) hand[(int64_t)p * nch + c] = x[p];
@@ -1625,8 +1637,6 @@ static mf_status launch_fast(mf_context *ctx, const SegArg &seg, const void *sam
         // first iteration in a light kernel, the later ones in k_fast_p3 and,
         // for the channels that need the normal equations, k_fast_p4
         float *xo = static_cast<float *>(out->x), *c2 = static_cast<float *>(out->chi2);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, hand_pending(hand, nch),
-                           MAX_RANGES);
         if (mlp_here) {
             const int64_t g = (nt + FAST_WARM_BLOCK - 1) / FAST_WARM_BLOCK;
             hipLaunchKernelGGL((k_fast_warm<TS, N, PED, true>), dim3((unsigned)g),
@@ -1652,7 +1662,7 @@ static mf_status launch_fast(mf_context *ctx, const SegArg &seg, const void *sam
         const int nr = seg.nrange ? seg.nrange : 1;
         const int64_t g4 = std::max<int64_t>(
             1, std::min<int64_t>((nt + FAST_P4_BLOCK - 1) / FAST_P4_BLOCK,
-                                 4 * (int64_t)ctx->cus / nr));
+                                 2 * (int64_t)ctx->cus / nr));
         auto p4 = l2 ? k_fast_p4<TS, N, PED, 2> : k_fast_p4<TS, N, PED, -1>;
         hipLaunchKernelGGL(p4, dim3((unsigned)g4, (unsigned)nr), dim3(FAST_P4_BLOCK),
                            sizeof(double) * 55 * FAST_P4_BLOCK, st, ctx->d_types, seg,
