@@ -326,6 +326,30 @@ def test_dofit_single_pulse_closed_form(pkg):
         assert abs(r["x"][0][0] - expect) <= 1e-10 * max(1, abs(expect))
 
 
+def test_dofit_pederr_scaling(pkg):
+    """DoFit's pederr (PulseChiSqSNNLS.cpp:176, the covariance pederr^2 R):
+    given per channel it equals a type of that pederr bit for bit; and with
+    pederr 2 every whitened quantity is the pederr-1 one scaled by a power of
+    two, so at zero pulse covariance the amplitudes are the same bit for bit
+    and chi2 is a quarter of it, except where a decision (max w < eps,
+    |dchi2| < 1e-3) falls between the two scales."""
+    from boundary_cases import npulse_types
+    from test_regression_amd import workloads
+    s, _, _ = workloads.config("c2").generate(600)
+    ct1 = npulse_types("zero", npulses=(10,))
+    ct2 = npulse_types("zero", npulses=(10,), pederr=2.0)
+    r1 = oracle_fit(ct1, s, None, oracle_opts())
+    r2 = oracle_fit(ct1, s, None, oracle_opts(), pederr=np.full(s.shape[1], 2.0))
+    r3 = oracle_fit(ct2, s, None, oracle_opts())
+    assert np.array_equal(r2["x"], r3["x"]) and np.array_equal(r2["chi2"], r3["chi2"])
+    same = (r1["x"] == r2["x"]).all(axis=1)
+    assert same.mean() > 0.99
+    assert np.array_equal(r2["chi2"][same] * 4.0, r1["chi2"][same])
+    # a planted mistake, pederr entering once instead of squared, breaks it
+    r4 = oracle_fit(ct1, s, None, oracle_opts(), pederr=np.full(s.shape[1], np.sqrt(2.0)))
+    assert not np.allclose(r4["chi2"] * 4.0, r1["chi2"], rtol=1e-6)
+
+
 # ---------------------------------------------------------------- planted mistakes
 def _py_inplace_fnnls(A, b, eps=1e-11, max_iter=1000, mistake=None):
     """Python copy of the oracle's FNNLS (pure loops, small sizes) for planting
