@@ -11,6 +11,7 @@
  *     expAmplitude, expTime), read by multifit_cpu/interface/Pulse.h:69-89;
  *   - the `RecoAndSim` tree the CPU driver writes (multifit.cpp:108-129:
  *     chi2, samplesReco, ipulseintime, ...).
+ * mf_io_write_tree writes a tree of such branches to a new file.
  * This reader covers what those files hold: the file's key list, zlib
  * compressed records, tree branches of one basic value per entry (/F, /D,
  * /I) or of std::vector<double> / std::vector<int>, and TGraph points.  It
@@ -66,6 +67,29 @@ mf_status mf_io_read_branch(mf_rootfile *file, const char *tree, const char *bra
  * *n: the number of points. */
 mf_status mf_io_read_graph(mf_rootfile *file, const char *name, double *x, double *y,
                            int64_t cap, int64_t *n);
+
+/* One branch of a tree to write: its name, kind (MF_IO_*), the values of
+ * all entries concatenated (as doubles; /F and /I branches store them
+ * rounded to float and truncated to int), and for the vector kinds the
+ * number of values of each entry. */
+typedef struct mf_io_column {
+    const char    *name;
+    int32_t        kind;
+    int32_t        pad;
+    const double  *values;
+    const int64_t *counts;
+} mf_io_column;
+
+/* Writes a new ROOT file at `path` holding one tree, `tree`, of nentries
+ * entries and ncols branches: what the reference's CPU driver writes with
+ * TTree::Branch / Fill / Write (multifit_cpu/src/multifit.cpp:107-129,
+ * 266-279; its RecoAndSim tree).  The file is uncompressed, in ROOT 6's
+ * small-file layout and the class versions ROOT 6 holds for TTree and its
+ * branches and leaves (no streamer info needed); baskets hold up to 32000
+ * bytes of entries.  Files beyond 2 GB are refused (MF_ERR_UNSUPPORTED).
+ * Branches of std::vector<std::vector<double>> are not written. */
+mf_status mf_io_write_tree(const char *path, const char *tree, const char *title,
+                           const mf_io_column *cols, int32_t ncols, int64_t nentries);
 
 #ifdef __cplusplus
 }

@@ -89,6 +89,12 @@ MF_PERSISTENT_AUTO = 2
 MF_IO_F32, MF_IO_F64, MF_IO_I32, MF_IO_VEC_F64, MF_IO_VEC_I32 = 0, 1, 2, 3, 4
 
 
+class IoColumn(C.Structure):
+    """mf_io_column (include/mf_io.h)."""
+    _fields_ = [("name", C.c_char_p), ("kind", C.c_int32), ("pad", C.c_int32),
+                ("values", C.c_void_p), ("counts", C.c_void_p)]
+
+
 class MFError(RuntimeError):
     pass
 
@@ -148,6 +154,8 @@ SIGNATURES = [
                                     C.POINTER(C.c_int64)]),
     ("mf_io_read_graph", C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p, C.c_int64,
                                    C.POINTER(C.c_int64)]),
+    ("mf_io_write_tree", C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_void_p, C.c_int32,
+                                   C.c_int64]),
 ]
 
 

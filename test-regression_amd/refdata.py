@@ -14,8 +14,9 @@ and its CPU driver's fit set-up on top of the engine.
 - fit_samples / write_reco: the driver's per-entry outputs
   (multifit.cpp:196-227): chi2, the in-time amplitude and samplesReco (the
   amplitudes by BX + 5), fitted on the GPU.  The reference stores them as the
-  `RecoAndSim` ROOT tree (:108-129); write_reco stores the same branch names
-  in a numpy .npz file.
+  `RecoAndSim` ROOT tree (:108-129, 266-279); write_reco writes that tree to a
+  ROOT file (mf_io_write_tree) with the same branches; write_tree writes any
+  tree of /F, /D, /I and vector branches.
 """
 from __future__ import annotations
 
@@ -216,14 +217,60 @@ def fit_samples(samples: np.ndarray, chtype: _abi.ChType, opts=None, device="cud
             "flags": r.flags.cpu().numpy()}
 
 
-def write_reco(path: str, samples: np.ndarray, amplitude_truth: np.ndarray, fit: dict,
-               ps: PulseShape) -> None:
-    """The RecoAndSim content (multifit.cpp:108-129, 247-262) as .npz: the
-    input branches, samplesReco, chi2 (the reference's branch keeps its
-    initial -99: the fit's chi2 never reaches it), ampl, activeBXs and
-    pulseShapeTemplate (fShape(IDSTART + 25 i + 75 - 250), i < 17)."""
+def write_tree(path: str, tree: str, branches: dict, title: str = "") -> None:
+    """Writes a ROOT file holding one tree (mf_io_write_tree).  branches:
+    name -> (kind, values) for /F, /D, /I (one value per entry) or (kind,
+    list of per-entry sequences) for the vector kinds, in branch order."""
+    cols, keep, n = [], [], None
+    for name, (kind, vals) in branches.items():
+        if kind in (_abi.MF_IO_VEC_F64, _abi.MF_IO_VEC_I32):
+            counts = np.array([len(v) for v in vals], dtype=np.int64)
+            flat = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.float64).ravel()
+                                                        for v in vals]) if len(vals) else
+                                        np.zeros(0), dtype=np.float64)
+            m = counts.size
+        else:
+            counts = None
+            flat = np.ascontiguousarray(np.asarray(vals, dtype=np.float64).ravel())
+            m = flat.size
+        if n is None:
+            n = m
+        if m != n:
+            raise _abi.MFError(f"write_tree: branch {name} has {m} entries, not {n}")
+        keep += [flat, counts, name.encode()]
+        cols.append(_abi.IoColumn(keep[-1], kind, 0, flat.ctypes.data_as(C.c_void_p),
+                                  counts.ctypes.data_as(C.c_void_p) if counts is not None
+                                  else None))
+    arr = (_abi.IoColumn * len(cols))(*cols)
+    _abi.check(_abi.lib().mf_io_write_tree(str(path).encode(), tree.encode(), title.encode(),
+                                           C.cast(arr, C.c_void_p), len(cols), n or 0),
+               "mf_io_write_tree")
+
+
+def write_reco(path: str, fit: dict, ps: PulseShape, source: dict | None = None) -> None:
+    """The CPU driver's output file (multifit.cpp:108-129, 247-262, 266-279):
+    a ROOT file with the tree RecoAndSim, one entry per fitted input entry.
+    Its branches are the input tree's (CloneTree(0)): source, name -> (kind,
+    values) as write_tree takes them, e.g. the Samples branches this reader
+    covers; then the driver's own, filled as the driver fills them: chi2
+    (/F; the driver never assigns it, so it keeps its initial -99),
+    samplesReco (the amplitudes by BX + 5), ipulseintime (/I, 0),
+    complete_chi2 and complete_pedestal (empty vectors), best_pedestal and
+    best_chi2 (/F, 0), pulseShapeTemplate (fShape(IDSTART + 25 i + 75 - 250),
+    i < 17) and activeBXs (-5..4).  The driver's complete_samplesReco
+    (std::vector<std::vector<double>>, empty) is not written."""
+    reco = np.asarray(fit["samplesReco"], dtype=np.float64)
+    n = reco.shape[0]
     template = np.array([fshape(ps, float(IDSTART + NFREQ * i + 3 * 25.0 - 500 / 2.0))
                          for i in range(NSAMPLES + 7 * (25 // NFREQ))])
-    np.savez(path, samples=samples.T, amplitudeTruth=amplitude_truth,
-             samplesReco=fit["samplesReco"], chi2=fit["chi2"], ampl=fit["ampl"],
-             activeBXs=np.array(ACTIVE_BX), pulseShapeTemplate=template)
+    br = dict(source or {})
+    br["chi2"] = (_abi.MF_IO_F32, np.full(n, -99.0))
+    br["samplesReco"] = (_abi.MF_IO_VEC_F64, list(reco))
+    br["ipulseintime"] = (_abi.MF_IO_I32, np.zeros(n))
+    br["complete_chi2"] = (_abi.MF_IO_VEC_F64, [np.zeros(0)] * n)
+    br["complete_pedestal"] = (_abi.MF_IO_VEC_F64, [np.zeros(0)] * n)
+    br["best_pedestal"] = (_abi.MF_IO_F32, np.zeros(n))
+    br["best_chi2"] = (_abi.MF_IO_F32, np.zeros(n))
+    br["pulseShapeTemplate"] = (_abi.MF_IO_VEC_F64, [template] * n)
+    br["activeBXs"] = (_abi.MF_IO_VEC_I32, [np.array(ACTIVE_BX, dtype=np.float64)] * n)
+    write_tree(path, "RecoAndSim", br)

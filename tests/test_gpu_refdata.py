@@ -51,16 +51,22 @@ def test_reference_events_match_oracle(pkg, gpu, mode):
 
 def test_reference_events_recover_truth(pkg, gpu, tmp_path):
     """With the generator's timing (IDSTART 150) the fitted in-time amplitude
-    matches amplitudeTruth within the noise; the .npz output keeps the
-    RecoAndSim branch names."""
+    matches amplitudeTruth within the noise; the RecoAndSim file the driver
+    writes is read back with the reader: the cloned input branches and the
+    fit's samplesReco, bit for bit."""
     from test_regression_amd import _abi, refdata
     s, a, ps, ct, _ = _setup(idstart=150.0)
     fit = refdata.fit_samples(s, ct, _abi.default_opts())
     dA = fit["ampl"] - a
     print(f"dA mean {dA.mean():.4f} std {dA.std():.4f}")
     assert abs(dA.mean()) < 0.02 and dA.std() < 0.05
-    out = tmp_path / "reco.npz"
-    refdata.write_reco(str(out), s, a, fit, ps)
-    z = np.load(out)
-    assert np.array_equal(z["samplesReco"], fit["samplesReco"])
-    assert np.array_equal(z["samples"], s.T) and z["pulseShapeTemplate"].shape == (17,)
+    out = tmp_path / "reco.root"
+    src = {"samples": (_abi.MF_IO_VEC_F64, list(s.T)),
+           "amplitudeTruth": (_abi.MF_IO_F64, a)}
+    refdata.write_reco(str(out), fit, ps, source=src)
+    with refdata.RootFile(str(out)) as f:
+        assert f.tree_entries("RecoAndSim") == s.shape[1]
+        assert np.array_equal(f.read_vectors("RecoAndSim", "samplesReco"), fit["samplesReco"])
+        assert np.array_equal(f.read_vectors("RecoAndSim", "samples"), s.T)
+        assert np.array_equal(f.read_branch("RecoAndSim", "amplitudeTruth", _abi.MF_IO_F64)[0], a)
+        assert f.read_vectors("RecoAndSim", "pulseShapeTemplate").shape == (s.shape[1], 17)
