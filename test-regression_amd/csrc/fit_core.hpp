@@ -48,6 +48,15 @@ namespace mf {
 // ---------------------------------------------------------------------------
 // scalar helpers
 // ---------------------------------------------------------------------------
+// Inside a function body: no contraction of a * b + c into an FMA there, so
+// that every kernel that inlines the function rounds it the same way (device
+// code contracts by default; the host builds of the tests do not).
+#if defined(__clang__)
+#define MF_NO_CONTRACT _Pragma("clang fp contract(off)")
+#else
+#define MF_NO_CONTRACT
+#endif
+
 MF_HD double mfma(double a, double b, double c) { return fma(a, b, c); }
 MF_HD float mfma(float a, float b, float c) { return fmaf(a, b, c); }
 MF_HD double mabs(double a) { return fabs(a); }
@@ -1158,9 +1167,10 @@ MF_HD void pt_times(const TypeMats<T> &tm, const T (&v)[10], T (&y)[10])
 // whether max_p w_p < eps.
 template <typename T, int NS, int NP, int D, typename TS>
 MF_HD bool stop_at_start(const TypeMats<T> &tm, const TS (&s)[10], const T (&x)[10],
-                         const T (&L)[10][10], const T (&ri)[10], T eps, T &chi2, T (&u)[10],
-                         T (&w)[10])
+                         const T (&L)[10][10], const T (&ri)[10], T eps, T &chi2, T (&w)[10])
 {
+    MF_NO_CONTRACT
+    T u[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) u[i] = T(0);
 #pragma unroll
@@ -1195,17 +1205,19 @@ MF_HD bool stop_at_start(const TypeMats<T> &tm, const TS (&s)[10], const T (&x)[
 // of equal ones), solves the passive system of {q} alone (x_q = Atb_q /
 // AtA_qq by the masked factorisation's steps; the other amplitudes kept)
 // and stops when no free w reaches eps (nnls/src/inplace_fnnls.cu:140-233).
-// Formed from the factor L and the u, w of stop_at_start: a_q = L^-1 P_q,
+// Formed from the factor L and the w, chi2 of stop_at_start: a_q = L^-1 P_q,
 // AtA_qq = a_q . a_q, g = L^-T a_q = C^-1 P_q, Atb_q = a_q . L^-1 s = g . s,
 // column q of AtA = P^T g, the new w = w - (P^T g) dx_q, the new chi2 =
-// |u + dx_q a_q|^2.  Returns false when the channel leaves that case (the
-// solve is not positive, a second index is added, max_iter < 2); x is then
-// unchanged and the caller runs the iteration in full.
+// |u + dx_q a_q|^2 = chi2 + dx_q (dx_q AtA_qq - 2 w_q) (a_q . u = -w_q), so
+// that neither u nor a_q is held past the solve.  Returns false when the
+// channel leaves that case (the solve is not positive, a second index is
+// added, max_iter < 2); x is then unchanged and the caller runs the
+// iteration in full.
 template <typename T, int NS, int NP, int D, typename TS>
 MF_HD bool single_add(const TypeMats<T> &tm, const TS (&s)[10], T (&x)[10], const T (&L)[10][10],
-                      const T (&ri)[10], const T (&u)[10], const T (&w)[10], T eps,
-                      int max_iter, T &chi2)
+                      const T (&ri)[10], const T (&w)[10], T eps, int max_iter, T &chi2)
 {
+    MF_NO_CONTRACT
     if (max_iter < 2) return false;
     int q = 0;
     T wq = w[0];
@@ -1246,12 +1258,7 @@ MF_HD bool single_add(const TypeMats<T> &tm, const TS (&s)[10], T (&x)[10], cons
     for (int p = 0; p < NP; ++p) stop = stop && (p == q || mfma(-c[p], dx, w[p]) < eps);
     if (!stop) return false;
     x[q] = sq;
-    chi2 = T(0);
-#pragma unroll
-    for (int i = 0; i < NS; ++i) {
-        const T e = mfma(dx, a[i], u[i]);
-        chi2 = mfma(e, e, chi2);
-    }
+    chi2 = mfma(dx, mfma(dx, att, T(-2) * wq), chi2);
     return true;
 }
 
@@ -1275,14 +1282,14 @@ MF_HD T minimize_step(const TypeMats<T> &tm0, const TS (&s)[10], const FitParams
     // convert each sample where they use it)
     static_assert(std::is_same<TS, T>::value || ONLY, "samples in T");
     if (LAZY && XS == XS_NONE && zc < 0 && prm.solver == 0) {
-        T L[10][10], ri[10], chi2, u[10], w[10];
+        T L[10][10], ri[10], chi2, w[10];
         build_cov<T, NS, NP, D>(tm, prm.use_pi != 0, x, L, T(prm.rho));
         if (!cholesky_inplace<T, NS>(L, ri)) flags |= F_LLT_FAIL;
-        if (stop_at_start<T, NS, NP, D>(tm, s, x, L, ri, T(prm.eps), chi2, u, w)) {
+        if (stop_at_start<T, NS, NP, D>(tm, s, x, L, ri, T(prm.eps), chi2, w)) {
             if (kept) *kept = true;
             return chi2;
         }
-        if (single_add<T, NS, NP, D>(tm, s, x, L, ri, u, w, T(prm.eps), prm.max_iter, chi2)) {
+        if (single_add<T, NS, NP, D>(tm, s, x, L, ri, w, T(prm.eps), prm.max_iter, chi2)) {
             st.outer++;
             st.inner++;
             return chi2;
@@ -1343,26 +1350,47 @@ MF_HD int minimize_from(const TypeMats<T> &tm, const TS (&s)[10], const FitParam
         flags |= F_MINIMIZE_CAP;
         return it;
     }
-    for (; it < prm.max_minimize; ++it) {                       // :154
+    if constexpr (!LAZY) {
+        for (; it < prm.max_minimize; ++it) {                   // :154
+            const T now =
+                minimize_step<T, NS, NP, XS, D>(tm, s, prm, park, x, st, flags, zc, ls);
+            const T d = now - chi2;
+            chi2 = now;
+            if (mabs(d) < T(prm.chi2_tol)) return it + 1;       // :162
+        }
+        flags |= F_MINIMIZE_CAP;
+        return it;
+    }
+    // LAZY: one exit from the loop (the iteration count of the end).  With
+    // a return inside the loop, k_fast_p2 (the whole later-iteration body in
+    // one kernel) kept a lane that had returned iterating with the rest of
+    // its wavefront: only its count and the cap flag showed it (the extra
+    // iterations kept x), and only in wavefronts whose other lanes ran on.
+    int done = -1;
+    for (; it < prm.max_minimize && done < 0; ++it) {           // :154
         bool kept = false, pd = false;
         const T now = minimize_step<T, NS, NP, XS, D, LAZY, ONLY>(tm, s, prm, park, x, st, flags,
                                                                   zc, ls, &kept, &pd);
         if (ONLY && pd) {
             *pend = true;
-            return it;
+            done = it;
+        } else {
+            const T d = now - chi2;
+            chi2 = now;
+            if (mabs(d) < T(prm.chi2_tol)) {                    // :162
+                done = it + 1;
+            } else if (LAZY && kept && now - now == T(0) && T(prm.chi2_tol) > T(0) &&
+                       it + 1 < prm.max_minimize) {
+                // LAZY: an iteration that kept x is followed by one that
+                // rebuilds the same covariance from the same x, passes the
+                // same stop test and returns the same chi2 bit for bit; its
+                // difference is 0, so Minimize ends after it (a finite chi2
+                // and a positive tolerance).  It is counted, not run.
+                done = it + 2;
+            }
         }
-        const T d = now - chi2;
-        chi2 = now;
-        if (mabs(d) < T(prm.chi2_tol)) return it + 1;           // :162
-        // LAZY: an iteration that kept x is followed by one that rebuilds the
-        // same covariance from the same x, passes the same stop test and
-        // returns the same chi2 bit for bit; its difference is 0, so Minimize
-        // ends after it (a finite chi2 and a positive tolerance).  It is
-        // counted, not run.
-        if (LAZY && kept && now - now == T(0) && T(prm.chi2_tol) > T(0) &&
-            it + 1 < prm.max_minimize)
-            return it + 2;
     }
+    if (done >= 0) return done;
     flags |= F_MINIMIZE_CAP;
     return it;
 }
