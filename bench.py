@@ -198,7 +198,7 @@ def errors(x, chi2, xr, chi2r, floor=1.0):
             "amplitude_floor_adc": floor}
 
 
-def event_stream(args, eng, w, log):
+def event_stream(args, eng, w, log, with_cpu=True):
     """Event stream: `events` events back to back, each one fit call on one
     event's channels (both types, given as channel ranges), replayed from a
     captured graph; per-event latency from HIP events around each replay.
@@ -346,14 +346,78 @@ def event_stream(args, eng, w, log):
         "accuracy_vs_plain_fp64": acc,
         "mlp": {k: v for k, v in mlp_info.items() if k != "report"},
     }
-    if not args.no_cpu:
+    if not args.no_cpu and with_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         cb = cpu_baseline(ct, sample, w.event_types(), args.cpu_seconds, threads)
         cb["sample"] = (f"one event ({cb['slice']} channels) fitted by the oracle: median of "
                         f"5 runs of {args.cpu_seconds / 6:.0f} s on {threads} threads; "
                         f"single_thread: its first {cb['slice_1t']} channels on one thread")
         line["cpu_baseline"] = cb
-    print(json.dumps(line), flush=True)
+    return line
+
+
+def batch_block(args, cfg, stream, threads):
+    """Another batch workload (zero pulse covariance) timed as the headline:
+    warm, cold and plain FP64 paths, autotuned, over the workload's whole
+    batch; the warm path's errors against the plain fit and its channel
+    accounting against the oracle over the whole batch; the warm kernels'
+    roofline."""
+    import torch
+    from oracle_binding import oracle_fit, oracle_opts
+    from test_regression_amd import _abi, engine, warmstart, workloads
+    w = workloads.config(cfg, "zero")
+    nch = w.nch
+    ct = w.chtypes()
+    s, t, _ = w.generate(nch, nthreads=threads)
+    eng = engine.Engine(list(ct))
+    mlp_info = warmstart.ensure_trained(eng, warmstart.training_workload(cfg),
+                                        path=warmstart.weights_for(cfg), log=log)
+    s64 = torch.from_numpy(s).cuda()
+    s32 = s64.float()
+    tt = torch.from_numpy(t).cuda() if t is not None else None
+    O = _abi.default_opts
+    auto = dict(block=0, persistent=_abi.MF_PERSISTENT_AUTO)
+    paths = {
+        "plain": (s64, O(precision=_abi.MF_PREC_FP64, block=0), _abi.MF_PREC_FP64),
+        "fast": (s32, O(precision=_abi.MF_PREC_FP32, **auto), _abi.MF_PREC_FP32),
+        "warm": (s32, O(precision=_abi.MF_PREC_FP32, warm_start=1, **auto), _abi.MF_PREC_FP32),
+    }
+    log(f"{cfg}: {nch} channels")
+    res, outs, tuned, graph_ms = time_paths(eng, paths, tt, nch, args, stream)
+    o_np = {k: np_out(v) for k, v in outs.items()}
+    del outs
+    s_r = np.ascontiguousarray(s.astype(np.float32).astype(np.float64))
+    ref = oracle_fit(ct, s_r, t, oracle_opts(record_margins=1), nthreads=threads)
+    orc = oracle_sample(ct, s_r, t, args.oracle_sample)
+    af = float(orc["aflops"].mean())
+    k = res["warm"]["fit_ms"] + res["warm"]["mlp_ms"]
+    a = af * nch / (k / 1e3) / 1e12
+    blk = {
+        "workload": f"{cfg}: {w.description}",
+        "channels": nch,
+        "ms_per_step": {p: v["ms"] for p, v in res.items()},
+        "channels_per_s": {p: nch / (v["ms"] / 1e3) for p, v in res.items()},
+        "kernel_ms": {p: v["fit_ms"] for p, v in res.items()},
+        "warm_graph_ms_per_step": graph_ms,
+        "speedup_warm_vs_plain_fp64": res["plain"]["ms"] / res["warm"]["ms"],
+        "speedup_cold_vs_plain_fp64": res["plain"]["ms"] / res["fast"]["ms"],
+        "roofline": {"warm": {"bound": "valu", "achieved": a, "peak": PEAK_TFLOPS["f32"],
+                              "unit": "TFLOP/s", "frac": a / PEAK_TFLOPS["f32"], "kernel_ms": k,
+                              "kernel": fit_kernel_name("zero") + ", per type"},
+                     "aflops_per_channel": af, "achieved_counts": ACHIEVED_COUNTS},
+        "accuracy": {"warm_vs_plain_fp64": errors(*o_np["warm"][:2], *o_np["plain"][:2]),
+                     "warm_vs_oracle": accounting(o_np["warm"], ref, 1e-4, False,
+                                                  sets=(ct, s_r, t))},
+        "solves_per_channel": {p: float(((o_np[p][3] >> 16) & 0xFFFF).mean())
+                               for p in ("warm", "fast")},
+        "autotune": tuned or None,
+        "mlp": {k2: v for k2, v in mlp_info.items() if k2 != "report"},
+    }
+    log(f"{cfg}: plain {res['plain']['ms']:.3f} ms, cold {res['fast']['ms']:.3f} ms, warm "
+        f"{res['warm']['ms']:.3f} ms ({blk['speedup_warm_vs_plain_fp64']:.2f}x)")
+    del eng, ref
+    torch.cuda.empty_cache()
+    return blk
 
 
 def main():
@@ -383,6 +447,9 @@ def main():
                          "of --probe-paths; no timing, no JSON")
     ap.add_argument("--no-live", dest="live", action="store_false",
                     help="skip the live-pulse-covariance block of the default workload")
+    ap.add_argument("--no-workloads", dest="workloads", action="store_false",
+                    help="skip the c3 (mixed batch) and c5 (event stream) blocks of the default "
+                         "run")
     ap.add_argument("--probe-paths", default="warm,fast",
                     help="paths of --traffic-probe, in order: any of plain, plain_chol, fast, "
                          "warm")
@@ -399,7 +466,8 @@ def main():
 
     w = workloads.config(args.config, args.pulse_cov)
     if w.event:
-        event_stream(args, engine.Engine(list(w.chtypes())), w, log)
+        print(json.dumps(event_stream(args, engine.Engine(list(w.chtypes())), w, log)),
+              flush=True)
         return
     nch = args.nch or w.nch
     ct = w.chtypes()
@@ -527,6 +595,13 @@ def main():
         line["mlp"]["train"] = mlp_info["report"]
     if args.live and args.pulse_cov == "zero" and args.config == "c2":
         line["live"] = live_block(args, stream, threads)
+    if args.workloads and args.pulse_cov == "zero" and args.config == "c2" and not args.nch:
+        # the other configurations, timed in the same run: the uneven
+        # mixed-type batch and the event stream
+        line["workloads"] = {"c3": batch_block(args, "c3", stream, threads)}
+        w5 = workloads.config("c5", "zero")
+        line["workloads"]["c5"] = event_stream(args, engine.Engine(list(w5.chtypes())), w5, log,
+                                               with_cpu=False)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         cb = cpu_baseline(ct, s, t, args.cpu_seconds, threads)
