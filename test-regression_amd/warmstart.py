@@ -24,6 +24,9 @@ WEIGHTS_BY_WORKLOAD = {"c3": WEIGHTS.parent / "mlp_c3.npy"}
 
 
 TRAIN_SEED_OFFSET = 1000
+# validation channels for choosing among candidate networks: a seed of their
+# own, distinct from the training channels and from every benchmark batch
+VAL_SEED_OFFSET = 2000
 
 
 def weights_for(workload: str) -> Path:
@@ -94,33 +97,65 @@ def evaluate(eng, samples32: torch.Tensor, xref: torch.Tensor) -> dict:
 
 
 def ensure_trained(eng, workload_cfg: str = "c4", n_train: int = 4_000_000, epochs: int = 400,
-                   path: Path | None = WEIGHTS, log=None) -> dict:
+                   path: Path | None = WEIGHTS, log=None, candidates: int = 1,
+                   n_val: int = 500_000) -> dict:
     """Loads saved parameters, or trains on n_train channels of a workload's
     distribution (c4: the c2 distribution; a mixed workload such as c3: its
     mixture of channel types, with their type ids), labelled by the plain
-    FP64 fit, and saves them.  The committed
-    parameters were made this way: 400 epochs of batch 8192, Adam with a
-    cosine learning rate from 3e-3, seed 0 (about two minutes on one
-    MI355X)."""
+    FP64 fit, and saves them.  Training: 400 epochs of batch 8192, Adam with
+    a cosine learning rate from 3e-3 (about two minutes on one MI355X).
+
+    candidates > 1 trains that many networks (initialisation and shuffling
+    seeds 0, 1, ...) and keeps the one whose predicted passive sets are most
+    often exact on n_val validation channels of the same distribution under a
+    seed of their own.  Training is not bit-reproducible (the gradient sums
+    use atomics), and networks trained alike differ by several points of
+    exact-set rate; the choice is made on the validation channels, never on a
+    benchmark batch.  The committed parameters are the best of 4."""
     from . import workloads
     if path is not None and Path(path).exists():
         eng.mlp_set_params(np.load(path))
         return {"source": str(path)}
-    w = workloads.config(workload_cfg)
-    if workload_cfg != "c4":
-        # held out: not the seed of the workload's own benchmark batch (c4 is
-        # the c2 distribution under a seed of its own already)
-        w.seed += TRAIN_SEED_OFFSET
-    s, t, _ = w.generate(n_train)
-    s64 = torch.from_numpy(s).cuda()
-    tt = torch.from_numpy(t).cuda() if t is not None else None
+
+    def channels(offset, n):
+        w = workloads.config(workload_cfg)
+        w.seed += offset
+        s, t, _ = w.generate(n)
+        return (torch.from_numpy(s).cuda(),
+                torch.from_numpy(t).cuda() if t is not None else None)
+
+    # held out: not the seed of the workload's own benchmark batch (c4 is the
+    # c2 distribution under a seed of its own already)
+    s64, tt = channels(TRAIN_SEED_OFFSET if workload_cfg != "c4" else 0, n_train)
     t0 = time.perf_counter()
     xref = label(eng, s64, tt)
     torch.cuda.synchronize()
     lab = time.perf_counter() - t0
-    rep = train(eng, s64.float(), xref, epochs=epochs, log=log)
+    s32 = s64.float()
+    del s64
+    val = None
+    if candidates > 1:
+        v64, vt = channels(VAL_SEED_OFFSET, n_val)
+        val = (v64.float(), label(eng, v64, vt))
+        del v64
+    best, scores, rep = None, [], None
+    for k in range(candidates):
+        rep = train(eng, s32, xref, epochs=epochs, seed=k, log=log)
+        if val is None:
+            break
+        ev = evaluate(eng, *val)["exact_active_set"]
+        scores.append(ev)
+        if log:
+            log(f"candidate {k}: exact passive set {ev:.4f} on {n_val} validation channels")
+        if best is None or ev > best[0]:
+            best = (ev, eng.mlp_get_params(), k, rep)
     rep.label_seconds = lab
+    info = {"source": "trained", "report": rep.__dict__}
+    if best is not None:
+        eng.mlp_set_params(best[1])
+        info.update(candidates=candidates, chosen=best[2], validation_exact=scores,
+                    report=best[3].__dict__)
     if path is not None:
         Path(path).parent.mkdir(parents=True, exist_ok=True)
         np.save(path, eng.mlp_get_params())
-    return {"source": "trained", "report": rep.__dict__}
+    return info
