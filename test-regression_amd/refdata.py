@@ -9,8 +9,8 @@ and its CPU driver's fit set-up on top of the engine.
 - driver_chtype: the DoFit constants multifit_cpu/src/multifit.cpp:44-83 sets
   up (template, noise correlation, active BX -5..4) with pedrms 1 (:194-196)
   and a zero pulse covariance.
-- read_samples: the `Samples` tree of gen/src/gen_data.cpp:224-248 (what
-  multifit.cpp:88-94 reads).
+- read_samples / write_samples: the `Samples` tree of
+  gen/src/gen_data.cpp:224-248 (what multifit.cpp:88-94 reads).
 - fit_samples / write_reco: the driver's per-entry outputs
   (multifit.cpp:196-227): chi2, the in-time amplitude and samplesReco (the
   amplitudes by BX + 5), fitted on the GPU.  The reference stores them as the
@@ -245,6 +245,45 @@ def write_tree(path: str, tree: str, branches: dict, title: str = "") -> None:
     _abi.check(_abi.lib().mf_io_write_tree(str(path).encode(), tree.encode(), title.encode(),
                                            C.cast(arr, C.c_void_p), len(cols), n or 0),
                "mf_io_write_tree")
+
+
+# The generator's Samples branches in its order (gen/src/gen_data.cpp:225-248)
+# with their kinds; wf_name (/C, a string) is not written.
+SAMPLES_BRANCHES = (
+    ("pulse_shift", _abi.MF_IO_F32), ("pileup_shift", _abi.MF_IO_F32),
+    ("nSmpl", _abi.MF_IO_I32), ("nFreq", _abi.MF_IO_F32),
+    ("amplitudeTruth", _abi.MF_IO_F64), ("samples", _abi.MF_IO_VEC_F64),
+    ("samples_noise", _abi.MF_IO_VEC_F64), ("nPU", _abi.MF_IO_F32), ("BX0", _abi.MF_IO_I32),
+    ("nBX", _abi.MF_IO_I32), ("nWF", _abi.MF_IO_I32), ("nMinBias", _abi.MF_IO_VEC_F64),
+    ("energyPU", _abi.MF_IO_VEC_F64), ("pulse_signal", _abi.MF_IO_VEC_F64),
+    ("pileup_signal", _abi.MF_IO_VEC_F64), ("signalTruth", _abi.MF_IO_F64),
+    ("sigmaNoise", _abi.MF_IO_F32), ("puFactor", _abi.MF_IO_F32),
+    ("pulse_tau", _abi.MF_IO_F32), ("input_pedestal", _abi.MF_IO_F32),
+    ("distortion_sample_4", _abi.MF_IO_F32))
+
+
+def write_samples(path: str, samples: np.ndarray, amplitude_truth: np.ndarray,
+                  extra: dict | None = None) -> None:
+    """Writes a `Samples` tree as the generator lays it out
+    (gen/src/gen_data.cpp:224-248), readable by read_samples and by the
+    reference's driver (multifit.cpp:88-94): samples [nsamples, n] (one
+    vector per entry), amplitudeTruth [n], nSmpl (the sample count), and any
+    other generator branch given in extra (name -> per-entry values, or a
+    list of per-entry sequences for the vector branches), all in the
+    generator's branch order."""
+    s = np.asarray(samples, dtype=np.float64)
+    if s.ndim != 2:
+        raise _abi.MFError("write_samples: samples must be [nsamples, n]")
+    n = s.shape[1]
+    vals = dict(extra or {})
+    unknown = set(vals) - {name for name, _ in SAMPLES_BRANCHES}
+    if unknown:
+        raise _abi.MFError(f"write_samples: not a generator branch: {sorted(unknown)}")
+    vals["samples"] = list(np.ascontiguousarray(s.T))
+    vals["amplitudeTruth"] = np.asarray(amplitude_truth, dtype=np.float64)
+    vals.setdefault("nSmpl", np.full(n, s.shape[0]))
+    write_tree(path, "Samples", {name: (kind, vals[name]) for name, kind in SAMPLES_BRANCHES
+                                 if name in vals})
 
 
 def write_reco(path: str, fit: dict, ps: PulseShape, source: dict | None = None) -> None:

@@ -223,3 +223,57 @@ def test_tree_record_matches_root(pkg, tmp_path):
             assert oo[k] == v, (name, k)
         (lc_o, lo), (lc_r, lr) = oo["fLeaves"]["items"][0], orr["fLeaves"]["items"][0]
         assert lc_o == lc_r and lo == lr, name
+
+
+def test_samples_tree_round_trip(pkg, tmp_path):
+    """write_samples -> read_samples gives the samples and amplitudes back bit
+    for bit; the extra generator branches read back as written."""
+    from test_regression_amd import _abi, refdata, workloads
+    w = workloads.config("c1")
+    s, _, truth = w.generate(500, with_truth=True)
+    amp = truth[5].copy()   # the in-time (BX 0) amplitude
+    out = tmp_path / "samples.root"
+    rng = np.random.default_rng(3)
+    extra = {"BX0": rng.integers(0, 10, 500), "pulse_tau": rng.normal(43, 1, 500),
+             "energyPU": [rng.normal(size=rng.integers(0, 12)) for _ in range(500)]}
+    refdata.write_samples(str(out), s, amp, extra)
+    s2, a2 = refdata.read_samples(str(out))
+    assert np.array_equal(s2, s) and np.array_equal(a2, amp)
+    with refdata.RootFile(str(out)) as f:
+        assert f.tree_entries("Samples") == 500
+        v, _ = f.read_branch("Samples", "nSmpl", _abi.MF_IO_I32)
+        assert np.array_equal(v, np.full(500, 10))
+        v, _ = f.read_branch("Samples", "BX0", _abi.MF_IO_I32)
+        assert np.array_equal(v, extra["BX0"])
+        v, _ = f.read_branch("Samples", "pulse_tau", _abi.MF_IO_F32)
+        assert np.array_equal(v, extra["pulse_tau"].astype(np.float32))
+        v, c = f.read_branch("Samples", "energyPU", _abi.MF_IO_VEC_F64)
+        assert np.array_equal(c, [len(e) for e in extra["energyPU"]])
+        assert np.array_equal(v, np.concatenate(extra["energyPU"]))
+    names = list(rootwalk.branches(rootwalk.tree(out.read_bytes(), "Samples")))
+    assert names == ["nSmpl", "amplitudeTruth", "samples", "BX0", "energyPU", "pulse_tau"]
+    with pytest.raises(_abi.MFError):
+        refdata.write_samples(str(tmp_path / "bad.root"), s, amp, {"no_such_branch": amp})
+
+
+@needs_ref
+def test_samples_branches_follow_the_generator_file(pkg):
+    """The writer's table of Samples branches is the reference file's branch
+    list, in its order and with its leaf classes (wf_name, a string, is the
+    one branch not written)."""
+    from test_regression_amd import _abi, refdata
+    t = rootwalk.tree(F100.read_bytes(), "Samples")
+    leaf = {_abi.MF_IO_F32: "TLeafF", _abi.MF_IO_F64: "TLeafD", _abi.MF_IO_I32: "TLeafI",
+            _abi.MF_IO_VEC_F64: None}
+    ref = []
+    for name, (cls, obj) in rootwalk.branches(t).items():
+        if name == "wf_name":
+            continue
+        tb = obj["branch"] if cls == "TBranchElement" else obj
+        ref.append((name, cls, tb["fLeaves"]["items"][0][0]))
+    assert [r[0] for r in ref] == [n for n, _ in refdata.SAMPLES_BRANCHES]
+    for (name, cls, lc), (_, kind) in zip(ref, refdata.SAMPLES_BRANCHES):
+        if kind == _abi.MF_IO_VEC_F64:
+            assert cls == "TBranchElement", name
+        else:
+            assert (cls, lc) == ("TBranch", leaf[kind]), name
