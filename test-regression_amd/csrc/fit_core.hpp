@@ -435,9 +435,11 @@ MF_HD bool solve_passive(const T (&M)[10][10], const T (&b)[10], uint32_t pm, T 
 // instead of being formed (and held) in registers.
 #if defined(__HIP_DEVICE_COMPILE__)
 template <typename T> using GlobalPtr = const __attribute__((address_space(1))) T *;
+template <typename T> using GlobalOut = __attribute__((address_space(1))) T *;
 typedef double Dbl2 __attribute__((ext_vector_type(2)));
 #else
 template <typename T> using GlobalPtr = const T *;
+template <typename T> using GlobalOut = T *;
 struct Dbl2 {
     double v[2];
     double operator[](int e) const { return v[e]; }

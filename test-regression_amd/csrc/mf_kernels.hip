@@ -83,10 +83,17 @@ struct Seg {
     const int64_t *doffset;  // device: first item's position in idx (null: 0)
     int64_t n;
 };
-__device__ __forceinline__ int64_t seg_count(const Seg &s) { return s.dcount ? *s.dcount : s.n; }
-__device__ __forceinline__ const int32_t *seg_list(const Seg &s)
+// The pointers of a Seg are global memory; read through the global address
+// space (pointers held in a struct argument would otherwise be read with
+// flat loads).
+__device__ __forceinline__ int64_t seg_count(const Seg &s)
 {
-    return s.idx ? s.idx + (s.doffset ? *s.doffset : 0) : nullptr;
+    return s.dcount ? *(GlobalPtr<int64_t>)s.dcount : s.n;
+}
+__device__ __forceinline__ GlobalPtr<int32_t> seg_list(const Seg &s)
+{
+    return s.idx ? (GlobalPtr<int32_t>)s.idx + (s.doffset ? *(GlobalPtr<int64_t>)s.doffset : 0)
+                 : nullptr;
 }
 
 // What one launch of a fit kernel covers: either one list of one type
@@ -112,7 +119,7 @@ __device__ __forceinline__ bool seg_item(const SegArg &a, int &t, int64_t &c,
     if (a.nrange == 0) {
         t = a.type_id;
         if (gt >= seg_count(a.list)) return false;
-        const int32_t *idx = seg_list(a.list);
+        const GlobalPtr<int32_t> idx = seg_list(a.list);
         c = idx ? (int64_t)idx[gt] : gt;
         return true;
     }
@@ -253,7 +260,9 @@ struct NoPark {
 };
 // Reads a channel's samples again where phase 2 uses them, so that they need
 // not stay in registers (or spill) through the solves.  The pointer is
-// laundered so the compiler does not reuse the values it loaded before.
+// laundered so the compiler does not reuse the values it loaded before, then
+// cast back to the global address space: a generic pointer would make these
+// flat loads, which the scalar loads' waits also wait for.
 template <typename TS, int N> struct SampleReload {
     const TS *s;
     int64_t nch, c;
@@ -261,8 +270,9 @@ template <typename TS, int N> struct SampleReload {
     {
         const TS *q = s;
         asm volatile("" : "+s"(q));
+        const GlobalPtr<TS> g = (GlobalPtr<TS>)q;
 #pragma unroll
-        for (int i = 0; i < 10; ++i) v[i] = i < N ? (float)q[(int64_t)i * nch + c] : 0.0f;
+        for (int i = 0; i < 10; ++i) v[i] = i < N ? (float)g[(int64_t)i * nch + c] : 0.0f;
     }
 };
 // Pending lists (after the hand-over's FP64 values): one count per range of
@@ -504,29 +514,34 @@ void k_fast_p3(const DevType *__restrict__ types, SegArg seg,
     bool pend;
     const int nmin = fast_p2_rest_lazy<N, N, D>(types[type_id], s, prm, x, chi2, st, flags, pend);
     // the stores' addresses are formed here again from laundered bases, not
-    // kept in registers through the iterations
+    // kept in registers through the iterations; the bases are cast back to
+    // the global address space (generic pointers would make flat stores)
     asm volatile("" : "+s"(hand), "+s"(xo), "+s"(chi2o), "+s"(flo), "+s"(ito));
+    const GlobalOut<double> hg = (GlobalOut<double>)hand;
+    const GlobalOut<int32_t> fg = (GlobalOut<int32_t>)flo, ig = (GlobalOut<int32_t>)ito;
     if (pend) {
 #pragma unroll
-        for (int p = 0; p < 10; ++p) hand[(int64_t)p * nch + c] = x[p];
-        hand[10 * nch + c] = chi2;
-        flo[c] = flags | F_PENDING | (st.hit_max ? F_FNNLS_CAP : 0) |
-                 (st.llt_fail ? F_LLT_FAIL : 0);
-        ito[c] = nmin | (st.outer << 8) | (st.inner << 16);
-        unsigned *pq = hand_pending(hand, nch);
-        const unsigned k = atomicAdd(pq + r, 1u);
+        for (int p = 0; p < 10; ++p) hg[(int64_t)p * nch + c] = x[p];
+        hg[10 * nch + c] = chi2;
+        fg[c] = flags | F_PENDING | (st.hit_max ? F_FNNLS_CAP : 0) |
+                (st.llt_fail ? F_LLT_FAIL : 0);
+        ig[c] = nmin | (st.outer << 8) | (st.inner << 16);
+        const GlobalOut<unsigned> pq = (GlobalOut<unsigned>)hand_pending(hand, nch);
+        const unsigned k =
+            __hip_atomic_fetch_add(pq + r, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const int64_t base = seg.nrange ? seg.wave0[r] * 64 : 0;
-        reinterpret_cast<int32_t *>(pq + 64)[base + k] =
+        ((GlobalOut<int32_t>)(pq + 64))[base + k] =
             (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
         return;
     }
     FitOut<float> out;
     fast_p2_out(x, chi2, flags, nmin, st, out);
+    const GlobalOut<float> xg = (GlobalOut<float>)xo, cg = (GlobalOut<float>)chi2o;
 #pragma unroll
-    for (int p = 0; p < 10; ++p) xo[(int64_t)p * nch + c] = out.x[p];
-    chi2o[c] = out.chi2;
-    flo[c] = out.flags;
-    ito[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
+    for (int p = 0; p < 10; ++p) xg[(int64_t)p * nch + c] = out.x[p];
+    cg[c] = out.chi2;
+    fg[c] = out.flags;
+    ig[c] = out.n_min | (out.n_outer << 8) | (out.n_inner << 16);
 }
 
 // The pending channels of k_fast_p3: grid row blockIdx.y takes range
@@ -557,7 +572,7 @@ void k_fast_p4(const DevType *__restrict__ types, SegArg seg,
         const int64_t gt = list[k];
         int64_t c;
         if (seg.nrange == 0) {
-            const int32_t *idx = seg_list(seg.list);
+            const GlobalPtr<int32_t> idx = seg_list(seg.list);
             c = idx ? (int64_t)idx[gt] : gt;
         } else {
             c = seg.rfirst[r] + (gt - seg.wave0[r] * 64);
@@ -682,7 +697,7 @@ void k_fast_p1q(const DevType *__restrict__ types, int type_id, Seg seg,
 {
     const DevType &ty = types[type_id];
     const int64_t nitems = seg_count(seg);
-    const int32_t *idx = seg_list(seg);
+    const GlobalPtr<int32_t> idx = seg_list(seg);
     int64_t c = -1, cn = -1;
     float Atb[10], svn[10], xn[10];
     float eps1 = 0.0f;   // the lane's threshold: eps times its channel's noise scale
@@ -775,7 +790,7 @@ void k_fast_p2q(const DevType *__restrict__ types, int type_id, Seg seg,
     double rhon = 1.0;
     const DevType &ty = types[type_id];
     const int64_t nitems = seg_count(seg);
-    const int32_t *idx = seg_list(seg);
+    const GlobalPtr<int32_t> idx = seg_list(seg);
     int64_t c = -1, cn = -1;
     double Atb[10];
     ActiveState<double> S;
