@@ -37,6 +37,8 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--candidates", type=int, default=4)
+    ap.add_argument("--first-seed", type=int, default=0,
+                    help="seed of the first candidate (the others follow)")
     ap.add_argument("--n-train", type=int, default=4_000_000)
     ap.add_argument("--n-val", type=int, default=500_000)
     ap.add_argument("--epochs", type=int, default=400)
@@ -55,7 +57,8 @@ def main():
         out.unlink()
     info = warmstart.ensure_trained(eng, warmstart.training_workload(args.workload),
                                     n_train=args.n_train, epochs=args.epochs, path=out,
-                                    log=log, candidates=args.candidates, n_val=args.n_val)
+                                    log=log, candidates=args.candidates, n_val=args.n_val,
+                                    first_seed=args.first_seed)
     info.pop("report", None)
     print(json.dumps(info), flush=True)
     eng.close()

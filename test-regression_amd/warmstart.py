@@ -98,7 +98,7 @@ def evaluate(eng, samples32: torch.Tensor, xref: torch.Tensor) -> dict:
 
 def ensure_trained(eng, workload_cfg: str = "c4", n_train: int = 4_000_000, epochs: int = 400,
                    path: Path | None = WEIGHTS, log=None, candidates: int = 1,
-                   n_val: int = 500_000) -> dict:
+                   n_val: int = 500_000, first_seed: int = 0) -> dict:
     """Loads saved parameters, or trains on n_train channels of a workload's
     distribution (c4: the c2 distribution; a mixed workload such as c3: its
     mixture of channel types, with their type ids), labelled by the plain
@@ -106,7 +106,7 @@ def ensure_trained(eng, workload_cfg: str = "c4", n_train: int = 4_000_000, epoc
     a cosine learning rate from 3e-3 (about two minutes on one MI355X).
 
     candidates > 1 trains that many networks (initialisation and shuffling
-    seeds 0, 1, ...) and keeps the one whose predicted passive sets are most
+    seeds first_seed, first_seed + 1, ...) and keeps the one whose predicted passive sets are most
     often exact on n_val validation channels of the same distribution under a
     seed of their own.  Training is not bit-reproducible (the gradient sums
     use atomics), and networks trained alike differ by several points of
@@ -139,21 +139,22 @@ def ensure_trained(eng, workload_cfg: str = "c4", n_train: int = 4_000_000, epoc
         val = (v64.float(), label(eng, v64, vt))
         del v64
     best, scores, rep = None, [], None
-    for k in range(candidates):
+    for k in range(first_seed, first_seed + candidates):
         rep = train(eng, s32, xref, epochs=epochs, seed=k, log=log)
         if val is None:
             break
         ev = evaluate(eng, *val)["exact_active_set"]
         scores.append(ev)
         if log:
-            log(f"candidate {k}: exact passive set {ev:.4f} on {n_val} validation channels")
+            log(f"candidate (seed {k}): exact passive set {ev:.4f} on {n_val} validation channels")
         if best is None or ev > best[0]:
             best = (ev, eng.mlp_get_params(), k, rep)
     rep.label_seconds = lab
     info = {"source": "trained", "report": rep.__dict__}
     if best is not None:
         eng.mlp_set_params(best[1])
-        info.update(candidates=candidates, chosen=best[2], validation_exact=scores,
+        info.update(candidates=candidates, first_seed=first_seed, chosen=best[2],
+                    validation_exact=scores,
                     report=best[3].__dict__)
     if path is not None:
         Path(path).parent.mkdir(parents=True, exist_ok=True)
