@@ -23,9 +23,10 @@ TOL_CHI2 = 1e-4
 ref_sets = {}
 
 
-def _fit_fast(cfg, pulse_cov, n, sample_dtype=np.float32, **kw):
+def _fit_fast(cfg, pulse_cov, n, sample_dtype=np.float32, seed=0, **kw):
     from test_regression_amd import engine, workloads, warmstart, _abi
     w = workloads.config(cfg, pulse_cov)
+    w.seed += seed
     ct = w.chtypes()
     s, t, _ = w.generate(n)
     s_in = s.astype(sample_dtype)
@@ -33,7 +34,7 @@ def _fit_fast(cfg, pulse_cov, n, sample_dtype=np.float32, **kw):
     if kw.get("warm_start"):
         warmstart.ensure_trained(eng, path=warmstart.weights_for(cfg))
     print(f"[gpu-fast] cfg={cfg} pulse_cov={pulse_cov} n={n} dtype={sample_dtype.__name__} "
-          f"{kw}", flush=True)
+          f"seed+{seed} {kw}", flush=True)
     r = eng.fit(torch.from_numpy(s_in).cuda(),
                 torch.from_numpy(t).cuda() if t is not None else None,
                 _abi.default_opts(precision=_abi.MF_PREC_FP32, **kw))
@@ -69,6 +70,17 @@ def test_fast_fit_matches_oracle(pkg, gpu, cfg, pulse_cov, n, warm):
     fl = r.flags.cpu().numpy()
     from test_regression_amd import _abi
     assert not np.any(fl & _abi.FLAGS["nonfinite"])
+
+
+@pytest.mark.parametrize("seed", [11, 12, 13])
+@pytest.mark.parametrize("cfg,pulse_cov,n", [("c2", "zero", 200_000), ("c2", "live", 100_000),
+                                             ("c3", "zero", 200_000)])
+def test_fast_fit_seed_sweep(pkg, gpu, cfg, pulse_cov, n, seed):
+    """The warm-started path on batches generated under other seeds than the
+    benchmark's and the training's: every channel accounted for against the
+    oracle, as above."""
+    r, ref = _fit_fast(cfg, pulse_cov, n, seed=seed, warm_start=1)
+    _check(r, ref, f"fast {cfg} {pulse_cov} warm seed+{seed}")
 
 
 def test_fast_fit_fp64_samples(pkg, gpu):
