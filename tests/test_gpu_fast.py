@@ -1,6 +1,6 @@
 """GPU parity of the fast path (MF_PREC_FP32) against the FP64 oracle.
 
-Bar (FP32 bulk, north-star accuracy), channel by channel (tests/parity.py):
+Bar (the FP32 fast path's accuracy target), channel by channel (tests/parity.py):
 every amplitude within 1e-4 of the oracle's relative to itself (floor 1 ADC),
 chi2 within 1e-4 relative (floor 1), the same active set and Minimize
 iteration count; a channel outside that must be explained by the oracle's

@@ -2,8 +2,8 @@
 
 mf_inplace_fnnls_batch / mf_fnnls_batch against the oracle's inplace_fnnls /
 fnnls, against exact brute force (m <= 7) and by the KKT conditions (a proof
-of optimality for this convex problem; scipy 1.15's nnls is not used as a
-judge: it returns non-optimal points on some of these inputs); mf_linalg_batch against the
+of optimality for this convex problem; scipy 1.15's nnls is not used as the
+reference: it returns non-optimal points on some of these inputs); mf_linalg_batch against the
 oracle's restatement of linalgebra/interface/*.hpp (same operation order:
 bit-identical in FP64 up to fma contraction, checked to 1e-13 relative).
 """

@@ -7,7 +7,7 @@ Checks, per oracle function:
   inplace_fnnls      scipy's Lawson-Hanson NNLS, brute force over all passive
                      sets, KKT conditions (scipy 1.15's nnls returns
                      non-optimal points on some inputs, so brute force and
-                     KKT are the judges where they disagree), and the reference's OpenCL
+                     KKT decide where they disagree), and the reference's OpenCL
                      incremental-Cholesky FNNLS (tests/golden/fnnls_cl_golden.npz)
   fnnls              scipy, brute force
   DoFit              closed forms: noise-free recovery, identity covariance
