@@ -18,10 +18,16 @@ def _event(w, k, seed_off):
 
 @pytest.mark.parametrize("mode,layout", [("plain", "type_of"), ("fast", "type_of"),
                                          ("warm", "type_of"), ("persistent", "type_of"),
-                                         ("plain", "ranges"), ("warm", "ranges")])
+                                         ("plain", "ranges"), ("warm", "ranges"),
+                                         ("warm_live", "type_of"), ("fast_live", "ranges"),
+                                         ("plain_live", "ranges")])
 def test_graph_replays_equal_eager(pkg, gpu, mode, layout):
+    """_live: a live pulse covariance (the fast path's hand-over scratch and
+    pending lists, cleared inside the captured kernels)."""
     from test_regression_amd import engine, workloads, warmstart, _abi
-    w = workloads.config("c5")
+    live = mode.endswith("_live")
+    mode = mode.removesuffix("_live")
+    w = workloads.config("c5", "live" if live else "zero")
     eng = engine.Engine(list(w.chtypes()))
     if mode == "warm":
         warmstart.ensure_trained(eng)
@@ -37,7 +43,7 @@ def test_graph_replays_equal_eager(pkg, gpu, mode, layout):
     if layout == "ranges":
         tt = None
         kw["type_ranges"] = [0, w.event[0], sum(w.event)]
-    print(f"[gpu-graph] mode={mode} layout={layout} nch={x.shape[1]}", flush=True)
+    print(f"[gpu-graph] mode={mode} live={live} layout={layout} nch={x.shape[1]}", flush=True)
     ref = eng.fit(x, tt, o, **kw)
     torch.cuda.synchronize()
     g = eng.capture(x, tt, o, **kw)
