@@ -18,6 +18,8 @@ import json
 import sys
 from pathlib import Path
 
+import numpy as np
+
 PKG = Path(__file__).resolve().parent
 
 
@@ -52,14 +54,17 @@ def main():
 
     w = workloads.config(args.workload)
     eng = engine.Engine(list(w.chtypes()))
-    out = Path(args.out)
-    if out.exists():
-        out.unlink()
+    # trained in memory (path None: nothing is loaded or saved there); the
+    # output file is written only once training has finished
     info = warmstart.ensure_trained(eng, warmstart.training_workload(args.workload),
-                                    n_train=args.n_train, epochs=args.epochs, path=out,
+                                    n_train=args.n_train, epochs=args.epochs, path=None,
                                     log=log, candidates=args.candidates, n_val=args.n_val,
                                     first_seed=args.first_seed)
+    out = Path(args.out)
+    out.parent.mkdir(parents=True, exist_ok=True)
+    np.save(out, eng.mlp_get_params())
     info.pop("report", None)
+    info["out"] = str(out)
     print(json.dumps(info), flush=True)
     eng.close()
 
