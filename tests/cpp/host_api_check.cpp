@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/mf_host.hpp"
+#include "../../include/mf_io.h"
 
 // --wrapper in.bin out.bin: doFitWrapper alone on a large batch in a compact
 // format.  input : int32 n, int32 ntypes; per type 100 samplecor, 19
@@ -83,11 +84,89 @@ static int wrapper_mode(const char *inp, const char *outp)
     return 0;
 }
 
+// --driver samples.root type.bin out.root: the CPU driver's loop
+// (multifit_cpu/src/multifit.cpp:88-94, 196-227, 266-279) in C++ on the
+// library: the Samples tree read with mf_io_*, each entry fitted by its own
+// PulseChiSqSNNLS with errors off, and the RecoAndSim tree written with
+// mf_io_write_tree: the input's samples and amplitudeTruth, chi2 as the
+// driver leaves it (-99), samplesReco (X by BX + 5) and activeBXs.
+// type.bin: 100 samplecor (row-major), 19 fullpulse, 361 fullpulsecov,
+// pederr, npulses, 10 bx (all as double).
+static int driver_mode(const char *inp, const char *typ, const char *outp)
+{
+    FILE *f = fopen(typ, "rb");
+    if (!f) return 2;
+    std::vector<double> b(100 + 19 + 361 + 1 + 1 + 10);
+    if (fread(b.data(), 8, b.size(), f) != b.size()) return 2;
+    fclose(f);
+    size_t k = 0;
+    mf::SampleMatrix sc;
+    for (int i = 0; i < 10; ++i)
+        for (int j = 0; j < 10; ++j) sc(i, j) = b[k++];
+    mf::FullSampleVector fp;
+    for (int i = 0; i < 19; ++i) fp(i) = b[k++];
+    mf::FullSampleMatrix fpc;
+    for (int i = 0; i < 19; ++i)
+        for (int j = 0; j < 19; ++j) fpc(i, j) = b[k++];
+    const double pederr = b[k++];
+    const int np = (int)b[k++];
+    mf::BXVector bx(np);
+    for (int i = 0; i < np; ++i) bx.coeffRef(i) = (char)b[k + (size_t)i];
+
+    mf_rootfile *rf = nullptr;
+    if (mf_io_open(inp, &rf) != MF_OK) return 3;
+    int64_t nv = 0, ne = 0;
+    // first call counts, second reads
+    mf_io_read_branch(rf, "Samples", "samples", MF_IO_VEC_F64, nullptr, 0, &nv, nullptr, 0, &ne);
+    std::vector<double> samples((size_t)nv);
+    std::vector<int64_t> counts((size_t)ne);
+    if (mf_io_read_branch(rf, "Samples", "samples", MF_IO_VEC_F64, samples.data(), nv, &nv,
+                          counts.data(), ne, &ne) != MF_OK)
+        return 3;
+    std::vector<double> truth((size_t)ne);
+    int64_t nt = 0, net = 0;
+    if (mf_io_read_branch(rf, "Samples", "amplitudeTruth", MF_IO_F64, truth.data(), ne, &nt,
+                          nullptr, 0, &net) != MF_OK || net != ne)
+        return 3;
+    mf_io_close(rf);
+
+    std::vector<double> chi2((size_t)ne, -99.0), reco((size_t)ne * 10, 0.0), bxs;
+    std::vector<int64_t> reco_n((size_t)ne, 10), bx_n((size_t)ne, np);
+    try {
+        size_t off = 0;
+        for (int64_t e = 0; e < ne; ++e) {
+            if (counts[(size_t)e] != 10) return 4;
+            mf::SampleVector s;
+            for (int i = 0; i < 10; ++i) s(i) = samples[off + (size_t)i];
+            off += 10;
+            mf::PulseChiSqSNNLS pulse;
+            pulse.disableErrorCalculation();
+            pulse.DoFit(s, sc, pederr, bx, fp, fpc);
+            for (int p = 0; p < np; ++p) reco[(size_t)e * 10 + (size_t)(bx.coeff(p) + 5)] = pulse.X()[p];
+            for (int p = 0; p < np; ++p) bxs.push_back(bx.coeff(p));
+        }
+    } catch (const mf::Error &err) {
+        fprintf(stderr, "mf::Error %d: %s\n", err.status, err.what());
+        return 1;
+    }
+    const mf_io_column cols[] = {
+        {"samples", MF_IO_VEC_F64, 0, samples.data(), counts.data()},
+        {"amplitudeTruth", MF_IO_F64, 0, truth.data(), nullptr},
+        {"chi2", MF_IO_F32, 0, chi2.data(), nullptr},
+        {"samplesReco", MF_IO_VEC_F64, 0, reco.data(), reco_n.data()},
+        {"activeBXs", MF_IO_VEC_I32, 0, bxs.data(), bx_n.data()},
+    };
+    return mf_io_write_tree(outp, "RecoAndSim", "", cols, 5, ne) == MF_OK ? 0 : 5;
+}
+
 int main(int argc, char **argv)
 {
     if (argc == 4 && std::string(argv[1]) == "--wrapper") return wrapper_mode(argv[2], argv[3]);
+    if (argc == 5 && std::string(argv[1]) == "--driver")
+        return driver_mode(argv[2], argv[3], argv[4]);
     if (argc != 3) {
-        fprintf(stderr, "usage: %s [--wrapper] in.bin out.bin\n", argv[0]);
+        fprintf(stderr, "usage: %s [--wrapper] in.bin out.bin | --driver samples.root type.bin out.root\n",
+                argv[0]);
         return 2;
     }
     FILE *f = fopen(argv[1], "rb");

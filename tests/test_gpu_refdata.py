@@ -70,3 +70,45 @@ def test_reference_events_recover_truth(pkg, gpu, tmp_path):
         assert np.array_equal(f.read_vectors("RecoAndSim", "samples"), s.T)
         assert np.array_equal(f.read_branch("RecoAndSim", "amplitudeTruth", _abi.MF_IO_F64)[0], a)
         assert f.read_vectors("RecoAndSim", "pulseShapeTemplate").shape == (s.shape[1], 17)
+
+
+def test_cpp_driver_round_trip(pkg, gpu, tmp_path):
+    """The CPU driver's loop in C++ on the library (tests/cpp/host_api_check
+    --driver): the events written as a Samples tree (refdata.write_samples),
+    read back with mf_io_*, fitted entry by entry by PulseChiSqSNNLS (errors
+    off, as the driver runs them), and written as RecoAndSim with
+    mf_io_write_tree.  The file read back holds the input branches bit for
+    bit and the amplitudes of the batched fit of the same events; both are
+    within the FP64 bar of the oracle."""
+    import subprocess
+    from test_regression_amd import _abi, refdata
+    prog = Path(__file__).resolve().parent / "cpp" / "host_api_check"
+    if not prog.exists():
+        pytest.fail("tests/cpp/host_api_check is not built (make -C test-regression_amd)")
+    s, a, ps, ct, cts = _setup()
+    samples = tmp_path / "samples.root"
+    refdata.write_samples(str(samples), s, a)
+    nc = np.ctypeslib.as_array(ct.noisecor)
+    tb = np.concatenate([nc.ravel(), np.ctypeslib.as_array(ct.fullpulse).ravel(),
+                         np.ctypeslib.as_array(ct.fullpulsecov).ravel(), [ct.pederr, ct.npulses],
+                         np.ctypeslib.as_array(ct.bx).astype(np.float64)[:10]])
+    typ = tmp_path / "type.bin"
+    tb.astype(np.float64).tofile(typ)
+    out = tmp_path / "reco.root"
+    print(f"[gpu-refdata] C++ driver on {s.shape[1]} events", flush=True)
+    r = subprocess.run([str(prog), "--driver", str(samples), str(typ), str(out)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+    fit = refdata.fit_samples(s, ct, _abi.default_opts())
+    ref = oracle_fit(cts, s, None, oracle_opts(), nthreads=4)
+    with refdata.RootFile(str(out)) as f:
+        assert f.tree_entries("RecoAndSim") == s.shape[1]
+        assert np.array_equal(f.read_vectors("RecoAndSim", "samples"), s.T)
+        assert np.array_equal(f.read_branch("RecoAndSim", "amplitudeTruth", _abi.MF_IO_F64)[0], a)
+        assert np.all(f.read_branch("RecoAndSim", "chi2", _abi.MF_IO_F32)[0] == -99.0)
+        reco = f.read_vectors("RecoAndSim", "samplesReco")
+        bxs = f.read_vectors("RecoAndSim", "activeBXs", _abi.MF_IO_VEC_I32)
+    assert np.array_equal(bxs, np.tile(np.array(refdata.ACTIVE_BX, dtype=np.float64), (s.shape[1], 1)))
+    scale = np.maximum(np.abs(ref["x"]).max(axis=1), 1.0)
+    assert (np.abs(reco - ref["x"]).max(axis=1) / scale).max() <= 1e-10
+    assert (np.abs(reco - fit["samplesReco"]).max(axis=1) / scale).max() <= 1e-12
