@@ -693,11 +693,27 @@ def oracle_accounting(ct, s, t, o_np, threads, eng, s64, tt, paths):
     from oracle_binding import oracle_fit, oracle_opts
     from test_regression_amd import _abi
     t0 = time.perf_counter()
+
+    def oracle_batch(samples, label):
+        # in slices of 4M channels, so that a large batch reports its progress
+        n, step = samples.shape[1], 4_000_000
+        parts = []
+        for a in range(0, n, step):
+            b = min(n, a + step)
+            parts.append(oracle_fit(ct, np.ascontiguousarray(samples[:, a:b]),
+                                    None if t is None else np.ascontiguousarray(t[a:b]),
+                                    oracle_opts(record_margins=1), nthreads=threads))
+            if n > step:
+                log(f"oracle on {label}: {b} of {n} channels ({time.perf_counter() - t0:.0f}s)")
+        return parts[0] if len(parts) == 1 else np.concatenate(parts)
+
     s_r = np.ascontiguousarray(s.astype(np.float32).astype(np.float64))
-    ref32 = oracle_fit(ct, s_r, t, oracle_opts(record_margins=1), nthreads=threads)
+    ref32 = oracle_batch(s_r, "FP32-rounded samples")
     sets = (ct, s_r, t)
-    out = {"vs_oracle": {"warm": accounting(o_np["warm"], ref32, 1e-4, False, sets=sets),
-                         "fast": accounting(o_np["fast"], ref32, 1e-4, False, sets=sets)}}
+    out = {"vs_oracle": {"warm": accounting(o_np["warm"], ref32, 1e-4, False, sets=sets)}}
+    log(f"warm path accounted ({time.perf_counter() - t0:.0f}s)")
+    out["vs_oracle"]["fast"] = accounting(o_np["fast"], ref32, 1e-4, False, sets=sets)
+    log(f"cold fast path accounted ({time.perf_counter() - t0:.0f}s)")
     fw = o_np["warm"][2]
     rej = (fw & _abi.FLAGS["warm_reject"]) != 0
     cw, cp = o_np["warm"][1], o_np["plain"][1]
@@ -708,8 +724,7 @@ def oracle_accounting(ct, s, t, o_np, threads, eng, s64, tt, paths):
         "chi2_rel_vs_plain_fp64_max": float(d[rej].max()) if rej.any() else 0.0,
         "chi2_rel_vs_plain_fp64_above_1e-4": int((d[rej] > 1e-4).sum()),
         "vs_oracle": accounting(o_np["warm"], ref32, 1e-4, False, mask=rej, sets=sets)}
-    ref64 = oracle_fit(ct, np.ascontiguousarray(s), t, oracle_opts(record_margins=1),
-                       nthreads=threads)
+    ref64 = oracle_batch(s, "FP64 samples")
     out["vs_oracle"]["plain"] = accounting(o_np["plain"], ref64, 1e-10, True)
     # FP64 samples into the FP32 path: rounded to FP32 at load, so the
     # outputs are those of the FP32 samples; their distance from the oracle

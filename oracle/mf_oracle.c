@@ -921,6 +921,19 @@ static void nnls_step(fitctx *c)
         c->force_margin = kkt_of_set(AtA, Atb, np, c->force_pm, c->o->eps, z,
                                      &c->force_feasible);
         for (int i = 0; i < np; ++i) c->x[i] = z[i];
+        if (c->mg.on) {   /* the set's block sets its amplitudes: its conditioning */
+            double B[LD * LD];
+            int idx[LD], k = 0;
+            memset(B, 0, sizeof(B));
+            for (int i = 0; i < np; ++i)
+                if ((c->force_pm >> i) & 1u) idx[k++] = i;
+            for (int a = 0; a < k; ++a)
+                for (int b2 = 0; b2 < k; ++b2) B[IDX(a, b2)] = AtA[IDX(idx[a], idx[b2])];
+            const double rc = k ? rcond1(B, k) : INFINITY;
+            for (int a = 0; a < k; ++a) c->mg.rc_of[idx[a]] = rc;
+            mmin(&c->mg.rc_min, rc);
+            c->mg.tracked = 1;
+        }
         c->force = 0;
         return;
     }

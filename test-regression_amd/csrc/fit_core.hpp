@@ -169,11 +169,20 @@ struct TypeHdr {
 // computed on the host in extended precision and rounded to FP64; slot
 // MASK_PACK - 1 holds the smallest pivot ratio (M_kk - sum_j L_kj^2) / M_kk
 // over k in m of the FP64 Cholesky of the block (<= 0 or NaN: that
-// factorization fails; the inverse is then NaN).
+// factorization fails; the inverse is then NaN).  A product with a rounded
+// inverse is not backward stable: its residual grows with the block's
+// condition number.  Where that (1-norm, from the extended-precision
+// inverse) exceeds MASK_FACTOR_COND, bit m of factored is set and fac[m]
+// holds the block's Cholesky factor, packed the same way with the
+// reciprocals of its diagonal and the same last slot, for two triangular
+// solves (solve_factor, at the end point of the steps: polish_end).
 constexpr int MASK_PACK = 56;
 constexpr int MASK_SETS = 1024;
+constexpr double MASK_FACTOR_COND = 1e6;
 struct MaskTab {
     alignas(16) double inv[MASK_SETS][MASK_PACK];
+    alignas(16) double fac[MASK_SETS][MASK_PACK];
+    uint32_t factored[MASK_SETS / 32];
 };
 
 struct DevType {
@@ -523,6 +532,43 @@ MF_HD double solve_inverse(const double *row, const double (&b)[10], uint32_t pm
     }
     if (NCU < NC) ratio = ((GlobalPtr<double>)row)[MASK_PACK - 1];
     return ratio;
+}
+
+// s = (L L^T)^{-1} b for a passive set whose MaskTab row holds the block's
+// Cholesky factor L (see there): the entries (i, j), j < i, of L by rows and
+// 1 / L_ii on the diagonal, zero outside the block.  Forward substitution
+// streams the rows in order, y_i = (b_i - sum_j L_ij y_j) / L_ii; backward
+// substitution streams them again from the last, s_k = y_k / L_kk and then
+// y_j -= L_kj s_k for j < k (s and y share the array).  A zero reciprocal
+// outside the block leaves s zero there.  Returns the pivot ratio (the row's
+// last slot), as solve_inverse does.
+template <int N>
+MF_HD double solve_factor(const double *row, const double (&b)[10], double (&s)[10])
+{
+    constexpr int NE = N * (N + 1) / 2;
+    const GlobalPtr<double> p = (GlobalPtr<double>)row;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) s[k] = k < N ? b[k] : 0.0;
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        const int i = TRI.i[e], j = TRI.j[e];
+        const double v = p[e];
+        if (j < i) s[i] = fma(-v, s[j], s[i]);
+        else s[i] *= v;
+    }
+    // the same row again for the transposed factor, last row first
+    GlobalPtr<double> q = p;
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(q) : "v"(s[N - 1]));
+#endif
+#pragma unroll
+    for (int e = NE - 1; e >= 0; --e) {
+        const int i = TRI.i[e], j = TRI.j[e];
+        const double v = q[e];
+        if (j == i) s[i] *= v;
+        else s[j] = fma(-v, s[i], s[j]);
+    }
+    return p[MASK_PACK - 1];
 }
 
 template <typename T>
@@ -1652,6 +1698,37 @@ MF_HD bool fnnls_step(const T (&M0)[10][10], const T (&Atb)[10], ActiveState<T> 
     return false;
 }
 
+// The KKT point of the table steps, solved again where its set has a MaskTab
+// factor row (an ill-conditioned block, where a
+// product with the rounded inverse is not backward stable): when that solve
+// is all positive it replaces x, and the caller repeats the KKT test by
+// running the steps again (returns true).  The steps before use the inverse
+// on every set: only the end point's accuracy rests on the solve, and the
+// factor is not on the path of the common case.  polished: the last set so
+// solved (0 for none), so that each set is solved this way once.
+template <int N>
+MF_HD bool polish_end(const MaskTab *tab, const double (&Atb)[10], ActiveState<double> &S,
+                      FnnlsStats &st, uint32_t &polished)
+{
+    const uint32_t pm = S.pm;
+    if (!pm || pm == polished || st.hit_max ||
+        !((((GlobalPtr<uint32_t>)tab->factored)[pm >> 5] >> (pm & 31u)) & 1u))
+        return false;
+    polished = pm;
+    double s[10];
+    bool allpos = solve_factor<N>(tab->fac[pm], Atb, s) > 0.0;
+    st.inner++;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        if ((pm >> i) & 1u) allpos = allpos && (s[i] > 0.0);
+    if (!allpos) return false;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        if ((pm >> i) & 1u) S.x[i] = s[i];
+    S.solve = false;
+    return true;
+}
+
 template <typename T, int N, bool UNI = false, bool TAB = false>
 MF_HD void fnnls_active(const T (&M0)[10][10], const T (&Atb)[10], T (&x)[10], uint32_t &pm,
                         T eps, T eps_rel, int max_iter, FnnlsStats &st, T tau = T(0),
@@ -1823,9 +1900,12 @@ MF_HD bool fast_p2_first(const DevType &ty, float (&s32)[10], const FitParams &p
     const bool from_empty = S.pm == 0;   // a start that is the cold start
     const MaskTab *tab = *opaque(&ty.tab);   // wave-uniform: one scalar load
     const double eps = prm.eps * prm.rho;
-    while (!fnnls_step<double, NP, true, TAB>(ty.d.AtA0, Atb64, S, eps, prm.eps_rel64,
-                                              prm.max_iter, st, 0.0, nullptr, tab)) {
-    }
+    uint32_t polished = 0;
+    do {
+        while (!fnnls_step<double, NP, true, TAB>(ty.d.AtA0, Atb64, S, eps, prm.eps_rel64,
+                                                  prm.max_iter, st, 0.0, nullptr, tab)) {
+        }
+    } while (TAB && polish_end<NP>(tab, Atb64, S, st, polished));
     flags |= F_REFINED;
     if (warm && st.outer != outer0) flags |= F_WARM_REJECT;
     if (warm && st.hit_max && !from_empty) {
@@ -1833,9 +1913,12 @@ MF_HD bool fast_p2_first(const DevType &ty, float (&s32)[10], const FitParams &p
         // reach max_iter is dropped and the channel fitted from x = 0.
         warm_fallback_reset(S, st);
         flags |= F_WARM_FALLBACK;
-        while (!fnnls_step<double, NP, true, TAB>(ty.d.AtA0, Atb64, S, eps, prm.eps_rel64,
-                                                  prm.max_iter, st, 0.0, nullptr, tab)) {
-        }
+        polished = 0;
+        do {
+            while (!fnnls_step<double, NP, true, TAB>(ty.d.AtA0, Atb64, S, eps, prm.eps_rel64,
+                                                      prm.max_iter, st, 0.0, nullptr, tab)) {
+            }
+        } while (TAB && polish_end<NP>(tab, Atb64, S, st, polished));
     }
     reload(s32);
     chi2 = fast_p2_chi2<NS, NP>(ty, s32, S.x) / prm.rho;

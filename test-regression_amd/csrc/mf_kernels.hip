@@ -799,6 +799,7 @@ void k_fast_p2q(const DevType *__restrict__ types, int type_id, Seg seg,
     // 0: the start is a prediction, 1: it fell back to x = 0, 2: it was the
     // empty set (the cold start: nothing to fall back to)
     int restart = 0;
+    uint32_t polished = 0;  // the set last solved by its factor (polish_end)
     float svn[10], xn[10];
     int fn = 0, itn = 0;
     QueueCursor q;
@@ -835,6 +836,7 @@ void k_fast_p2q(const DevType *__restrict__ types, int type_id, Seg seg,
             flags = fn & F_FP64_PIVOT;
             const bool begun = fast_p2_begin<N, N>(ty, svn, prm, xn, Atb, S, flags);
             restart = S.pm == 0 ? 2 : 0;
+            polished = 0;
             if (!begun) {
 #pragma unroll
                 for (int p = 0; p < 10; ++p) xo[(int64_t)p * nch + c] = 0.0f;
@@ -847,7 +849,8 @@ void k_fast_p2q(const DevType *__restrict__ types, int type_id, Seg seg,
         if (c >= 0 &&
             fnnls_step<double, N, true, true>(ty.d.AtA0, Atb, S, prm.eps * prm.rho, prm.eps_rel64,
                                               prm.max_iter, st, 0.0, nullptr, *opaque(&ty.tab)) &&
-            !(warm && st.hit_max && restart == 0)) {
+            !(warm && st.hit_max && restart == 0) &&
+            !polish_end<N>(*opaque(&ty.tab), Atb, S, st, polished)) {
             flags |= F_REFINED;
             if (warm && st.outer != outer0) flags |= F_WARM_REJECT;
             if (restart == 1) flags |= F_WARM_FALLBACK;
@@ -870,6 +873,7 @@ void k_fast_p2q(const DevType *__restrict__ types, int type_id, Seg seg,
         if (c >= 0 && warm && st.hit_max && restart == 0) {
             warm_fallback_reset(S, st);
             restart = 1;
+            polished = 0;
         }
         if (q.exhausted(nitems) && __ballot(c >= 0 || cn >= 0) == 0) break;
     }

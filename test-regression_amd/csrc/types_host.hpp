@@ -187,6 +187,33 @@ inline void build_mask_tab(const DevType &d, MaskTab &t)
                 }
         }
         row[MASK_PACK - 1] = ratio;
+        if (spd) {
+            // 1-norm condition number of the block, from the extended-precision inverse
+            long double nm = 0, ni = 0;
+            for (int a = 0; a < k; ++a) {
+                long double cm = 0, ci = 0;
+                for (int b2 = 0; b2 < k; ++b2) {
+                    cm += fabsl((long double)M[idx[a]][idx[b2]]);
+                    long double v = 0;
+                    for (int c = (a > b2 ? a : b2); c < k; ++c) v += W[c][a] * W[c][b2];
+                    ci += fabsl(v);
+                }
+                if (cm > nm) nm = cm;
+                if (ci > ni) ni = ci;
+            }
+            if (nm * ni > (long double)MASK_FACTOR_COND) {
+                // and the block's Cholesky factor: L_ij below the diagonal,
+                // 1 / L_ii on it (solve_factor)
+                t.factored[m >> 5] |= 1u << (m & 31);
+                double *f = t.fac[m];
+                for (int a = 0; a < k; ++a)
+                    for (int b2 = 0; b2 <= a; ++b2) {
+                        const int i = idx[a], j = idx[b2];
+                        f[i * (i + 1) / 2 + j] = a == b2 ? (double)(1.0L / C[a][a]) : (double)C[a][b2];
+                    }
+                f[MASK_PACK - 1] = row[MASK_PACK - 1];
+            }
+        }
     }
 }
 

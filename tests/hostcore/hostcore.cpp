@@ -153,16 +153,19 @@ static double eps_rel64 = 1e-15;
 // Row m of the passive-block inverse table of the type's x = 0 normal
 // matrix, as the kernels read it (MASK_PACK values: packed lower triangle,
 // pivot ratio in the last slot), and the type's AtA0 (100 values,
-// row-major).  Returns nonzero on a bad type.
+// row-major).  Where the set also has a factor row (an ill-conditioned
+// block), that row instead, and the return value is 2.  Returns 1 on a bad
+// type.
 extern "C" int hc_mask_inverse(const mf_chtype *type, int m, double *row, double *ata0)
 {
     DevType d;
     if (expand_type(*type, d)) return 1;
     std::vector<MaskTab> t(1);
     build_mask_tab(d, t[0]);
-    for (int e = 0; e < MASK_PACK; ++e) row[e] = t[0].inv[m][e];
+    const bool f = (t[0].factored[m >> 5] >> (m & 31)) & 1u;
+    for (int e = 0; e < MASK_PACK; ++e) row[e] = f ? t[0].fac[m][e] : t[0].inv[m][e];
     for (int i = 0; i < 100; ++i) ata0[i] = d.d.AtA0[i / 10][i % 10];
-    return 0;
+    return f ? 2 : 0;   // 2: the row is the set's factor row (fac)
 }
 
 // 1: the FP64 phase of the fast path reads the passive-block inverse tables

@@ -28,7 +28,11 @@ close its fit came to a different outcome (or_result.m_*, pivot):
     exactly on the first call's FP64 normal equations, has positive
     amplitudes and passes the stop test (within BAND_D rounding units of
     w); the oracle's fit continued from it (or_dofit_from_set) gives the
-    code's amplitudes, chi2 and Minimize count to the tolerance.
+    code's chi2 and Minimize count to the tolerance, and its amplitudes to
+    the tolerance or, where that set's passive block is ill-conditioned
+    (early-BX pulses that are nearly the same template tail), to BAND_X of
+    that block's rounding units: the optimum's chi2 is unique, its
+    minimizer is fixed only to the block's conditioning.
 A channel with none of these is a failure, and the numbers are capped: at
 most 0.5% of the batch outside tolerance, at most 0.1% explained only by a
 decision margin or another end point (at least 8 channels either way).
@@ -98,7 +102,13 @@ def account(x, chi2, ref, tol, *, tol_chi2=None, floor=1.0, iters=None, flags=No
         pm = ((x[:, cand] > 0) * (1 << np.arange(10))[:, None]).sum(axis=0)
         k = oracle_kkt_sets(ct, s, type_of, cand, pm, pederr=sets[3] if len(sets) > 3 else None)
         z = k["fit"]["x"].T
-        close = (np.abs(x[:, cand] - z) <= tol * np.maximum(np.abs(z), floor)).all(axis=0) & \
+        # amplitudes to the tolerance, or to the conditioning band of the
+        # continued fit's own passive blocks (as for the oracle's end point);
+        # chi2 always to the tolerance: the optimum's value is well defined
+        # however ill-conditioned its minimizer
+        band = BAND_X * _units(k["fit"]) * scale[cand]
+        close = (np.abs(x[:, cand] - z) <=
+                 np.maximum(tol * np.maximum(np.abs(z), floor), band)).all(axis=0) & \
             (np.abs(np.asarray(chi2, dtype=np.float64)[cand] - k["fit"]["chi2"]) <=
              tol_chi2 * np.maximum(k["fit"]["chi2"], 1.0))
         if iters is not None:

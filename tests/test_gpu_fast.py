@@ -8,6 +8,8 @@ record of its conditioning or of a decision taken within rounding, and at
 most 0.1% of channels may be outside.  (The fast path reaches the optimum by
 its own route, so its FNNLS counts are not compared.)
 """
+from pathlib import Path
+
 import numpy as np
 import pytest
 import torch
@@ -81,6 +83,34 @@ def test_fast_fit_seed_sweep(pkg, gpu, cfg, pulse_cov, n, seed):
     oracle, as above."""
     r, ref = _fit_fast(cfg, pulse_cov, n, seed=seed, warm_start=1)
     _check(r, ref, f"fast {cfg} {pulse_cov} warm seed+{seed}")
+
+
+def test_warm_fit_ill_conditioned_blocks(pkg, gpu):
+    """The three c2 channels of tests/golden/degenerate_channels.npz, whose
+    warm starts (the committed network's predictions) put both BX -5 and BX
+    -4 in the passive set (condition numbers up to 1.3e14): every chi2 is
+    the oracle's optimum (to the FP32 output's rounding), and every channel
+    is accounted for (the amplitudes of such a block are fixed only to its
+    conditioning; the oracle continued from the same set agrees to that)."""
+    from test_regression_amd import _abi, engine, warmstart, workloads
+    z = np.load(Path(__file__).resolve().parent / "golden" / "degenerate_channels.npz")
+    w = workloads.config("c2")
+    ct = w.chtypes()
+    s32 = z["samples"].astype(np.float32)
+    eng = engine.Engine(list(ct))
+    warmstart.ensure_trained(eng, path=warmstart.weights_for("c2"))
+    print("[gpu-fast] ill-conditioned warm starts", flush=True)
+    r = eng.fit(torch.from_numpy(s32).cuda(), None,
+                _abi.default_opts(precision=_abi.MF_PREC_FP32, warm_start=1))
+    torch.cuda.synchronize()
+    s = s32.astype(np.float64)
+    ref = oracle_fit(ct, s, None, oracle_opts())
+    ref_sets[id(ref)] = (ct, s, None)
+    chi2 = r.chi2.double().cpu().numpy()
+    assert np.all(np.abs(chi2 - ref["chi2"]) <= 1e-6 * np.maximum(ref["chi2"], 1.0)), chi2
+    rep = parity.account(r.x.double().cpu().numpy(), chi2, ref, TOL_X, tol_chi2=TOL_CHI2,
+                         iters=r.iters.cpu().numpy(), compare_outer=False, sets=(ct, s, None))
+    assert rep["unexplained"] == 0, rep
 
 
 def test_fast_fit_fp64_samples(pkg, gpu):
