@@ -2,7 +2,7 @@
 """Writes tests/golden/degenerate_channels.npz: three channels of the c2
 workload (seed 2; channel indices 11962371, 22806810 and 48707875 of its
 stream, made by the library's own generator) whose warm-started fits reach
-passive sets holding both BX -5 and BX -4.  Those two pulses are nearly the
+passive sets holding BX -5 to BX -1 together.  Those pulses are nearly the
 same template tail, so the block is ill-conditioned (condition numbers up to
 1.3e14) and a product with its rounded inverse alone misses the block's
 least-squares optimum (chi2 0.0017 instead of 4.5e-12 on the first channel).

@@ -85,10 +85,12 @@ def test_fast_fit_seed_sweep(pkg, gpu, cfg, pulse_cov, n, seed):
     _check(r, ref, f"fast {cfg} {pulse_cov} warm seed+{seed}")
 
 
-def test_warm_fit_ill_conditioned_blocks(pkg, gpu):
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_warm_fit_ill_conditioned_blocks(pkg, gpu, persistent):
     """The three c2 channels of tests/golden/degenerate_channels.npz, whose
-    warm starts (the committed network's predictions) put both BX -5 and BX
-    -4 in the passive set (condition numbers up to 1.3e14): every chi2 is
+    warm starts (the committed network's predictions) put BX -5 to BX -1
+    together in the passive set (condition numbers up to 1.3e14), through
+    the one-launch kernel and the persistent one: every chi2 is
     the oracle's optimum (to the FP32 output's rounding), and every channel
     is accounted for (the amplitudes of such a block are fixed only to its
     conditioning; the oracle continued from the same set agrees to that)."""
@@ -99,9 +101,10 @@ def test_warm_fit_ill_conditioned_blocks(pkg, gpu):
     s32 = z["samples"].astype(np.float32)
     eng = engine.Engine(list(ct))
     warmstart.ensure_trained(eng, path=warmstart.weights_for("c2"))
-    print("[gpu-fast] ill-conditioned warm starts", flush=True)
+    print(f"[gpu-fast] ill-conditioned warm starts, persistent={persistent}", flush=True)
     r = eng.fit(torch.from_numpy(s32).cuda(), None,
-                _abi.default_opts(precision=_abi.MF_PREC_FP32, warm_start=1))
+                _abi.default_opts(precision=_abi.MF_PREC_FP32, warm_start=1,
+                                  persistent=persistent))
     torch.cuda.synchronize()
     s = s32.astype(np.float64)
     ref = oracle_fit(ct, s, None, oracle_opts())

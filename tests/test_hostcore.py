@@ -313,7 +313,7 @@ def test_lazy_split_is_bit_identical(pkg, cfg, max_iter):
 
 def test_ill_conditioned_blocks_reach_the_optimum(pkg):
     """Three c2 channels (tests/golden/degenerate_channels.npz) whose fits
-    reach passive sets holding both BX -5 and BX -4, started from the full
+    reach passive sets holding BX -5 to BX -1 together, started from the full
     passive set (every pulse; the block's condition number is 1.3e14).  A
     product with the block's rounded inverse alone ends the first channel at
     chi2 0.0017 where the optimum is 4.5e-12; with the refinement step of
