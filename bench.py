@@ -45,7 +45,7 @@ PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
 # factors that make the counters read a 256 MiB device copy as 256 MiB each
 # way.  Other workloads report null.
 MEASURED_TRAFFIC = {("c2", "zero", 1_000_000): {
-    "bytes": 94.3e6, "how": "rocprofv3 --pmc FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00), "
+    "bytes": 94.5e6, "how": "rocprofv3 --pmc FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00), "
                              "bench.py --traffic-probe, copy-calibrated"}}
 
 
